@@ -1,0 +1,106 @@
+/*
+ * rls_hip.h -- C ABI of the MI355X recycled-subspace sigma_min(C - zI) solver.
+ *
+ * Drop-in boundary for RLS's psTriRecy (src/core/psTriRecy.jl:889-913).  A host shim
+ * (Julia `ccall`, Python ctypes) keeps the reference's entry points and control structs and
+ * forwards here.  Plain C types only: pointers, sizes, doubles; complex numbers are
+ * interleaved (re, im) doubles, the layout of Julia's ComplexF64 and C's double _Complex.
+ * Matrices are column-major (Julia's layout).
+ *
+ * Every function returns 0 on success or a negative RLS_E* code; the message of the last
+ * failure on a context is returned by rls_last_error().  One context per host thread.
+ */
+#ifndef RLS_HIP_H
+#define RLS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RLS_OK 0
+#define RLS_E_ARG -1      /* argument / dimension error  (reference: ArgumentError, DimensionMismatch) */
+#define RLS_E_HIP -2      /* HIP runtime error (allocation, launch)                                    */
+#define RLS_E_STATE -3    /* call order error, e.g. solve before an operator was set                   */
+#define RLS_E_NUMERIC -4  /* numerical breakdown (reference: LAPACKException / PosDefException)        */
+
+typedef struct rls_ctx rls_ctx;
+
+/* Solver knobs of PsTriRecyControl (src/core/psTriRecy.jl:350-370), same names and meaning.
+ * Grid-derived quantities the reference reads from `rec::Recursion` are passed alongside:
+ * rec_k (the zigzag column height, psTriRecy.jl:388-393) and rec_dx / rec_dy (the grid
+ * steps used by the default recycle_fail_dist_thresh, src/core/recySpace.jl:22-38). */
+typedef struct rls_tri_control {
+    double tol;                       /* 1e-3 */
+    double normC;                     /* 1.0 */
+    double shift;                     /* normC^2 * eps * 100 */
+    double gap_tau;                   /* 1e-10 */
+    double gap_tau_scale;             /* 0.01 */
+    double recy_tau;                  /* 1.5 */
+    double recycle_fail_dist_thresh;  /* NaN -> 1.01 * max(rec_dx, rec_dy) */
+    int32_t r;                        /* 6 */
+    int32_t p;                        /* 50 */
+    int32_t p_min;                    /* 1 */
+    double lobpcg_tol_scale;          /* 0.1 */
+    char lobpcg_side;                 /* 'L' */
+    int32_t implict_update;           /* 0 (rotations of J3 are exact either way, S = I) */
+    int32_t implict_recycle;          /* 1 (the only mode of this library) */
+    int32_t collect_timing;           /* 1 */
+    int32_t maxit;                    /* LOBPCG maxit, 2000 (src/core/lobpcgSvd.jl:24) */
+    int32_t rec_k;
+    double rec_dx, rec_dy;
+    /* --- MI355X execution knobs (no counterpart in the reference) --- */
+    int32_t mode;                     /* RLS_MODE_PLAIN_FP64 or RLS_MODE_BATCHED */
+    int32_t window;                   /* max shifts per speculative window (batched mode), 0 = auto */
+    uint64_t seed;                    /* seed of the initial block (reference: rand!(X)) */
+} rls_tri_control;
+
+#define RLS_MODE_PLAIN_FP64 0   /* one shift at a time, FP64 everywhere                        */
+#define RLS_MODE_BATCHED 1      /* speculative windows of shifts sharing one recycle snapshot  */
+
+/* Per-point statistics of PsTriRecyStats (src/core/psTriRecy.jl:257-292).  Array pointers
+ * may be NULL; non-NULL arrays hold nZ entries. */
+typedef struct rls_tri_stats {
+    int64_t* p_count;
+    int64_t* r_recy;
+    double* pts;                      /* complex, nZ entries (first n_pts used) */
+    int64_t* M_vec_apps;
+    int64_t* M_vec_apps_no_recycle;
+    int64_t* R_vec_apps;
+    int64_t* lobpcg_it;               /* extra: LOBPCG iterations per point (0 = recycled) */
+    uint64_t preprocess_time_ns;
+    uint64_t recycle_time_ns;
+    uint64_t qr_time_ns;
+    uint64_t refine_time_ns;
+    uint64_t update_time_ns;
+    int64_t n_it;
+    int64_t n_pts;
+    int64_t n_windows;                /* extra: speculative windows launched (batched mode) */
+    int64_t n_window_shifts;          /* extra: shifts evaluated inside windows          */
+} rls_tri_stats;
+
+/* Context lifetime.  `device` is the HIP device ordinal. */
+int rls_ctx_create(int device, rls_ctx** out);
+int rls_ctx_destroy(rls_ctx* ctx);
+const char* rls_last_error(const rls_ctx* ctx);
+
+/* Fill `c` with PsTriRecyControl's defaults for a given normC (psTriRecy.jl:350-370). */
+void rls_tri_control_default(rls_tri_control* c, double normC);
+
+/* Dense triangular operator: the upper-triangular Schur factor T of C = Z T Z^H, computed
+ * by the host (the reference's pre_process, src/core/psTriRecy.jl:6-14).  Replaces the
+ * `M = UpperTriangular(F.T), S = I` pair; T (n x n, leading dimension ld >= n, complex
+ * column-major) is copied to device memory. */
+int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld);
+
+/* Run psTriRecy over the nZ shifts `Z` (complex, traversal order, i.e. rec.Z) and write
+ * sigma_min to `sigma_out` (nZ doubles, NaN where unset).  Replaces _psTriRecy_impl
+ * (src/core/psTriRecy.jl:708-881).  `stats` may be NULL. */
+int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_control* control,
+                  double* sigma_out, rls_tri_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RLS_HIP_H */

@@ -1,0 +1,336 @@
+"""MI355X recycled-subspace pseudospectrum solver: the psTriRecy path of RLS on gfx950.
+
+Host-side mirror of the reference's Julia API (same names, fields, defaults, argument
+meaning and errors) above the C ABI of ``librls_hip.so`` (include/rls_hip.h):
+
+    rec = recursion_grid(x_st, x_ed, y_st, y_ed, n_x, n_y, k)      # src/core/recuGrid.jl:68-73
+    sigma_min, stats = psTriRecy(C, rec, control=PsTriRecyControl(tol=1e-4, normC=...))
+    grid = reorder_to_natural_grid(rec, sigma_min)                  # src/core/recuGrid.jl:229-232
+
+The compute path is the HIP library only; importing works without a GPU, but solving
+raises if the library or a device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import sys
+import time
+from dataclasses import dataclass, field, fields
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+
+__all__ = [
+    "Recursion", "recursion_grid", "recursion_grid_zigzag", "recursion_natural_permutation",
+    "reorder_to_natural", "reorder_to_natural_grid", "PsTriRecyControl", "PsTriRecyStats",
+    "psTriRecy", "schur_preprocess", "TriSolver",
+]
+
+EPS = float(np.finfo(np.float64).eps)
+
+
+# ----------------------------------------------------------------------------------------
+# Recursion grid (src/core/recuGrid.jl)
+# ----------------------------------------------------------------------------------------
+@dataclass
+class Recursion:
+    """Grid metadata and traversal order (src/core/recuGrid.jl:1-38)."""
+    x_st: float
+    x_ed: float
+    y_st: float
+    y_ed: float
+    n_x: int
+    n_y: int
+    dx: float
+    dy: float
+    k: int
+    Z: np.ndarray
+
+    @classmethod
+    def make(cls, x_st, x_ed, y_st, y_ed, n_x, n_y, k) -> "Recursion":
+        n_x, n_y, k = int(n_x), int(n_y), int(k)
+        if n_x < 2:
+            raise ValueError("n_x must be >= 2")
+        if n_y < 2:
+            raise ValueError("n_y must be >= 2")
+        if k < 1:
+            raise ValueError("k must be >= 1")
+        dx = (float(x_ed) - float(x_st)) / (n_x - 1)
+        dy = (float(y_ed) - float(y_st)) / (n_y - 1)
+        return cls(float(x_st), float(x_ed), float(y_st), float(y_ed), n_x, n_y, dx, dy, k,
+                   np.zeros(n_x * n_y, dtype=np.complex128))
+
+
+def recursion_grid_zigzag(x_st, x_ed, y_st, y_ed, n_x, n_y, k) -> Recursion:
+    """Block zigzag: block rows of k points, x direction alternating per block row, y
+    direction alternating per column (src/core/recuGrid.jl:157-176, 48-60)."""
+    rec = Recursion.make(x_st, x_ed, y_st, y_ed, n_x, n_y, k)
+    if rec.n_y % rec.k != 0:
+        raise ValueError("n_y must be a multiple of k")
+    nb = rec.n_y // rec.k
+    out = []
+    ly = np.arange(rec.k)
+    for by in range(nb):
+        xs = np.arange(rec.n_x) if by % 2 == 0 else np.arange(rec.n_x - 1, -1, -1)
+        for j, ix in enumerate(xs):
+            iy = by * rec.k + (ly if j % 2 == 0 else ly[::-1])
+            out.append((rec.x_st + ix * rec.dx) + 1j * (rec.y_st + iy * rec.dy))
+    rec.Z[:] = np.concatenate(out)
+    return rec
+
+
+def recursion_grid(x_st, x_ed, y_st, y_ed, n_x, n_y, k) -> Recursion:
+    """Default traversal (src/core/recuGrid.jl:68-73)."""
+    return recursion_grid_zigzag(x_st, x_ed, y_st, y_ed, n_x, n_y, k)
+
+
+def recursion_natural_permutation(rec: Recursion) -> np.ndarray:
+    """0-based permutation with rec.Z[perm] in row-major natural order
+    (src/core/recuGrid.jl:179-208)."""
+    ix = np.rint((rec.Z.real - rec.x_st) / rec.dx).astype(np.int64)
+    iy = np.rint((rec.Z.imag - rec.y_st) / rec.dy).astype(np.int64)
+    if np.any((ix < 0) | (ix >= rec.n_x) | (iy < 0) | (iy >= rec.n_y)):
+        raise ValueError("point is outside the recursion grid bounds")
+    j = iy * rec.n_x + ix
+    perm = np.full(rec.n_x * rec.n_y, -1, dtype=np.int64)
+    if len(np.unique(j)) != len(j):
+        raise ValueError("duplicate grid point detected")
+    perm[j] = np.arange(len(j))
+    if np.any(perm < 0):
+        raise ValueError("missing grid point")
+    return perm
+
+
+def reorder_to_natural(rec: Recursion, v) -> np.ndarray:
+    """src/core/recuGrid.jl:216-221."""
+    v = np.asarray(v)
+    if v.shape[0] != rec.n_x * rec.n_y:
+        raise ValueError(f"v length {v.shape[0]} does not match grid size {rec.n_x * rec.n_y}")
+    return v[recursion_natural_permutation(rec)]
+
+
+def reorder_to_natural_grid(rec: Recursion, v) -> np.ndarray:
+    """n_y x n_x array with A[iy, ix] at (x[ix], y[iy]) (src/core/recuGrid.jl:229-232)."""
+    return reorder_to_natural(rec, v).reshape(rec.n_y, rec.n_x)
+
+
+# ----------------------------------------------------------------------------------------
+# Control / stats (src/core/psTriRecy.jl:257-304, 350-370)
+# ----------------------------------------------------------------------------------------
+@dataclass
+class PsTriRecyControl:
+    """Same fields and defaults as the reference's PsTriRecyControl; ``shift`` defaults to
+    normC^2 * eps * 100 evaluated with the given normC (Base.@kwdef semantics).
+
+    Execution knobs of the GPU build (no reference counterpart): ``mode`` ("plain" = one
+    shift at a time, "batched" = speculative windows over one recycle snapshot),
+    ``window`` (max shifts per window, 0 = auto), ``seed`` (initial block), ``maxit``."""
+    tol: float = 1e-3
+    normC: float = 1.0
+    shift: Optional[float] = None
+    gap_tau: float = 1e-10
+    gap_tau_scale: float = 0.01
+    recy_tau: float = 1.5
+    recycle_fail_dist_thresh: float = float("nan")
+    r: int = 6
+    p: int = 50
+    p_min: int = 1
+    lobpcg_tol_scale: float = 0.1
+    lobpcg_side: str = "L"
+    implict_update: bool = False
+    implict_recycle: bool = True
+    collect_timing: bool = True
+    inner_verbose_override: Optional[int] = None
+    mode: str = "batched"
+    window: int = 0
+    seed: int = 0
+    maxit: int = 2000
+
+    def __post_init__(self):
+        if self.shift is None:
+            self.shift = self.normC ** 2 * EPS * 100
+        if self.lobpcg_side not in ("L", "S"):
+            raise ValueError("lobpcg_side must be 'L' or 'S'")
+        if self.mode not in ("plain", "batched"):
+            raise ValueError("mode must be 'plain' or 'batched'")
+
+
+@dataclass
+class PsTriRecyStats:
+    """src/core/psTriRecy.jl:257-292 (+ lobpcg_it, n_windows, n_window_shifts)."""
+    p_count: np.ndarray
+    r_recy: np.ndarray
+    pts: np.ndarray
+    M_vec_apps: np.ndarray
+    M_vec_apps_no_recycle: np.ndarray
+    R_vec_apps: np.ndarray
+    preprocess_time_ns: int = 0
+    recycle_time_ns: int = 0
+    qr_time_ns: int = 0
+    refine_time_ns: int = 0
+    update_time_ns: int = 0
+    mv_stats_available: bool = True
+    n_it: int = 0
+    n_pts: int = 0
+    lobpcg_it: np.ndarray = None
+    n_windows: int = 0
+    n_window_shifts: int = 0
+
+    # legacy lowercase aliases (psTriRecy.jl:295-304)
+    @property
+    def m_vec_apps(self):
+        return self.M_vec_apps
+
+    @property
+    def m_vec_apps_no_recycle(self):
+        return self.M_vec_apps_no_recycle
+
+    @property
+    def r_vec_apps(self):
+        return self.R_vec_apps
+
+
+# ----------------------------------------------------------------------------------------
+# Preprocessing and solver
+# ----------------------------------------------------------------------------------------
+def schur_preprocess(C) -> np.ndarray:
+    """Upper-triangular Schur factor T of C = Z T Z^H (pre_process, psTriRecy.jl:6-14).
+    Sparse input is densified: sigma_min(C - zI) is invariant under the unitary change of
+    basis, so the Schur path serves the sparse overload (psTriRecy.jl:47-61) too."""
+    import scipy.linalg as sla
+    import scipy.sparse as sp
+    if sp.issparse(C):
+        C = C.toarray()
+    C = np.asarray(C)
+    if C.ndim != 2 or C.shape[0] != C.shape[1]:
+        raise ValueError(f"pre_process only supports square matrices, got size {C.shape}")
+    T, _ = sla.schur(C.astype(np.complex128), output="complex")
+    return np.asfortranarray(T)
+
+
+_CTX = {}
+
+
+def _context(device: int = 0) -> _capi.Context:
+    ctx = _CTX.get(device)
+    if ctx is None:
+        ctx = _capi.Context(device)
+        _CTX[device] = ctx
+    return ctx
+
+
+def _c_control(control: PsTriRecyControl, rec: Recursion) -> _capi.TriControl:
+    c = _capi.TriControl()
+    c.tol = control.tol
+    c.normC = control.normC
+    c.shift = float(control.shift)
+    c.gap_tau = control.gap_tau
+    c.gap_tau_scale = control.gap_tau_scale
+    c.recy_tau = control.recy_tau
+    c.recycle_fail_dist_thresh = control.recycle_fail_dist_thresh
+    c.r = control.r
+    c.p = control.p
+    c.p_min = control.p_min
+    c.lobpcg_tol_scale = control.lobpcg_tol_scale
+    c.lobpcg_side = control.lobpcg_side.encode()
+    c.implict_update = int(control.implict_update)
+    c.implict_recycle = int(control.implict_recycle)
+    c.collect_timing = int(control.collect_timing)
+    c.maxit = control.maxit
+    c.rec_k = rec.k
+    c.rec_dx = rec.dx
+    c.rec_dy = rec.dy
+    c.mode = _capi.RLS_MODE_BATCHED if control.mode == "batched" else _capi.RLS_MODE_PLAIN_FP64
+    c.window = control.window
+    c.seed = control.seed
+    return c
+
+
+class TriSolver:
+    """A device-resident Schur factor; solve() may be called repeatedly (the bench times it)."""
+
+    def __init__(self, T: np.ndarray, device: int = 0):
+        self.ctx = _context(device)
+        T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
+        n = T.shape[0]
+        self.n = n
+        self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, n, n))
+
+    def solve(self, Z: np.ndarray, control: PsTriRecyControl, rec: Recursion):
+        Z = np.ascontiguousarray(Z, dtype=np.complex128)
+        nZ = len(Z)
+        sigma = np.full(nZ, np.nan)
+        arrs = {k: np.zeros(nZ, dtype=np.int64) for k in
+                ("p_count", "r_recy", "M_vec_apps", "M_vec_apps_no_recycle", "R_vec_apps", "lobpcg_it")}
+        pts = np.zeros(nZ, dtype=np.complex128)
+        st = _capi.TriStats()
+        for k, a in arrs.items():
+            setattr(st, k, a.ctypes.data_as(C.POINTER(C.c_int64)))
+        st.pts = pts.ctypes.data_as(C.POINTER(C.c_double))
+        cc = _c_control(control, rec)
+        self.ctx.check(self.ctx.lib.rls_tri_solve(self.ctx.h, Z.ctypes.data, nZ, C.byref(cc),
+                                                  sigma.ctypes.data, C.byref(st)))
+        stats = PsTriRecyStats(arrs["p_count"], arrs["r_recy"], pts, arrs["M_vec_apps"],
+                               arrs["M_vec_apps_no_recycle"], arrs["R_vec_apps"],
+                               recycle_time_ns=st.recycle_time_ns, qr_time_ns=st.qr_time_ns,
+                               refine_time_ns=st.refine_time_ns, update_time_ns=st.update_time_ns,
+                               n_it=st.n_it, n_pts=st.n_pts, lobpcg_it=arrs["lobpcg_it"],
+                               n_windows=st.n_windows, n_window_shifts=st.n_window_shifts)
+        return sigma, stats
+
+
+def _print_startup(io, rec: Recursion, control: PsTriRecyControl):
+    """Startup dump (psTriRecy.jl:554-577)."""
+    io.write("[psTriRecy] rec: x_range=[%.6f, %.6f], y_range=[%.6f, %.6f], n_x=%d, n_y=%d, k=%d, "
+             "n_pts=%d\n" % (rec.x_st, rec.x_ed, rec.y_st, rec.y_ed, rec.n_x, rec.n_y, rec.k,
+                            rec.n_x * rec.n_y))
+    io.write("[psTriRecy] control:\n")
+    for f in fields(control):
+        io.write(f"  {f.name} = {getattr(control, f.name)!r}\n")
+
+
+def _print_steps(io, rec, sigma, stats):
+    """Per-point summary lines (psTriRecy.jl:323-348), written after the solve."""
+    nZ = len(rec.Z)
+    for i in range(nZ):
+        it = int(stats.lobpcg_it[i])
+        state = f"refine({it})" if it > 0 else "recycled"
+        z = rec.Z[i]
+        io.write("[%4d/%4d] z=%9.4f%+9.4fim | main: state=%-12s p=%3d r_svdrr=%3d σ=%9.3e\n"
+                 % (i + 1, nZ, z.real, z.imag, state, stats.p_count[i], stats.r_recy[i], sigma[i]))
+
+
+def psTriRecy(C, rec: Recursion, control: PsTriRecyControl = None, verbose: int = 1,
+              io=None, device: int = 0):
+    """sigma_min(C - zI) over rec.Z with triangular preprocessing and recycled subspaces
+    (src/core/psTriRecy.jl:889-913).  Returns (sigma_min, stats); sigma_min is in rec.Z
+    order, NaN where unset."""
+    control = control or PsTriRecyControl()
+    io = io or sys.stdout
+    if verbose >= 1:
+        _print_startup(io, rec, control)
+    t0 = time.perf_counter_ns()
+    T = schur_preprocess(C)
+    t_pre = time.perf_counter_ns() - t0
+    solver = TriSolver(T, device)
+    sigma, stats = solver.solve(rec.Z, control, rec)
+    stats.preprocess_time_ns = int(t_pre)
+    if verbose >= 2:
+        _print_steps(io, rec, sigma, stats)
+    if verbose >= 1:
+        io.write(f"Total M vec apps for singular value problem: {int(stats.M_vec_apps.sum())}\n")
+        if control.collect_timing:
+            io.write("Total preprocess time for singular value problem: %.6f seconds\n" % (t_pre / 1e9))
+            io.write("Total recycle time for singular value problem: %.6f seconds\n"
+                     % (stats.recycle_time_ns / 1e9))
+            io.write("Total refine time for singular value problem: %.6f seconds\n"
+                     % (stats.refine_time_ns / 1e9))
+            io.write("Total update time for singular value problem: %.6f seconds\n"
+                     % (stats.update_time_ns / 1e9))
+        io.write(f"Total R vec apps for singular value problem: {int(stats.R_vec_apps.sum())}\n")
+        io.write(f"Total it for singular value problem: {stats.n_it}\n")
+        io.write(f"Total pt for singular value problem: {stats.n_pts}\n")
+    return sigma, stats

@@ -1,0 +1,136 @@
+"""ctypes binding of librls_hip.so (declarations in include/rls_hip.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librls_hip.so")
+
+RLS_OK = 0
+RLS_MODE_PLAIN_FP64 = 0
+RLS_MODE_BATCHED = 1
+
+EXPORTED = [
+    "rls_ctx_create",
+    "rls_ctx_destroy",
+    "rls_last_error",
+    "rls_tri_control_default",
+    "rls_set_tri_dense",
+    "rls_tri_solve",
+]
+
+
+class TriControl(C.Structure):
+    """Mirror of ``rls_tri_control`` (PsTriRecyControl, src/core/psTriRecy.jl:350-370)."""
+    _fields_ = [
+        ("tol", C.c_double),
+        ("normC", C.c_double),
+        ("shift", C.c_double),
+        ("gap_tau", C.c_double),
+        ("gap_tau_scale", C.c_double),
+        ("recy_tau", C.c_double),
+        ("recycle_fail_dist_thresh", C.c_double),
+        ("r", C.c_int32),
+        ("p", C.c_int32),
+        ("p_min", C.c_int32),
+        ("lobpcg_tol_scale", C.c_double),
+        ("lobpcg_side", C.c_char),
+        ("implict_update", C.c_int32),
+        ("implict_recycle", C.c_int32),
+        ("collect_timing", C.c_int32),
+        ("maxit", C.c_int32),
+        ("rec_k", C.c_int32),
+        ("rec_dx", C.c_double),
+        ("rec_dy", C.c_double),
+        ("mode", C.c_int32),
+        ("window", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class TriStats(C.Structure):
+    """Mirror of ``rls_tri_stats`` (PsTriRecyStats, src/core/psTriRecy.jl:257-292)."""
+    _fields_ = [
+        ("p_count", C.POINTER(C.c_int64)),
+        ("r_recy", C.POINTER(C.c_int64)),
+        ("pts", C.POINTER(C.c_double)),
+        ("M_vec_apps", C.POINTER(C.c_int64)),
+        ("M_vec_apps_no_recycle", C.POINTER(C.c_int64)),
+        ("R_vec_apps", C.POINTER(C.c_int64)),
+        ("lobpcg_it", C.POINTER(C.c_int64)),
+        ("preprocess_time_ns", C.c_uint64),
+        ("recycle_time_ns", C.c_uint64),
+        ("qr_time_ns", C.c_uint64),
+        ("refine_time_ns", C.c_uint64),
+        ("update_time_ns", C.c_uint64),
+        ("n_it", C.c_int64),
+        ("n_pts", C.c_int64),
+        ("n_windows", C.c_int64),
+        ("n_window_shifts", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load librls_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rls_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.rls_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.rls_ctx_create.restype = C.c_int
+    lib.rls_ctx_destroy.argtypes = [vp]
+    lib.rls_ctx_destroy.restype = C.c_int
+    lib.rls_last_error.argtypes = [vp]
+    lib.rls_last_error.restype = C.c_char_p
+    lib.rls_tri_control_default.argtypes = [C.POINTER(TriControl), C.c_double]
+    lib.rls_tri_control_default.restype = None
+    lib.rls_set_tri_dense.argtypes = [vp, vp, C.c_int64, C.c_int64]
+    lib.rls_set_tri_dense.restype = C.c_int
+    lib.rls_tri_solve.argtypes = [vp, vp, C.c_int64, C.POINTER(TriControl), vp, C.POINTER(TriStats)]
+    lib.rls_tri_solve.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class RlsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"librls_hip error {code}: {msg}")
+        self.code = code
+
+
+class Context:
+    """One device context (rls_ctx); not shared between threads."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.rls_ctx_create(int(device), C.byref(h))
+        if rc != RLS_OK:
+            raise RlsError(rc, f"rls_ctx_create(device={device}) failed (no HIP device?)")
+        self.h = h
+
+    def check(self, rc: int):
+        if rc != RLS_OK:
+            raise RlsError(rc, self.lib.rls_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rls_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

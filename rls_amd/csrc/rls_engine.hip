@@ -1,0 +1,740 @@
+// Host driver of the recycled-subspace sigma_min(C - zI) solver and its C ABI.
+//
+// The control flow follows _psTriRecy_impl (src/core/psTriRecy.jl:708-881) point by point:
+// recycled Rayleigh-Ritz (fastSvdRaylRitz_implicit!, fastRaylRitz.jl:99-248) -> residual
+// test (_residual_norm_ok, psTriRecy.jl:237-245) -> refine by block LOBPCG on
+// (T - zI)^{-H} (lobpcgSvd, lobpcgSvd.jl:365-477) -> updateRecycledSpace! (recySpace.jl:160-453).
+// All n-length data (T, V, J1, J2, J5, the LOBPCG blocks) live in device memory; the host
+// only reads back the scalars that steer the control flow.
+//
+// Layout with S = I (Schur path, psTriRecy.jl:10-14): J6 = S V = V, J3 = S^H J5 = J5 and
+// J4 = V (orth_S), so the six reference caches are four device arrays V, J1 = T^H T V,
+// J2 = T^H V, J5 = T V, each n x (p_max r), column-major, block j in columns [j r, (j+1) r).
+//
+// RLS_MODE_BATCHED evaluates a window of consecutive shifts against the same recycle-space
+// snapshot in one batched pass.  The reference only changes the recycle space at refined
+// points, so every shift before the first failing one is exactly the sequential result;
+// the first failure is refined, the space updated, and the next window starts after it.
+#include "../../include/rls_hip.h"
+#include "rls_common.h"
+#include "rls_kernels.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <vector>
+
+namespace {
+
+using cplx = std::complex<double>;
+constexpr double EPS64 = 2.220446049250313e-16;
+
+template <typename T>
+struct DArr {
+    T* p = nullptr;
+    size_t n = 0;
+    void ensure(size_t cnt) {
+        if (cnt <= n) return;
+        if (p) HIP_TRY(hipFree(p));
+        p = nullptr;
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(T)));
+        n = cnt;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct rls_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    // operator
+    int64_t n = 0;
+    DArr<zd> T;
+    // recycle space
+    int r = 0, pmax = 0, p = 0, prm = 0;
+    int64_t ld = 0;
+    DArr<zd> V, J1, J2, J5, L, H1, H2;
+    std::vector<cplx> pts;
+    // LOBPCG blocks
+    DArr<zd> S, CS, S2, CS2, Wt, Dinv, trsm_tmp;
+    // small / reduction workspaces
+    DArr<zd> part, gram, Gs, Rm, Rinv, Racc, Rtmp, Wtot, Wk, Wneg, U;
+    DArr<double> sig, lam, offI, nrm2;
+    DArr<int> info;
+    // recycle RR workspaces
+    DArr<zd> Hz, heevW, Gev, C1, CzX, Gc, V2, CW, CG, Wr, Xr, zdev;
+    DArr<double> lamev, sigr, tausd;
+    DArr<int> rr, coff;
+    // last X (Ritz vectors) of the final state, for callers
+    std::vector<cplx> lastX;
+
+    ~rls_ctx() {
+        DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp,
+                          &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
+                          &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev};
+        for (auto* a : zs) a->release();
+        DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd};
+        for (auto* a : ds) a->release();
+        info.release();
+        rr.release();
+        coff.release();
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void sync() { HIP_TRY(hipStreamSynchronize(st)); }
+    template <typename T_>
+    void d2h(T_* h, const T_* d, size_t cnt) {
+        HIP_TRY(hipMemcpyAsync(h, d, cnt * sizeof(T_), hipMemcpyDeviceToHost, st));
+        sync();
+    }
+    template <typename T_>
+    void h2d(T_* d, const T_* h, size_t cnt) {
+        HIP_TRY(hipMemcpyAsync(d, h, cnt * sizeof(T_), hipMemcpyHostToDevice, st));
+    }
+
+    // ---------------- primitive operations ----------------
+    void gram_op(const zd* A, int64_t lda, int ka, const zd* B, int64_t ldb, int kb, zd* C, int64_t ldc) {
+        part.ensure(gram_partial_elems(n, ka, kb));
+        launch_gram(st, n, A, lda, ka, B, ldb, kb, C, ldc, part.p);
+    }
+
+    // Y = beta Y + A B  (A: n x K)
+    void gemm1(const zd* A, int64_t lda, int K, const zd* B, int64_t ldb, int k, zd* Y, int64_t ldy,
+               double beta) {
+        TallOperands ops{};
+        ops.A[0] = A; ops.lda[0] = lda; ops.K[0] = K; ops.nops = 1;
+        launch_tall_gemm(st, n, ops, B, ldb, 0, k, Y, ldy, 0, beta, 1);
+    }
+
+    // Cholesky-QR passes in place on X (n x k), accumulating R = R_m ... R_1 into Racc
+    // (k x k, ld k).  Passes stop once ||X^H X - I||_F < 2 sqrt(k) eps (the criterion of
+    // _rls_qr_shiftchol!, RLSLinearAlgebra.jl:747, 775-782), at most 3, decided on device.
+    void tall_qr(zd* X, int64_t ldx, int k, bool want_r) {
+        gram.ensure((size_t)k * k);
+        Rm.ensure((size_t)k * k);
+        Rinv.ensure((size_t)k * k);
+        Racc.ensure((size_t)k * k);
+        Rtmp.ensure((size_t)k * k);
+        offI.ensure(1);
+        info.ensure(1);
+        const double shift_scale = 11.0 * ((double)n * k + (double)k * (k + 1));
+        const double conv_tol = 2.0 * std::sqrt((double)k) * EPS64;
+        for (int pass = 0; pass < 3; ++pass) {
+            gram_op(X, ldx, k, X, ldx, k, gram.p, k);
+            launch_chol_rinv_cond(pass == 0 ? -1.0 : conv_tol, shift_scale, k);
+            gemm1(X, ldx, k, Rinv.p, k, k, X, ldx, 0.0);
+            if (want_r) {
+                if (pass == 0) {
+                    launch_copy2d(st, k, k, Rm.p, k, Racc.p, k);
+                } else {
+                    launch_small_gemm(st, k, k, k, Rm.p, k, Racc.p, k, Rtmp.p, k, 0.0);
+                    launch_copy2d(st, k, k, Rtmp.p, k, Racc.p, k);
+                }
+            }
+        }
+    }
+    void launch_chol_rinv_cond(double skip_tol, double shift_scale, int k);
+
+    // rls_orth!(V, X): two passes of projection + QR (RLSLinearAlgebra.jl:1127-1167)
+    void orth(const zd* Vb, int64_t ldv, int kV, zd* X, int64_t ldx, int kX) {
+        Wk.ensure((size_t)kV * kX);
+        Wneg.ensure((size_t)kV * kX);
+        for (int pass = 0; pass < 2; ++pass) {
+            gram_op(Vb, ldv, kV, X, ldx, kX, Wk.p, kV);
+            launch_neg2d(st, kV, kX, Wk.p, kV, Wneg.p, kV);
+            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+            tall_qr(X, ldx, kX, false);
+        }
+    }
+
+    // rls_orth_robust!(V, X, W, R) with two passes (RLSLinearAlgebra.jl:1225-1291):
+    // X_in = V Wtot + X_out Rtot.  Wtot -> Wout (kV x kX, ld ldw), Rtot -> Rout (kX x kX, ld ldr)
+    void orth_robust(const zd* Vb, int64_t ldv, int kV, zd* X, int64_t ldx, int kX, zd* Wout,
+                     int64_t ldw, zd* Rout, int64_t ldr) {
+        Wk.ensure((size_t)kV * kX);
+        Wneg.ensure((size_t)kV * kX);
+        Wtot.ensure((size_t)kV * kX);
+        DArr<zd>& R1 = Rtmp;  // reuse after tall_qr copies
+        std::vector<zd> dummy;
+        // pass 1
+        if (kV > 0) {
+            gram_op(Vb, ldv, kV, X, ldx, kX, Wtot.p, kV);
+            launch_neg2d(st, kV, kX, Wtot.p, kV, Wneg.p, kV);
+            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+        }
+        tall_qr(X, ldx, kX, true);
+        // R1 <- Racc
+        R1save.ensure((size_t)kX * kX);
+        launch_copy2d(st, kX, kX, Racc.p, kX, R1save.p, kX);
+        // pass 2
+        if (kV > 0) {
+            gram_op(Vb, ldv, kV, X, ldx, kX, Wk.p, kV);
+            launch_neg2d(st, kV, kX, Wk.p, kV, Wneg.p, kV);
+            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+        }
+        tall_qr(X, ldx, kX, true);
+        // Wtot += W2 R1 ; Rtot = R2 R1
+        if (kV > 0) {
+            launch_small_gemm(st, kV, kX, kX, Wk.p, kV, R1save.p, kX, Wtot.p, kV, 1.0);
+            if (Wout) launch_copy2d(st, kV, kX, Wtot.p, kV, Wout, ldw);
+        }
+        if (Rout) launch_small_gemm(st, kX, kX, kX, Racc.p, kX, R1save.p, kX, Rout, ldr, 0.0);
+        (void)R1;
+        (void)dummy;
+    }
+    DArr<zd> R1save;
+
+    // ---------------- solver pieces ----------------
+    void setup_space(int r_, int pmax_) {
+        r = r_;
+        pmax = pmax_;
+        prm = pmax * r;
+        ld = n;
+        size_t big = (size_t)ld * prm;
+        V.ensure(big); J1.ensure(big); J2.ensure(big); J5.ensure(big);
+        L.ensure((size_t)prm * prm); H1.ensure((size_t)prm * prm); H2.ensure((size_t)prm * prm);
+        HIP_TRY(hipMemsetAsync(L.p, 0, sizeof(zd) * prm * prm, st));
+        HIP_TRY(hipMemsetAsync(H1.p, 0, sizeof(zd) * prm * prm, st));
+        HIP_TRY(hipMemsetAsync(H2.p, 0, sizeof(zd) * prm * prm, st));
+        p = 0;
+        pts.assign(pmax, cplx(0, 0));
+        S.ensure((size_t)n * 3 * r); CS.ensure((size_t)n * 3 * r);
+        S2.ensure((size_t)n * 3 * r); CS2.ensure((size_t)n * 3 * r);
+        Wt.ensure((size_t)n * 3 * r);
+        trsm_tmp.ensure((size_t)n * 3 * r);
+        Dinv.ensure((size_t)cdiv(n, 64) * 4096);
+        U.ensure((size_t)std::max(pmax, 1) * 4 * r * r);
+    }
+
+    // operator applications (S = I, Schur factor)
+    void apply_C(zd* X, int k) {   // X <- (T - zI)^{-H} X   (mul!(CX, InvMat(Rz'), X))
+        for (int c0 = 0; c0 < k; c0 += 8) {
+            int kc = std::min(8, k - c0);
+            launch_trsm_upper_h(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, trsm_tmp.p);
+        }
+    }
+    void apply_CH(zd* X, int k) {  // X <- (T - zI)^{-1} X
+        for (int c0 = 0; c0 < k; c0 += 8) {
+            int kc = std::min(8, k - c0);
+            launch_trsm_upper(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, trsm_tmp.p);
+        }
+    }
+
+    // svdRaylRitz! + basisUpdate! on the first nS columns of (S, CS): new X|P (2r or r cols)
+    void svd_rr_update(int nS, bool side_L, int keep) {
+        gram.ensure((size_t)nS * nS);
+        Gs.ensure((size_t)nS * nS);
+        sig.ensure(nS);
+        lam.ensure(nS);
+        info.ensure(1);
+        gram_op(CS.p, n, nS, CS.p, n, nS, gram.p, nS);
+        const double shift_scale = 11.0 * ((double)n * nS + (double)nS * (nS + 1));
+        launch_svdrr_small(st, gram.p, 0, nS, r, side_L, shift_scale, Gs.p, 0, sig.p, lam.p, 0, info.p, 1);
+        // XP' = S G[:, :keep], CXP' = CS G[:, :keep]
+        gemm1(S.p, n, nS, Gs.p, nS, keep, S2.p, n, 0.0);
+        gemm1(CS.p, n, nS, Gs.p, nS, keep, CS2.p, n, 0.0);
+        launch_copy2d(st, n, keep, S2.p, n, S.p, n);
+        launch_copy2d(st, n, keep, CS2.p, n, CS.p, n);
+    }
+
+    // residual!(C, X, CX, W, lam, side) into S[:, 2r:3r] (or P slot for the first call)
+    void residual_into(zd* W, bool side_L) {
+        launch_copy2d(st, n, r, CS.p, n, W, n);    // W = CX
+        apply_CH(W, r);                             // W = C^H CX
+        launch_scale_sub(st, n, W, n, S.p, n, r, lam.p, nullptr, side_L ? 1 : 2);
+    }
+
+    double colnorm(const zd* A, int64_t lda) {
+        nrm2.ensure(1);
+        part.ensure(gram_partial_elems(n, 1, 1) + 8);
+        launch_colnorm2(st, n, A, lda, 1, nrm2.p, part.p);
+        double h = 0;
+        d2h(&h, nrm2.p, 1);
+        return std::sqrt(h);
+    }
+
+    // lobpcgSvd on C = (T - zI)^{-H}, side 'L', K = I, starting from S[:, 0:r].
+    // Returns (sigma_min, iterations).
+    std::pair<double, int> lobpcg(cplx z, const rls_tri_control& c) {
+        const bool side_L = (c.lobpcg_side != 'S');
+        if (!side_L) throw RlsError{RLS_E_ARG, "lobpcg_side 'S' is not supported on the Schur path"};
+        const double tol = c.lobpcg_tol_scale * c.tol;
+        launch_tri_diag_inv(st, n, T.p, n, zmk(z.real(), z.imag()), Dinv.p);
+        zd* X = S.p;
+        zd* P = S.p + (int64_t)n * r;
+        zd* W = S.p + (int64_t)n * 2 * r;
+        // iteration 1
+        tall_qr(X, n, r, false);
+        launch_copy2d(st, n, r, X, n, CS.p, n);
+        apply_C(CS.p, r);
+        svd_rr_update(r, side_L, r);
+        residual_into(P, side_L);
+        orth(X, n, r, P, n, r);
+        launch_copy2d(st, n, r, P, n, CS.p + (int64_t)n * r, n);
+        apply_C(CS.p + (int64_t)n * r, r);
+        int it = c.maxit;
+        for (int i = 1; i <= c.maxit; ++i) {
+            if (i == 1) svd_rr_update(2 * r, side_L, 2 * r);
+            else svd_rr_update(3 * r, side_L, 2 * r);
+            residual_into(W, side_L);
+            double res = colnorm(W, n);
+            if (res < tol) { it = i; break; }
+            orth(S.p, n, 2 * r, W, n, r);
+            launch_copy2d(st, n, r, W, n, CS.p + (int64_t)n * 2 * r, n);
+            apply_C(CS.p + (int64_t)n * 2 * r, r);
+        }
+        double s0 = 0;
+        d2h(&s0, sig.p, 1);
+        return {s0, it};
+    }
+
+    // ---- recycle space maintenance ----
+    void shift_blocks(int l) {
+        if (l <= 0) return;
+        const int keep = (p - l) * r;
+        zd* arrs[4] = {V.p, J1.p, J2.p, J5.p};
+        for (zd* A : arrs) {
+            if (keep > 0) {
+                // move through a temporary in chunks of 3r columns (Wt holds n x 3r)
+                for (int c0 = 0; c0 < keep; c0 += 3 * r) {
+                    int kc = std::min(3 * r, keep - c0);
+                    launch_copy2d(st, n, kc, A + (int64_t)n * (l * r + c0), n, Wt.p, n);
+                    launch_copy2d(st, n, kc, Wt.p, n, A + (int64_t)n * c0, n);
+                }
+            }
+        }
+        // small matrices: [dst,dst] <- [src,src] via Hz scratch
+        Hz.ensure((size_t)std::max(keep, 1) * std::max(keep, 1));
+        zd* mats[3] = {L.p, H1.p, H2.p};
+        for (zd* M : mats) {
+            if (keep > 0) {
+                launch_copy2d(st, keep, keep, M + (l * r) + (int64_t)prm * (l * r), prm, Hz.p, keep);
+                launch_copy2d(st, keep, keep, Hz.p, keep, M, prm);
+            }
+        }
+        for (int i = 0; i < p - l; ++i) pts[i] = pts[i + l];
+        p -= l;
+    }
+
+    void add_blocks(const zd* X, cplx z) {
+        const int old = p * r;
+        zd* Vn = V.p + (int64_t)n * old;
+        launch_copy2d(st, n, r, X, n, Vn, n);
+        if (p > 0) {
+            orth_robust(V.p, n, old, Vn, n, r, L.p + (int64_t)prm * old, prm,
+                        L.p + old + (int64_t)prm * old, prm);
+        } else {
+            tall_qr(Vn, n, r, true);
+            launch_copy2d(st, r, r, Racc.p, r, L.p + old + (int64_t)prm * old, prm);
+        }
+        zd* J5n = J5.p + (int64_t)n * old;
+        zd* J1n = J1.p + (int64_t)n * old;
+        zd* J2n = J2.p + (int64_t)n * old;
+        launch_trmm_upper(st, n, T.p, n, Vn, n, r, J5n, n);     // J5 = M V
+        launch_trmmh_upper(st, n, T.p, n, J5n, n, r, J1n, n);   // J1 = M^H J5
+        launch_trmmh_upper(st, n, T.p, n, Vn, n, r, J2n, n);    // J2 = M^H J6, J6 = V
+        zd* h1nn = H1.p + old + (int64_t)prm * old;
+        zd* h2nn = H2.p + old + (int64_t)prm * old;
+        gram_op(Vn, n, r, J1n, n, r, h1nn, prm);                 // H1[new,new] = V^H J1
+        gram_op(Vn, n, r, J2n, n, r, h2nn, prm);                 // H2[new,new] = V^H J2
+        if (p > 0) {
+            gram_op(J1.p, n, old, Vn, n, r, H1.p + (int64_t)prm * old, prm);   // H1[old,new]
+            gram_op(J5.p, n, old, Vn, n, r, H2.p + (int64_t)prm * old, prm);   // H2[old,new] = J3^H V
+            launch_conj_transpose(st, r, old, H1.p + (int64_t)prm * old, prm, H1.p + old, prm);
+            gram_op(Vn, n, r, J2.p, n, old, H2.p + old, prm);                  // H2[new,old]
+        }
+        pts[p] = z;
+        p += 1;
+    }
+
+    void block_unitary_update() {
+        if (p <= 1) return;
+        launch_block_rot_small(st, L.p, H1.p, H2.p, prm, p, r, U.p);
+        zd* arrs[4] = {V.p, J1.p, J2.p, J5.p};
+        launch_block_rot_cols(st, n, arrs, 4, n, r, p, U.p);
+    }
+
+    // updateRecycledSpace! (recySpace.jl:160-223) with the refined block X (n x r).
+    void update_space(const zd* X, cplx z, double tau, bool force_add) {
+        if (p > 0) {
+            const int pr = p * r;
+            // residual of x = X[:, 0] against V and its projection coefficients c = V^H x
+            zd* x = Wt.p;   // n x 1 scratch (Wt >= n x 3r)
+            launch_copy2d(st, n, 1, X, n, x, n);
+            DArr<zd>& cdev = Gc;
+            cdev.ensure((size_t)pr + 1);
+            orth_robust(V.p, n, pr, x, n, 1, cdev.p, pr, cdev.p + pr, 1);
+            std::vector<cplx> ch(pr + 1);
+            d2h(reinterpret_cast<zd*>(ch.data()), cdev.p, pr + 1);
+            const double res0 = std::abs(ch[pr]);
+            int l = 0;
+            if (!force_add) {
+                // residual after dropping the i oldest blocks: res0^2 + sum_{j < i r} |c_j|^2
+                const int l_max = std::min(p - 1, p + 1 - pmax_min);
+                l = l_max;
+                double acc = res0 * res0;
+                for (int i = 1; i <= l_max + 1; ++i) {
+                    for (int j = (i - 1) * r; j < i * r && j < pr; ++j) acc += std::norm(ch[j]);
+                    const double resl = std::sqrt(acc);
+                    if (resl / res0 > tau) { l = i - 1; break; }
+                }
+            }
+            const int p_before = p;
+            if (l > 0) shift_blocks(l);
+            if (l == 0 && p_before == pmax) shift_blocks(1);
+        }
+        add_blocks(X, z);
+        block_unitary_update();
+    }
+    int pmax_min = 1;
+
+    // Batched recycled SVD Rayleigh-Ritz for shifts zs against the current snapshot.
+    // Outputs per shift b: sigma (r values), res = ||W[:,0]||, rr; X_b at Xr + b*n*r.
+    void recycle_rr(const std::vector<cplx>& zs, const std::vector<double>& taus, double shift,
+                    std::vector<double>& sig_out, std::vector<double>& res_out,
+                    std::vector<int>& rr_out) {
+        const int B = (int)zs.size();
+        const int pr = p * r;
+        const int nvec = std::min(pr, 64);
+        zdev.ensure(B);
+        tausd.ensure(B);
+        h2d(zdev.p, reinterpret_cast<const zd*>(zs.data()), B);
+        h2d(tausd.p, taus.data(), B);
+        Hz.ensure((size_t)B * pr * pr);
+        launch_form_hz(st, H1.p, H2.p, prm, pr, zdev.p, shift, Hz.p, (int64_t)pr * pr, B);
+        heevW.ensure((size_t)B * heev_workspace_elems(pr, nvec));
+        lamev.ensure((size_t)B * pr);
+        Gev.ensure((size_t)B * pr * nvec);
+        rr.ensure(B);
+        launch_heev_rr(st, Hz.p, pr, (int64_t)pr * pr, pr, B, heevW.p, lamev.p, pr, Gev.p, pr,
+                       (int64_t)pr * nvec, rr.p, tausd.p, r, nvec, 0);
+        std::vector<int> rrh(B);
+        d2h(rrh.data(), rr.p, B);
+        // shifts whose Ritz dimension exceeds the on-device limit are reported as failed
+        std::vector<int> rrc(B), coffh(B);
+        int tot = 0, rmax = 0;
+        for (int b = 0; b < B; ++b) {
+            rrc[b] = std::min(rrh[b], nvec);
+            coffh[b] = tot;
+            tot += rrc[b];
+            rmax = std::max(rmax, rrc[b]);
+        }
+        rr_out = rrh;
+        coff.ensure(B);
+        DArr<int>& rrd = info;   // clamped rr on device
+        rrd.ensure(B);
+        h2d(coff.p, coffh.data(), B);
+        h2d(rrd.p, rrc.data(), B);
+        C1.ensure((size_t)2 * pr * tot);
+        launch_recy_coef1(st, Gev.p, pr, (int64_t)pr * nvec, rrd.p, coff.p, zdev.p, pr, rmax, C1.p, B);
+        CzX.ensure((size_t)n * tot);
+        {
+            TallOperands ops{};
+            ops.A[0] = J5.p; ops.lda[0] = n; ops.K[0] = pr;
+            ops.A[1] = V.p; ops.lda[1] = n; ops.K[1] = pr;
+            ops.nops = 2;
+            launch_tall_gemm(st, n, ops, C1.p, 2 * pr, 0, tot, CzX.p, n, 0, 0.0, 1);
+        }
+        // per-shift Gram of CzX_ (rr_b x rr_b, padded to ld rmax)
+        Gc.ensure((size_t)B * rmax * rmax);
+        for (int b = 0; b < B; ++b)
+            gram_op(CzX.p + (int64_t)n * coffh[b], n, rrc[b], CzX.p + (int64_t)n * coffh[b], n, rrc[b],
+                    Gc.p + (int64_t)b * rmax * rmax, rmax);
+        V2.ensure((size_t)B * rmax * rmax);
+        sigr.ensure((size_t)B * rmax);
+        const double shift_scale = 11.0 * ((double)n * rmax + (double)rmax * (rmax + 1));
+        launch_gram_svd_asc(st, Gc.p, (int64_t)rmax * rmax, rrd.p, rmax, shift_scale, V2.p,
+                            (int64_t)rmax * rmax, sigr.p, rmax, B);
+        CW.ensure((size_t)4 * pr * B * r);
+        CG.ensure((size_t)pr * B * r);
+        launch_recy_coef2(st, Gev.p, pr, (int64_t)pr * nvec, rrd.p, V2.p, rmax, (int64_t)rmax * rmax,
+                          sigr.p, rmax, zdev.p, pr, r, CW.p, CG.p, B);
+        Wr.ensure((size_t)n * B * r);
+        Xr.ensure((size_t)n * B * r);
+        {
+            TallOperands ops{};
+            ops.A[0] = J1.p; ops.lda[0] = n; ops.K[0] = pr;
+            ops.A[1] = J2.p; ops.lda[1] = n; ops.K[1] = pr;
+            ops.A[2] = J5.p; ops.lda[2] = n; ops.K[2] = pr;
+            ops.A[3] = V.p; ops.lda[3] = n; ops.K[3] = pr;
+            ops.nops = 4;
+            launch_tall_gemm(st, n, ops, CW.p, 4 * pr, 0, B * r, Wr.p, n, 0, 0.0, 1, Xr.p, n, 0, 3, CG.p,
+                             pr);
+        }
+        nrm2.ensure((size_t)B * r);
+        part.ensure(gram_partial_elems(n, B * r, 1) + 8);
+        launch_colnorm2(st, n, Wr.p, n, B * r, nrm2.p, part.p);
+        std::vector<double> nh((size_t)B * r), sh((size_t)B * rmax);
+        d2h(nh.data(), nrm2.p, (size_t)B * r);
+        d2h(sh.data(), sigr.p, (size_t)B * rmax);
+        sig_out.assign((size_t)B * r, 0.0);
+        res_out.assign(B, 0.0);
+        for (int b = 0; b < B; ++b) {
+            res_out[b] = std::sqrt(nh[(size_t)b * r]);
+            for (int j = 0; j < r; ++j) sig_out[(size_t)b * r + j] = sh[(size_t)b * rmax + j];
+            if (rrh[b] > nvec) res_out[b] = std::numeric_limits<double>::infinity();
+        }
+    }
+};
+
+void rls_ctx::launch_chol_rinv_cond(double skip_tol, double shift_scale, int k) {
+    // skip_tol < 0: always factor; otherwise R = Rinv = I when ||G - I||_F < skip_tol
+    launch_chol_rinv(st, gram.p, 0, k, shift_scale, Rm.p, Rinv.p, 0, offI.p, info.p, 1);
+    if (skip_tol >= 0) launch_identity_if_orth(st, k, offI.p, skip_tol, Rm.p, Rinv.p);
+}
+
+// --------------------------------------------------------------------------------------
+// psTriRecy driver
+// --------------------------------------------------------------------------------------
+namespace {
+
+double resolve_fail_dist(const rls_tri_control& c) {
+    if (std::isfinite(c.recycle_fail_dist_thresh)) {
+        if (c.recycle_fail_dist_thresh < 0)
+            throw RlsError{RLS_E_ARG, "recycle_fail_dist_thresh must be nonnegative"};
+        return c.recycle_fail_dist_thresh;
+    }
+    return 1.01 * std::max(std::fabs(c.rec_dx), std::fabs(c.rec_dy));
+}
+
+bool residual_ok(double res, double sigma, double tol, double normC) {
+    return res < normC * std::max(sigma * tol, normC * std::sqrt(2.0) * EPS64);
+}
+
+int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c, double* sigma,
+              rls_tri_stats* stats) {
+    if (ctx->n <= 0 || !ctx->T.p) throw RlsError{RLS_E_STATE, "no operator set (call rls_set_tri_dense)"};
+    if (c.r <= 0 || c.r > 16) throw RlsError{RLS_E_ARG, "r must be in 1..16"};
+    if (c.p <= 0) throw RlsError{RLS_E_ARG, "p_max must be positive"};
+    if (c.p_min < 0 || c.p_min > c.p) throw RlsError{RLS_E_ARG, "p_min must satisfy 0 <= p_min <= p_max"};
+    if ((int64_t)c.p * c.r > 1024) throw RlsError{RLS_E_ARG, "p*r must be <= 1024"};
+    if (!c.implict_recycle) throw RlsError{RLS_E_ARG, "implict_recycle=false is not supported"};
+    const int64_t n = ctx->n;
+    const int r = c.r;
+    ctx->setup_space(r, c.p);
+    ctx->pmax_min = c.p_min;
+    const double thresh = resolve_fail_dist(c);
+    const bool timing = c.collect_timing != 0;
+    for (int64_t i = 0; i < nZ; ++i) sigma[i] = std::numeric_limits<double>::quiet_NaN();
+    auto set_stat = [&](int64_t* arr, int64_t i, int64_t v) { if (arr) arr[i] = v; };
+
+    // initial block: uniform [0,1) real and imaginary parts (rand!(X), psTriRecy.jl:607)
+    {
+        std::mt19937_64 gen(c.seed);
+        std::uniform_real_distribution<double> U01(0.0, 1.0);
+        std::vector<cplx> X0((size_t)n * r);
+        for (auto& v : X0) { double a = U01(gen); double b = U01(gen); v = cplx(a, b); }
+        ctx->h2d(ctx->S.p, reinterpret_cast<zd*>(X0.data()), X0.size());
+    }
+    int scaled_remaining = 0;
+    bool have_failed = false;
+    cplx last_failed(0, 0);
+    int64_t n_it = 0, n_pts = 0, n_windows = 0, n_wshifts = 0;
+    uint64_t t_rec = 0, t_ref = 0, t_upd = 0;
+    const bool batched = (c.mode == RLS_MODE_BATCHED);
+    int window = c.window > 0 ? c.window : 64;
+    int run_hint = 8;
+
+    int64_t i = 0;
+    while (i < nZ) {
+        const cplx z = Z[i];
+        bool recycled = false;
+        int64_t fail_at = i;
+        if (ctx->p > 0) {
+            uint64_t t0 = timing ? now_ns() : 0;
+            int B = 1;
+            if (batched) B = (int)std::min<int64_t>({(int64_t)window, (int64_t)std::max(run_hint, 4), nZ - i});
+            std::vector<cplx> zs(Z + i, Z + i + B);
+            std::vector<double> taus(B);
+            for (int b = 0; b < B; ++b)
+                taus[b] = (scaled_remaining - b > 0) ? c.gap_tau * c.gap_tau_scale : c.gap_tau;
+            std::vector<double> sg, res;
+            std::vector<int> rrv;
+            ctx->recycle_rr(zs, taus, c.shift, sg, res, rrv);
+            n_windows += batched ? 1 : 0;
+            n_wshifts += batched ? B : 0;
+            int f = B;
+            for (int b = 0; b < B; ++b) {
+                if (!residual_ok(res[b], sg[(size_t)b * r], c.tol, c.normC)) { f = b; break; }
+            }
+            // accepted points
+            for (int b = 0; b < f; ++b) {
+                const int64_t k = i + b;
+                set_stat(stats ? stats->p_count : nullptr, k, ctx->p);
+                set_stat(stats ? stats->r_recy : nullptr, k, rrv[b]);
+                set_stat(stats ? stats->lobpcg_it : nullptr, k, 0);
+                set_stat(stats ? stats->M_vec_apps : nullptr, k, 0);
+                set_stat(stats ? stats->M_vec_apps_no_recycle : nullptr, k, 0);
+                set_stat(stats ? stats->R_vec_apps : nullptr, k, 0);
+                sigma[k] = sg[(size_t)b * r];
+                scaled_remaining = scaled_remaining > 0 ? scaled_remaining - 1 : 0;
+            }
+            if (timing) { ctx->sync(); t_rec += now_ns() - t0; }
+            if (batched) run_hint = (f == B) ? std::min(window, 2 * B) : std::max(4, f + 1);
+            if (f == B) { i += B; continue; }
+            fail_at = i + f;
+            // keep_tri.X <- X_recy of the failing shift (psTriRecy.jl:749-757)
+            launch_copy2d(ctx->st, n, r, ctx->Xr.p + (int64_t)n * r * f, n, ctx->S.p, n);
+            scaled_remaining = scaled_remaining > 0 ? scaled_remaining - 1 : 0;
+            set_stat(stats ? stats->r_recy : nullptr, fail_at, rrv[f]);
+            (void)recycled;
+        } else {
+            set_stat(stats ? stats->r_recy : nullptr, fail_at, 0);
+        }
+        // refine at fail_at
+        const int64_t k = fail_at;
+        const cplx zk = Z[k];
+        set_stat(stats ? stats->p_count : nullptr, k, ctx->p);
+        uint64_t t1 = timing ? now_ns() : 0;
+        auto [s_min, it] = ctx->lobpcg(zk, c);
+        sigma[k] = s_min;
+        if (timing) t_ref += now_ns() - t1;
+        int64_t r_apps = it > 0 ? (int64_t)2 * r + (int64_t)it * r + (int64_t)(it - 1) * r : 0;
+        int64_t m_apps = 0;
+        n_it += it;
+        if (stats && stats->pts) { stats->pts[2 * n_pts] = zk.real(); stats->pts[2 * n_pts + 1] = zk.imag(); }
+        n_pts += 1;
+        set_stat(stats ? stats->lobpcg_it : nullptr, k, it);
+        if (k < nZ - 1 && it > 0) {
+            uint64_t t2 = timing ? now_ns() : 0;
+            const bool force_add = have_failed && std::abs(zk - last_failed) <= thresh;
+            ctx->update_space(ctx->S.p, zk, c.recy_tau, force_add);
+            scaled_remaining = force_add ? std::max(c.rec_k, 0) : 0;
+            m_apps += 3 * r;
+            if (timing) { ctx->sync(); t_upd += now_ns() - t2; }
+        }
+        have_failed = true;
+        last_failed = zk;
+        set_stat(stats ? stats->M_vec_apps : nullptr, k, m_apps);
+        set_stat(stats ? stats->M_vec_apps_no_recycle : nullptr, k, m_apps);
+        set_stat(stats ? stats->R_vec_apps : nullptr, k, r_apps);
+        i = k + 1;
+    }
+    ctx->sync();
+    ctx->lastX.resize((size_t)n * r);
+    ctx->d2h(reinterpret_cast<zd*>(ctx->lastX.data()), ctx->S.p, (size_t)n * r);
+    if (stats) {
+        stats->recycle_time_ns = t_rec;
+        stats->refine_time_ns = t_ref;
+        stats->update_time_ns = t_upd;
+        stats->qr_time_ns = 0;
+        stats->preprocess_time_ns = 0;
+        stats->n_it = n_it;
+        stats->n_pts = n_pts;
+        stats->n_windows = n_windows;
+        stats->n_window_shifts = n_wshifts;
+    }
+    return RLS_OK;
+}
+
+template <typename F>
+int guarded(rls_ctx* ctx, F&& f) {
+    if (!ctx) return RLS_E_ARG;
+    try {
+        ctx->err.clear();
+        return f();
+    } catch (const RlsError& e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return RLS_E_HIP;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rls_ctx_create(int device, rls_ctx** out) {
+    if (!out) return RLS_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RLS_E_HIP;
+    if (device < 0 || device >= ndev) return RLS_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return RLS_E_HIP;
+    rls_ctx* c = new rls_ctx();
+    c->device = device;
+    if (hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return RLS_E_HIP;
+    }
+    *out = c;
+    return RLS_OK;
+}
+
+int rls_ctx_destroy(rls_ctx* ctx) {
+    if (!ctx) return RLS_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return RLS_OK;
+}
+
+const char* rls_last_error(const rls_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void rls_tri_control_default(rls_tri_control* c, double normC) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->tol = 1e-3;
+    c->normC = normC;
+    c->shift = normC * normC * EPS64 * 100;
+    c->gap_tau = 1e-10;
+    c->gap_tau_scale = 0.01;
+    c->recy_tau = 1.5;
+    c->recycle_fail_dist_thresh = std::numeric_limits<double>::quiet_NaN();
+    c->r = 6;
+    c->p = 50;
+    c->p_min = 1;
+    c->lobpcg_tol_scale = 0.1;
+    c->lobpcg_side = 'L';
+    c->implict_update = 0;
+    c->implict_recycle = 1;
+    c->collect_timing = 1;
+    c->maxit = 2000;
+    c->rec_k = 0;
+    c->rec_dx = 0;
+    c->rec_dy = 0;
+    c->mode = RLS_MODE_PLAIN_FP64;
+    c->window = 0;
+    c->seed = 0;
+}
+
+int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
+    return guarded(ctx, [&]() -> int {
+        if (!T || n <= 0 || ld < n) throw RlsError{RLS_E_ARG, "rls_set_tri_dense: bad T / n / ld"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->n = n;
+        ctx->T.ensure((size_t)n * n);
+        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->sync();
+        return RLS_OK;
+    });
+}
+
+int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_control* control,
+                  double* sigma_out, rls_tri_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        if (!Z || nZ < 0 || !control || !sigma_out) throw RlsError{RLS_E_ARG, "rls_tri_solve: null argument"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        return solve_tri(ctx, reinterpret_cast<const cplx*>(Z), nZ, *control, sigma_out, stats);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// Launch wrappers of the gfx950 kernels (host-side declarations).
+#pragma once
+#include "rls_common.h"
+
+// ---- tall-skinny products (rls_tall.hip) -----------------------------------------------
+constexpr int TALL_MAX_OPS = 6;
+struct TallOperands {
+    const zd* A[TALL_MAX_OPS];   // n x K[q] column-major blocks, leading dimension lda[q]
+    int64_t lda[TALL_MAX_OPS];
+    int K[TALL_MAX_OPS];
+    int nops;
+};
+
+// Y_b = beta * Y_b + [A_0 .. A_{nops-1}] * B_b for b in [0, batch):
+// B_b = B + b*sB (ldb x k), Y_b = Y + b*sY (ldy x k).  Optional second output
+// Y2_b = A_{q2} * B2_b (B2 = B2 + b*sB, ld ldb2) when Y2 != nullptr.
+void launch_tall_gemm(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
+                      int64_t sB, int k, zd* Y, int64_t ldy, int64_t sY, double beta, int batch,
+                      zd* Y2 = nullptr, int64_t ldy2 = 0, int64_t sY2 = 0, int q2 = -1,
+                      const zd* B2 = nullptr, int64_t ldb2 = 0);
+
+// C (ka x kb, ld ldc) = A^H B with A n x ka, B n x kb.  `part` is a workspace of at least
+// gram_partial_elems(n, ka, kb) complex values.
+size_t gram_partial_elems(int64_t n, int ka, int kb);
+void launch_gram(hipStream_t st, int64_t n, const zd* A, int64_t lda, int ka, const zd* B,
+                 int64_t ldb, int kb, zd* C, int64_t ldc, zd* part);
+
+// colnorm2[j] = ||A[:, j]||^2 (j < k); part workspace >= gram_partial_elems(n, k, 1)
+void launch_colnorm2(hipStream_t st, int64_t n, const zd* A, int64_t lda, int k, double* out,
+                     zd* part);
+
+// Y[:, j] = Y[:, j] * s_j - X[:, j] * t_j   (j < k), s/t device arrays (nullptr -> 1 / 0)
+void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X, int64_t ldx,
+                      int k, const double* s, const double* t, int mode);
+
+// Dense triangular operators with T upper triangular (n x n, ld) and shift z:
+//   trmm:   Y = T X              (upper, no shift)
+//   trmmh:  Y = T^H X
+//   diag_inv: Dinv_j = (T_jj - z I)^{-1} for 64 x 64 diagonal blocks
+//   trsm_u: X <- (T - zI)^{-1} X
+//   trsm_h: X <- (T - zI)^{-H} X
+void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
+                       int64_t ldx, int k, zd* Y, int64_t ldy);
+void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
+                        int64_t ldx, int k, zd* Y, int64_t ldy);
+constexpr int TRI_NB = 64;
+void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* Dinv);
+void launch_trsm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
+                       zd* X, int64_t ldx, int k, zd* tmp);
+void launch_trsm_upper_h(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
+                         zd* X, int64_t ldx, int k, zd* tmp);
+
+// Block rotations of the recycle space: for i = 0..p-2 (block index i, newest block pb),
+// [A_pb A_i] <- [A_pb A_i] * U_i^H with U_i (2r x 2r) stored at U + i*4r^2 (column-major).
+void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int narr, int64_t ld,
+                           int r, int p, const zd* U);
+
+// ---- small dense (rls_small.hip) ---------------------------------------------------------
+size_t heev_workspace_elems(int n, int nvec_max);
+void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
+                    zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
+                    const double* taus, int r, int nvec_max, int mode_fixed);
+void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, int r, bool side_L,
+                        double shift_scale, zd* G, int64_t sG, double* sig, double* lam,
+                        int64_t sSig, int* info, int batch);
+void launch_chol_rinv(hipStream_t st, const zd* Gram, int64_t sGram, int k, double shift_scale,
+                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch);
+void launch_gram_svd_asc(hipStream_t st, const zd* Gram, int64_t sGram, const int* kk, int kmax,
+                         double shift_scale, zd* V, int64_t sV, double* sig, int64_t sS, int batch);
+void launch_form_hz(hipStream_t st, const zd* H1, const zd* H2, int64_t ldh, int pr, const zd* z,
+                    double shift, zd* Hz, int64_t sHz, int batch);
+void launch_recy_coef1(hipStream_t st, const zd* G, int64_t ldg, int64_t sG, const int* rr,
+                       const int* coff, const zd* z, int pr, int rmax, zd* C1, int batch);
+void launch_recy_coef2(hipStream_t st, const zd* G, int64_t ldg, int64_t sG, const int* rr,
+                       const zd* G2, int64_t ldg2, int64_t sG2, const double* sig, int64_t sS,
+                       const zd* z, int pr, int r, zd* CW, zd* CG, int batch);
+void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, int p, int r, zd* U);
+void launch_conj_transpose(hipStream_t st, int m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd);
+void launch_copy2d(hipStream_t st, int64_t m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd);
+void launch_neg2d(hipStream_t st, int m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd);
+void launch_small_gemm(hipStream_t st, int m, int n, int k, const zd* A, int64_t lda, const zd* B,
+                       int64_t ldb, zd* C, int64_t ldc, double beta);
+void launch_identity_if_orth(hipStream_t st, int k, const double* offI, double tol, zd* R, zd* Rinv);

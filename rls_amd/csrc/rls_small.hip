@@ -1,0 +1,1108 @@
+// Small dense FP64 kernels, one workgroup per problem (batched over blockIdx.x):
+//   * Hermitian eigensolver with index-range eigenvectors   (replaces heevr in rls_eigen!,
+//     src/types/RLSLinearAlgebra.jl:895-982, and the Ritz-gap search of find_rr_dim!,
+//     src/core/fastRaylRitz.jl:54-96)
+//   * one-sided Jacobi SVD of a small square factor          (replaces gesdd in rls_svd!,
+//     src/types/RLSLinearAlgebra.jl:334-418)
+//   * Cholesky with diagonal-shift retry                      (the Gram step of
+//     _rls_qr_shiftchol!, src/types/RLSLinearAlgebra.jl:725-796)
+//   * Householder QR with full Q of a small matrix            (rls_block_unitary_rot!,
+//     src/types/RLSLinearAlgebra.jl:164-191, and the LQ of svdRaylRitz!, lobpcgSvd.jl:171-179)
+#include "rls_common.h"
+#include "rls_kernels.h"
+
+// ---------------------------------------------------------------------------------------
+// Hermitian eigensolver.
+// Householder tridiagonalisation (LAPACK zhetd2 lower form) in a per-problem global
+// workspace, all eigenvalues by Sturm-count bisection (one thread per index), eigenvectors
+// 0..nvec-1 by inverse iteration on the tridiagonal with cluster re-orthogonalisation,
+// back-transformed by the stored reflectors.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr int HEEV_THREADS = 256;
+
+__device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+    // number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) cnt++;
+    for (int i = 1; i < n; ++i) {
+        q = d[i] - x - e2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+
+// Solve (T - lam I) y = b in place for a real symmetric tridiagonal T by Gaussian
+// elimination with partial pivoting (LAPACK dlagtf/dlagts idea), sequential, one thread.
+__device__ void tri_shifted_solve(const double* d, const double* e, int n, double lam,
+                                  double* y, double* wk, double eps_floor) {
+    // wk layout: a[n] (diag), b[n] (super), c[n] (sub), u2[n] (second super after pivot), piv[n]
+    double* a = wk;
+    double* b = wk + n;
+    double* u2 = wk + 2 * n;
+    double* pv = wk + 3 * n;
+    for (int i = 0; i < n; ++i) { a[i] = d[i] - lam; b[i] = (i < n - 1) ? e[i] : 0.0; u2[i] = 0.0; pv[i] = 0.0; }
+    // forward elimination
+    for (int i = 0; i < n - 1; ++i) {
+        double sub = e[i];  // T[i+1, i]
+        if (fabs(a[i]) >= fabs(sub)) {
+            if (fabs(a[i]) < eps_floor) a[i] = eps_floor;
+            double m = sub / a[i];
+            pv[i] = 0.0;
+            a[i + 1] -= m * b[i];
+            // rhs
+            y[i + 1] -= m * y[i];
+            u2[i] = 0.0;
+            // store multiplier not needed afterwards
+        } else {
+            // swap rows i and i+1
+            double m = a[i] / sub;
+            pv[i] = 1.0;
+            double ai = a[i], bi = b[i];
+            a[i] = sub;
+            b[i] = a[i + 1];
+            u2[i] = (i < n - 2) ? b[i + 1] : 0.0;
+            a[i + 1] = bi - m * b[i];
+            if (i < n - 2) b[i + 1] = -m * u2[i];
+            double yi = y[i];
+            y[i] = y[i + 1];
+            y[i + 1] = yi - m * y[i];
+            (void)ai;
+        }
+    }
+    if (fabs(a[n - 1]) < eps_floor) a[n - 1] = eps_floor;
+    // back substitution
+    y[n - 1] /= a[n - 1];
+    if (n >= 2) y[n - 2] = (y[n - 2] - b[n - 2] * y[n - 1]) / a[n - 2];
+    for (int i = n - 3; i >= 0; --i) y[i] = (y[i] - b[i] * y[i + 1] - u2[i] * y[i + 2]) / a[i];
+}
+
+
+// Eigenvalues with indices idx[0..k) (0-based, ascending order) of the symmetric tridiagonal
+// (d, e2 = e^2) by multisection: each wanted index gets blockDim/k threads that evaluate
+// Sturm counts at equally spaced interior points of its bracket every round.
+__device__ void multisect(const double* d, const double* e2, int n, const int* idx, int k,
+                          double* lam, double gl, double gu, double pivmin) {
+    __shared__ double s_lo[64], s_hi[64];
+    __shared__ double s_x[HEEV_THREADS];
+    __shared__ int s_c[HEEV_THREADS];
+    __shared__ int s_active;
+    const double eps = 2.220446049250313e-16;
+    const int tid = threadIdx.x;
+    if (k <= 0) return;
+    const int G = blockDim.x / k;          // threads per eigenvalue (>= 4 since k <= 64)
+    if (tid < k) { s_lo[tid] = gl; s_hi[tid] = gu; }
+    __syncthreads();
+    for (int round = 0; round < 200; ++round) {
+        const int j = tid / G, t = tid % G;
+        if (j < k) {
+            double lo = s_lo[j], hi = s_hi[j];
+            double x = lo + (hi - lo) * (double)(t + 1) / (double)(G + 1);
+            s_x[tid] = x;
+            s_c[tid] = (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin)
+                           ? -1 : sturm_count(d, e2, n, x, pivmin);
+        }
+        __syncthreads();
+        if (tid == 0) s_active = 0;
+        __syncthreads();
+        if (tid < k) {
+            const int base = tid * G;
+            if (s_c[base] >= 0) {
+                double lo = s_lo[tid], hi = s_hi[tid];
+                for (int q = 0; q < G; ++q) {
+                    if (s_c[base + q] <= idx[tid]) lo = s_x[base + q];
+                    else { hi = s_x[base + q]; break; }
+                }
+                s_lo[tid] = lo; s_hi[tid] = hi;
+                atomicAdd(&s_active, 1);
+            }
+        }
+        __syncthreads();
+        if (s_active == 0) break;
+        __syncthreads();
+    }
+    if (tid < k) lam[idx[tid]] = 0.5 * (s_lo[tid] + s_hi[tid]);
+    __syncthreads();
+}
+
+}  // namespace
+
+// A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
+//    batch stride sA (elements).  W: workspace, per problem n*n complex + 8n doubles.
+// Outputs: lam (n per problem, ascending), G (ldg x nvec_max complex per problem, stride sG),
+//          rr (per problem: the Ritz dimension chosen by the gap rule, see below).
+// Gap rule (find_rr_dim!): gap = lam_max / tau * eps; windows n_chk = n_init, n_init+2r, ...
+// (<= n); r_ = first i with |lam_i - lam_{i+1}| > gap inside the largest window, then
+// max(r_, r); if none, r_ = n.  Eigenvectors 1..min(r_, nvec_max) are returned.
+// mode_fixed > 0: skip the gap rule and return eigenvectors 1..mode_fixed (rr = mode_fixed).
+__global__ void __launch_bounds__(HEEV_THREADS)
+k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restrict__ Wk,
+          int64_t sW, double* __restrict__ lam_out, int64_t sL, zd* __restrict__ G, int64_t ldg,
+          int64_t sG, int* __restrict__ rr_out, const double* __restrict__ taus, int r,
+          int nvec_max, int mode_fixed) {
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const zd* Ab = A + (int64_t)b * sA;
+    zd* M = Wk + (int64_t)b * sW;                  // n x n full Hermitian copy, ld = n
+    double* dws = reinterpret_cast<double*>(M + (int64_t)n * n);
+    double* d = dws;            // n
+    double* e = d + n;          // n
+    double* e2 = e + n;         // n
+    double* lam = e2 + n;       // n
+    zd* tau = reinterpret_cast<zd*>(lam + n);  // n complex
+    double* Y = reinterpret_cast<double*>(tau + n);   // n x nvec_max real eigenvectors of T
+    double* scratch = Y + (int64_t)n * nvec_max;     // per-vector tri-solve workspace 4n x nvec
+    __shared__ double red[HEEV_THREADS / 64];
+    __shared__ zd sv[1024];   // Householder vector (n <= 1024)
+    __shared__ zd sw[1024];   // w
+    __shared__ int s_rr;
+
+    // 1. copy with Hermitian completion from the upper triangle
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += blockDim.x) {
+        int i = (int)(idx % n), j = (int)(idx / n);
+        zd v = (i <= j) ? Ab[i + lda * j] : zconj(Ab[j + lda * i]);
+        if (i == j) v.y = 0.0;
+        M[i + (int64_t)n * j] = v;
+    }
+    __syncthreads();
+
+    // 2. tridiagonalisation, lower form
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;                 // length of x = M[k+1:n, k]
+        zd* x = M + (k + 1) + (int64_t)n * k;
+        // xnorm^2 of x[1:]
+        double part = 0.0;
+        for (int i = 1 + tid; i < m; i += blockDim.x) part += zabs2(x[i]);
+        double xn2 = block_sum(part, red);
+        zd alpha = x[0];
+        zd t = zzero();
+        double beta;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            beta = alpha.x;
+            t = zzero();
+            for (int i = tid; i < m; i += blockDim.x) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zzero();
+        } else {
+            double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = (alpha.x >= 0.0) ? -nrm : nrm;
+            t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+            zd scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+            for (int i = tid; i < m; i += blockDim.x) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zmul(x[i], scal);
+        }
+        __syncthreads();
+        if (tid == 0) { e[k] = beta; tau[k] = t; d[k] = M[k + (int64_t)n * k].x; }
+        // store v below the subdiagonal (x[1:] <- v[1:]) for the back-transformation
+        for (int i = 1 + tid; i < m; i += blockDim.x) x[i] = sv[i];
+        __syncthreads();
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        // w = tau * A22 v   (A22 = M[k+1:, k+1:])
+        zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
+        for (int i = tid; i < m; i += blockDim.x) {
+            zd acc = zzero();
+            for (int j = 0; j < m; ++j) acc = zfma(A22[i + (int64_t)n * j], sv[j], acc);
+            sw[i] = zmul(t, acc);
+        }
+        __syncthreads();
+        // alpha2 = -1/2 tau (w^H v)
+        double pr = 0.0, pi = 0.0;
+        for (int i = tid; i < m; i += blockDim.x) {
+            zd c = zcmul(sw[i], sv[i]);
+            pr += c.x; pi += c.y;
+        }
+        double whv_r = block_sum(pr, red);
+        double whv_i = block_sum(pi, red);
+        zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
+        for (int i = tid; i < m; i += blockDim.x) sw[i] = zfma(a2, sv[i], sw[i]);
+        __syncthreads();
+        // A22 -= v w^H + w v^H
+        for (int i = tid; i < m; i += blockDim.x) {
+            zd vi = sv[i], wi = sw[i];
+            for (int j = 0; j < m; ++j) {
+                zd a = A22[i + (int64_t)n * j];
+                a = zsub(a, zmul(vi, zconj(sw[j])));
+                a = zsub(a, zmul(wi, zconj(sv[j])));
+                A22[i + (int64_t)n * j] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[n - 1] = M[(n - 1) + (int64_t)n * (n - 1)].x;
+        e[n - 1] = 0.0;
+        tau[n - 1] = zzero();
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) e2[i] = e[i] * e[i];
+    __syncthreads();
+
+    // 3. Gershgorin interval
+    double gl = 1e300, gu = -1e300;
+    for (int i = 0; i < n; ++i) {
+        double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        gl = fmin(gl, d[i] - off);
+        gu = fmax(gu, d[i] + off);
+    }
+    const double tnorm = fmax(fabs(gl), fabs(gu));
+    const double eps = 2.220446049250313e-16;
+    const double pivmin = fmax(1e-300, tnorm * 1e-300);
+    gl -= 2.0 * eps * tnorm * n + pivmin;
+    gu += 2.0 * eps * tnorm * n + pivmin;
+    double* lo_ = lam_out + (int64_t)b * sL;
+    for (int i = tid; i < n; i += blockDim.x) lam[i] = nan("");
+    __syncthreads();
+
+    // 4. eigenvalues on demand (multisection) + Ritz-dimension rule of find_rr_dim!:
+    //    gap = lam_max / tau * eps; windows n_chk = n_init, n_init + 2r, ... (<= n);
+    //    r_ = first i with |lam_i - lam_{i+1}| > gap, then max(r_, r); none -> n.
+    {
+        __shared__ int s_idx[64];
+        __shared__ int s_k, s_kc;
+        __shared__ int s_done;
+        int have = 0;   // eigenvalues 0..have-1 computed
+        bool have_max = false;
+        if (tid == 0) { s_rr = 0; s_done = 0; }
+        __syncthreads();
+        const int n_init = (mode_fixed > 0) ? (mode_fixed < n ? mode_fixed : n)
+                                            : ((2 * r < n) ? 2 * r : n);
+        int target = n_init;
+        while (true) {
+            // wanted indices: have..target-1 and n-1 (once)
+            if (tid == 0) {
+                int k = 0, kc = 0;
+                if (!have_max && mode_fixed <= 0) s_idx[k++] = n - 1;
+                for (int i = have; i < target && k < 64; ++i) {
+                    if (i == n - 1 && !have_max && mode_fixed <= 0) { ++kc; continue; }
+                    s_idx[k++] = i; ++kc;
+                }
+                s_k = k; s_kc = kc;
+            }
+            __syncthreads();
+            multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
+            __syncthreads();
+            have_max = true;
+            have += s_kc;
+            if (have < target) continue;   // more than 64 requested: loop
+            if (tid == 0) {
+                if (mode_fixed > 0) {
+                    s_rr = n_init; s_done = 1;
+                } else {
+                    double tauv = taus[b];
+                    double gap = lam[n - 1] / tauv * eps;
+                    int first = 0;
+                    for (int i = 0; i + 1 < target; ++i)
+                        if (fabs(lam[i] - lam[i + 1]) > gap) { first = i + 1; break; }
+                    if (first > 0) {
+                        s_rr = first > r ? first : r;
+                        if (s_rr > n) s_rr = n;
+                        s_done = 1;
+                    } else if (target + 2 * r <= n) {
+                        s_done = 0;
+                    } else {
+                        s_rr = n; s_done = 1;
+                    }
+                }
+            }
+            __syncthreads();
+            if (s_done) break;
+            target = target + 2 * r;
+            __syncthreads();
+        }
+        // eigenvalues needed for the eigenvectors 0..rr-1
+        int need = s_rr < nvec_max ? s_rr : nvec_max;
+        while (have < need) {
+            if (tid == 0) {
+                int k = 0;
+                for (int i = have; i < need && k < 64; ++i) s_idx[k++] = i;
+                s_k = k; s_kc = k;
+            }
+            __syncthreads();
+            multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
+            __syncthreads();
+            have += s_kc;
+        }
+        if (tid == 0) rr_out[b] = s_rr;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) lo_[i] = lam[i];
+    const int nv = s_rr < nvec_max ? s_rr : nvec_max;
+
+    // 5. inverse iteration for eigenvectors 0..nv-1 (thread j owns vector j)
+    const double ortol = 1e-3 * tnorm;
+    const double floorv = eps * tnorm + pivmin;
+    for (int it = 0; it < 3; ++it) {
+        for (int j = tid; j < nv; j += blockDim.x) {
+            double* y = Y + (int64_t)j * n;
+            if (it == 0) {
+                // deterministic start vector
+                uint32_t s = 0x9E3779B9u * (uint32_t)(j + 1);
+                for (int i = 0; i < n; ++i) {
+                    s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+                    y[i] = (double)(s & 0xFFFFFF) / 16777216.0 - 0.5;
+                }
+            }
+            // small perturbation keeps (T - lam I) nonsingular for repeated eigenvalues
+            double lj = lam[j];
+            tri_shifted_solve(d, e, n, lj, y, scratch + (int64_t)j * 4 * n, floorv);
+            double nr = 0.0;
+            for (int i = 0; i < n; ++i) nr += y[i] * y[i];
+            nr = 1.0 / sqrt(nr);
+            for (int i = 0; i < n; ++i) y[i] *= nr;
+        }
+        __syncthreads();
+        // modified Gram-Schmidt inside clusters (eigenvalues closer than ortol), in index order
+        for (int j = 1; j < nv; ++j) {
+            for (int q = 0; q < j; ++q) {
+                if (fabs(lam[j] - lam[q]) > ortol) continue;
+                double* yj = Y + (int64_t)j * n;
+                const double* yq = Y + (int64_t)q * n;
+                double part2 = 0.0;
+                for (int i = tid; i < n; i += blockDim.x) part2 += yj[i] * yq[i];
+                double dot = block_sum(part2, red);
+                for (int i = tid; i < n; i += blockDim.x) yj[i] -= dot * yq[i];
+                __syncthreads();
+            }
+            double* yj = Y + (int64_t)j * n;
+            double part3 = 0.0;
+            for (int i = tid; i < n; i += blockDim.x) part3 += yj[i] * yj[i];
+            double nr = block_sum(part3, red);
+            nr = 1.0 / sqrt(nr);
+            for (int i = tid; i < n; i += blockDim.x) yj[i] *= nr;
+            __syncthreads();
+        }
+    }
+
+    // 6. back-transformation G = H(0) H(1) ... H(n-2) y, applied as y <- H(k) y for k descending
+    zd* Gb = G + (int64_t)b * sG;
+    for (int j = 0; j < nv; ++j) {
+        const double* y = Y + (int64_t)j * n;
+        for (int i = tid; i < n; i += blockDim.x) Gb[i + ldg * j] = zmk(y[i], 0.0);
+    }
+    __syncthreads();
+    for (int k = n - 2; k >= 0; --k) {
+        zd t = tau[k];
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        const int m = n - k - 1;
+        const zd* x = M + (k + 1) + (int64_t)n * k;   // v[0] = 1 implicit, v[1:] stored
+        for (int j = 0; j < nv; ++j) {
+            zd* g = Gb + (k + 1) + ldg * j;
+            double pr = 0.0, pi = 0.0;
+            for (int i = tid; i < m; i += blockDim.x) {
+                zd vi = (i == 0) ? zmk(1.0, 0.0) : x[i];
+                zd c = zcmul(vi, g[i]);
+                pr += c.x; pi += c.y;
+            }
+            double sr = block_sum(pr, red), si = block_sum(pi, red);
+            zd f = zmul(t, zmk(sr, si));   // H = I - tau v v^H
+            for (int i = tid; i < m; i += blockDim.x) {
+                zd vi = (i == 0) ? zmk(1.0, 0.0) : x[i];
+                g[i] = zsub(g[i], zmul(vi, f));
+            }
+            __syncthreads();
+        }
+    }
+}
+
+size_t heev_workspace_elems(int n, int nvec_max) {
+    // complex elements per problem
+    size_t doubles = (size_t)8 * n + (size_t)n * nvec_max + (size_t)4 * n * nvec_max;
+    return (size_t)n * n + (doubles + 1) / 2 + 8;
+}
+
+void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
+                    zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
+                    const double* taus, int r, int nvec_max, int mode_fixed) {
+    if (n > 1024) throw RlsError{-1, "heev: n > 1024 unsupported"};
+    int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
+    hipLaunchKernelGGL(k_heev_rr, dim3(batch), dim3(HEEV_THREADS), 0, st, A, lda, sA, n, Wk, sW,
+                       lam, sL, G, ldg, sG, rr, taus, r, nvec_max, mode_fixed);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Block helpers for small matrices held in LDS (one workgroup)
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr int SMALL_THREADS = 256;
+
+// Cholesky of a Hermitian k x k matrix in LDS (column-major, ld k), upper factor in place:
+// A = R^H R.  Returns false on a non-positive pivot.  All threads participate.
+__device__ bool lds_cholesky_upper(zd* A, int k, int* s_fail) {
+    const int tid = threadIdx.x;
+    if (tid == 0) *s_fail = 0;
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+        // R[j,j] = sqrt(A[j,j] - sum_{m<j} |R[m,j]|^2) ; already updated right-looking
+        __syncthreads();
+        double djj = A[j + k * j].x;
+        if (!(djj > 0.0) || !isfinite(djj)) {
+            if (tid == 0) *s_fail = 1;
+            __syncthreads();
+            return false;
+        }
+        double rjj = sqrt(djj);
+        __syncthreads();
+        // row j of R: R[j, c] = A[j, c] / rjj for c > j
+        for (int c = j + 1 + tid; c < k; c += blockDim.x) A[j + k * c] = zscale(A[j + k * c], 1.0 / rjj);
+        if (tid == 0) A[j + k * j] = zmk(rjj, 0.0);
+        __syncthreads();
+        // trailing update A[a, c] -= conj(R[j,a]) R[j,c] for j < a <= c
+        const int m = k - j - 1;
+        for (int idx = tid; idx < m * m; idx += blockDim.x) {
+            int a = j + 1 + idx % m, c = j + 1 + idx / m;
+            if (a <= c) A[a + k * c] = zsub(A[a + k * c], zcmul(A[j + k * a], A[j + k * c]));
+        }
+    }
+    __syncthreads();
+    // zero the strict lower triangle
+    for (int idx = tid; idx < k * k; idx += blockDim.x) {
+        int a = idx % k, c = idx / k;
+        if (a > c) A[idx] = zzero();
+    }
+    __syncthreads();
+    return true;
+}
+
+// Inverse of an upper-triangular k x k matrix R (LDS) into Ri (LDS): thread per column.
+__device__ void lds_upper_inverse(const zd* R, zd* Ri, int k) {
+    const int tid = threadIdx.x;
+    for (int c = tid; c < k; c += blockDim.x) {
+        for (int a = 0; a < k; ++a) Ri[a + k * c] = zzero();
+        // solve R x = e_c, x[a] = 0 for a > c
+        for (int a = c; a >= 0; --a) {
+            zd s = (a == c) ? zmk(1.0, 0.0) : zzero();
+            for (int m = a + 1; m <= c; ++m) s = zsub(s, zmul(R[a + k * m], Ri[m + k * c]));
+            Ri[a + k * c] = zdiv(s, R[a + k * a]);
+        }
+    }
+    __syncthreads();
+}
+
+// One-sided Jacobi SVD of the k x k matrix A (LDS, column-major).  On exit the columns of A
+// are U*Sigma, V (LDS) holds the right singular vectors, sig[j] = ||A[:, j]||, and perm[]
+// lists column indices by descending singular value.
+__device__ void lds_jacobi_svd(zd* A, zd* V, double* sig, int* perm, int k, int* s_rot) {
+    const int tid = threadIdx.x;
+    const double eps = 2.220446049250313e-16;
+    for (int idx = tid; idx < k * k; idx += blockDim.x) V[idx] = (idx % k == idx / k) ? zmk(1.0, 0.0) : zzero();
+    __syncthreads();
+    const double tolj = eps * (k > 4 ? k : 4);
+    const int kk = (k + 1) & ~1;                 // even player count (player k is a dummy)
+    const int npair = kk / 2;
+    const int gsz = 8;                           // threads per pair
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        if (tid == 0) *s_rot = 0;
+        __syncthreads();
+        for (int step = 0; step < kk - 1; ++step) {
+            for (int pb = tid / gsz; pb < npair; pb += blockDim.x / gsz) {
+                // circle method: player kk-1 fixed, others rotate
+                int a = (pb == 0) ? kk - 1 : (step + pb) % (kk - 1);
+                int b = (step + kk - 1 - pb) % (kk - 1);
+                if (pb == 0) b = step % (kk - 1);
+                int p = a < b ? a : b, q = a < b ? b : a;
+                const int lane = tid % gsz;
+                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                if (q < k) {
+                    for (int i = lane; i < k; i += gsz) {
+                        zd ap = A[i + k * p], aq = A[i + k * q];
+                        al += zabs2(ap); be += zabs2(aq);
+                        zd g = zcmul(ap, aq);
+                        gr += g.x; gi += g.y;
+                    }
+                }
+                for (int o = gsz / 2; o > 0; o >>= 1) {
+                    al += __shfl_xor(al, o); be += __shfl_xor(be, o);
+                    gr += __shfl_xor(gr, o); gi += __shfl_xor(gi, o);
+                }
+                double ag = sqrt(gr * gr + gi * gi);
+                if (q < k && ag > tolj * sqrt(al * be) && ag > 1e-300) {
+                    double zeta = (be - al) / (2.0 * ag);
+                    double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                    zd ph = zmk(gr / ag, -gi / ag);   // e^{-i phi}
+                    for (int i = lane; i < k; i += gsz) {
+                        zd ap = A[i + k * p], aq = zmul(ph, A[i + k * q]);
+                        A[i + k * p] = zsub(zscale(ap, c), zscale(aq, s));
+                        A[i + k * q] = zadd(zscale(ap, s), zscale(aq, c));
+                        zd vp = V[i + k * p], vq = zmul(ph, V[i + k * q]);
+                        V[i + k * p] = zsub(zscale(vp, c), zscale(vq, s));
+                        V[i + k * q] = zadd(zscale(vp, s), zscale(vq, c));
+                    }
+                    if (lane == 0) atomicAdd(s_rot, 1);
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (*s_rot == 0) break;
+        __syncthreads();
+    }
+    for (int j = tid; j < k; j += blockDim.x) {
+        double s2 = 0.0;
+        for (int i = 0; i < k; ++i) s2 += zabs2(A[i + k * j]);
+        sig[j] = sqrt(s2);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int j = 0; j < k; ++j) perm[j] = j;
+        for (int a = 1; a < k; ++a) {          // insertion sort, descending
+            int v = perm[a], bidx = a - 1;
+            while (bidx >= 0 && sig[perm[bidx]] < sig[v]) { perm[bidx + 1] = perm[bidx]; --bidx; }
+            perm[bidx + 1] = v;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// svdRaylRitz! small part (src/core/lobpcgSvd.jl:126-184) from the Gram of the image block.
+// In: Gram = CS^H CS (nS x nS).  R = chol(Gram) (diagonal shift retry as in
+// _rls_qr_shiftchol!, RLSLinearAlgebra.jl:757-789) has the singular values of CS; its SVD
+// gives sigma and the Ritz coefficients G.  side 'L' keeps the largest singular values and
+// inverts them.  For nS > r the P-direction coefficients G[:, r:2r] = G_rest Q^H with Q the
+// orthonormal-row factor of the LQ of G[0:r, r:nS].
+// Out (per problem b): G (nS x nS, ld nS), sig (nS), lam (nS), info (0 ok, >0 shifts used).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(SMALL_THREADS)
+k_svdrr_small(const zd* __restrict__ Gram, int64_t sGram, int nS, int r, int side_L,
+              double shift_scale, zd* __restrict__ Gout, int64_t sG, double* __restrict__ sig_out,
+              double* __restrict__ lam_out, int64_t sSig, int* __restrict__ info) {
+    extern __shared__ zd lds[];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int k = nS;
+    zd* A = lds;              // k*k
+    zd* V = A + k * k;        // k*k
+    zd* Q = V + k * k;        // k*k scratch
+    __shared__ double sig[128];
+    __shared__ int perm[128];
+    __shared__ int s_fail, s_rot;
+    const zd* Gb = Gram + (int64_t)b * sGram;
+    // trace for the shift
+    double tr = 0.0;
+    for (int j = 0; j < k; ++j) tr += Gb[j + (int64_t)k * j].x;
+    const double eps = 2.220446049250313e-16;
+    int nshift = 0;
+    double shift = 0.0;
+    for (int attempt = 0; attempt < 60; ++attempt) {
+        for (int idx = tid; idx < k * k; idx += blockDim.x) {
+            int a = idx % k, c = idx / k;
+            zd v = (a <= c) ? Gb[a + (int64_t)k * c] : zconj(Gb[c + (int64_t)k * a]);
+            if (a == c) v = zmk(v.x + shift, 0.0);
+            A[idx] = v;
+        }
+        __syncthreads();
+        if (lds_cholesky_upper(A, k, &s_fail)) break;
+        ++nshift;
+        shift += shift_scale * eps * tr;   // 11 (m n + n (n+1)) eps ||X||_g^2 accumulated
+        __syncthreads();
+    }
+    // SVD of R (A holds R)
+    lds_jacobi_svd(A, V, sig, perm, k, &s_rot);
+    // order: side 'L' -> descending; else ascending
+    zd* Go = Gout + (int64_t)b * sG;
+    double* so = sig_out + (int64_t)b * sSig;
+    double* lo = lam_out + (int64_t)b * sSig;
+    for (int idx = tid; idx < k * k; idx += blockDim.x) {
+        int a = idx % k, c = idx / k;
+        int src = side_L ? perm[c] : perm[k - 1 - c];
+        Q[a + k * c] = V[a + k * src];
+    }
+    __syncthreads();
+    if (tid < k) {
+        int src = side_L ? perm[tid] : perm[k - 1 - tid];
+        double s = sig[src];
+        double sv = s, lv = 0.0;
+        if (tid < r) {
+            if (side_L) { sv = 1.0 / s; lv = sv * sv; } else { lv = s * s; }
+        }
+        so[tid] = sv;
+        lo[tid] = lv;
+    }
+    __syncthreads();
+    // G_P via LQ of G[0:r, r:k]: Q1 = thin Q of (G[0:r, r:k])^H  (k-r x r), MGS twice
+    if (k > r) {
+        const int m = k - r;
+        zd* P = V;   // reuse: m x r matrix (column-major, ld m) = (G[0:r, r:k])^H
+        for (int idx = tid; idx < m * r; idx += blockDim.x) {
+            int a = idx % m, c = idx / m;       // P[a, c] = conj(G[c, r + a])
+            P[a + m * c] = zconj(Q[c + k * (r + a)]);
+        }
+        __syncthreads();
+        __shared__ double red[SMALL_THREADS / 64];
+        for (int c = 0; c < r; ++c) {
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int q = 0; q < c; ++q) {
+                    double pr = 0.0, pi = 0.0;
+                    for (int a = tid; a < m; a += blockDim.x) {
+                        zd t = zcmul(P[a + m * q], P[a + m * c]);
+                        pr += t.x; pi += t.y;
+                    }
+                    double dr = block_sum(pr, red), di = block_sum(pi, red);
+                    for (int a = tid; a < m; a += blockDim.x)
+                        P[a + m * c] = zsub(P[a + m * c], zmul(P[a + m * q], zmk(dr, di)));
+                    __syncthreads();
+                }
+            }
+            double pn = 0.0;
+            for (int a = tid; a < m; a += blockDim.x) pn += zabs2(P[a + m * c]);
+            double nr = block_sum(pn, red);
+            nr = nr > 0 ? 1.0 / sqrt(nr) : 0.0;
+            for (int a = tid; a < m; a += blockDim.x) P[a + m * c] = zscale(P[a + m * c], nr);
+            __syncthreads();
+        }
+        // G_P[:, c] = sum_a G[:, r + a] * P[a, c]   (G_rest Q^H with Q^H = P)
+        zd* GP = A;   // k x r scratch
+        for (int idx = tid; idx < k * r; idx += blockDim.x) {
+            int row = idx % k, c = idx / k;
+            zd acc = zzero();
+            for (int a = 0; a < m; ++a) acc = zfma(Q[row + k * (r + a)], P[a + m * c], acc);
+            GP[row + k * c] = acc;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k * r; idx += blockDim.x) {
+            int row = idx % k, c = idx / k;
+            Q[row + k * (r + c)] = GP[row + k * c];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < k * k; idx += blockDim.x) Go[idx] = Q[idx];
+    if (tid == 0) info[b] = nshift;
+}
+
+void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, int r, bool side_L,
+                        double shift_scale, zd* G, int64_t sG, double* sig, double* lam,
+                        int64_t sSig, int* info, int batch) {
+    if (nS > 64) throw RlsError{-1, "svdrr: nS > 64 unsupported"};
+    size_t lds = (size_t)3 * nS * nS * sizeof(zd);
+    hipLaunchKernelGGL(k_svdrr_small, dim3(batch), dim3(SMALL_THREADS), lds, st, Gram, sGram, nS,
+                       r, side_L ? 1 : 0, shift_scale, G, sG, sig, lam, sSig, info);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Cholesky factor for a Cholesky-QR pass: Gram (k x k) -> Rinv (k x k upper) and R, with the
+// shift retry of _rls_qr_shiftchol! (RLSLinearAlgebra.jl:757-789).  Also reports
+// ||Gram - I||_F (the pass-convergence test of RLSLinearAlgebra.jl:775-782).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(SMALL_THREADS)
+k_chol_rinv(const zd* __restrict__ Gram, int64_t sGram, int k, double shift_scale,
+            zd* __restrict__ Rout, zd* __restrict__ Rinv, int64_t sR, double* __restrict__ offI,
+            int* __restrict__ info) {
+    extern __shared__ zd lds[];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    zd* A = lds;
+    zd* Ri = A + k * k;
+    __shared__ int s_fail;
+    __shared__ double red[SMALL_THREADS / 64];
+    const zd* Gb = Gram + (int64_t)b * sGram;
+    double tr = 0.0;
+    for (int j = 0; j < k; ++j) tr += Gb[j + (int64_t)k * j].x;
+    double part = 0.0;
+    for (int idx = tid; idx < k * k; idx += blockDim.x) {
+        int a = idx % k, c = idx / k;
+        zd v = Gb[idx];
+        if (a == c) v.x -= 1.0;
+        part += zabs2(v);
+    }
+    double fro = block_sum(part, red);
+    const double eps = 2.220446049250313e-16;
+    int nshift = 0;
+    double shift = 0.0;
+    bool ok = false;
+    for (int attempt = 0; attempt < 60; ++attempt) {
+        for (int idx = tid; idx < k * k; idx += blockDim.x) {
+            int a = idx % k, c = idx / k;
+            zd v = (a <= c) ? Gb[a + (int64_t)k * c] : zconj(Gb[c + (int64_t)k * a]);
+            if (a == c) v = zmk(v.x + shift, 0.0);
+            A[idx] = v;
+        }
+        __syncthreads();
+        if (lds_cholesky_upper(A, k, &s_fail)) { ok = true; break; }
+        ++nshift;
+        shift += shift_scale * eps * (tr > 0 ? tr : 1e-300);
+        __syncthreads();
+    }
+    lds_upper_inverse(A, Ri, k);
+    zd* Ro = Rout + (int64_t)b * sR;
+    zd* Rio = Rinv + (int64_t)b * sR;
+    for (int idx = tid; idx < k * k; idx += blockDim.x) { Ro[idx] = A[idx]; Rio[idx] = Ri[idx]; }
+    if (tid == 0) { info[b] = ok ? nshift : -1; offI[b] = sqrt(fro); }
+}
+
+void launch_chol_rinv(hipStream_t st, const zd* Gram, int64_t sGram, int k, double shift_scale,
+                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch) {
+    if (k > 64) throw RlsError{-1, "chol_rinv: k > 64 unsupported"};
+    size_t lds = (size_t)2 * k * k * sizeof(zd);
+    hipLaunchKernelGGL(k_chol_rinv, dim3(batch), dim3(SMALL_THREADS), lds, st, Gram, sGram, k,
+                       shift_scale, R, Rinv, sR, offI, info);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Small SVD for the recycled Rayleigh-Ritz (fastRaylRitz.jl:190-193): Gram of CzX_ (k x k)
+// -> R -> SVD ascending; outputs sig (k) and V (k x k) ascending order.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(SMALL_THREADS)
+k_gram_svd_asc(const zd* __restrict__ Gram, int64_t sGram, const int* __restrict__ kk_arr, int kmax,
+               double shift_scale, zd* __restrict__ Vout, int64_t sV, double* __restrict__ sig_out,
+               int64_t sS) {
+    extern __shared__ zd lds[];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int k = kk_arr ? (kk_arr[b] < kmax ? kk_arr[b] : kmax) : kmax;
+    zd* A = lds;
+    zd* V = A + k * k;
+    __shared__ double sig[128];
+    __shared__ int perm[128];
+    __shared__ int s_fail, s_rot;
+    const zd* Gb = Gram + (int64_t)b * sGram;   // ld = kmax
+    double tr = 0.0;
+    for (int j = 0; j < k; ++j) tr += Gb[j + (int64_t)kmax * j].x;
+    const double eps = 2.220446049250313e-16;
+    double shift = 0.0;
+    for (int attempt = 0; attempt < 60; ++attempt) {
+        for (int idx = tid; idx < k * k; idx += blockDim.x) {
+            int a = idx % k, c = idx / k;
+            zd v = (a <= c) ? Gb[a + (int64_t)kmax * c] : zconj(Gb[c + (int64_t)kmax * a]);
+            if (a == c) v = zmk(v.x + shift, 0.0);
+            A[idx] = v;
+        }
+        __syncthreads();
+        if (lds_cholesky_upper(A, k, &s_fail)) break;
+        shift += shift_scale * eps * (tr > 0 ? tr : 1e-300);
+        __syncthreads();
+    }
+    lds_jacobi_svd(A, V, sig, perm, k, &s_rot);
+    zd* Vo = Vout + (int64_t)b * sV;     // ld = kmax
+    double* so = sig_out + (int64_t)b * sS;
+    for (int idx = tid; idx < k * k; idx += blockDim.x) {
+        int a = idx % k, c = idx / k;
+        Vo[a + (int64_t)kmax * c] = V[a + k * perm[k - 1 - c]];
+    }
+    for (int c = tid; c < k; c += blockDim.x) so[c] = sig[perm[k - 1 - c]];
+}
+
+void launch_gram_svd_asc(hipStream_t st, const zd* Gram, int64_t sGram, const int* kk, int kmax,
+                         double shift_scale, zd* V, int64_t sV, double* sig, int64_t sS, int batch) {
+    if (kmax > 64) throw RlsError{-1, "gram_svd: k > 64 unsupported"};
+    size_t lds = (size_t)2 * kmax * kmax * sizeof(zd);
+    hipLaunchKernelGGL(k_gram_svd_asc, dim3(batch), dim3(SMALL_THREADS), lds, st, Gram, sGram, kk,
+                       kmax, shift_scale, V, sV, sig, sS);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Glue kernels of the recycled Rayleigh-Ritz and the recycle-space update
+// ---------------------------------------------------------------------------------------
+
+// Hz_b = H1 - z_b H2 - conj(z_b) H2^H + (|z_b|^2 + shift) I      (affine_conbine!,
+// fastRaylRitz.jl:6-28 with H3 = I since S^H S = I, recySpace.jl:394-399)
+__global__ void k_form_hz(const zd* __restrict__ H1, const zd* __restrict__ H2, int64_t ldh, int pr,
+                          const zd* __restrict__ z, double shift, zd* __restrict__ Hz, int64_t sHz) {
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= pr * pr) return;
+    const int i = idx % pr, j = idx / pr;
+    const zd zb = z[b];
+    zd v = H1[i + ldh * j];
+    v = zsub(v, zmul(zb, H2[i + ldh * j]));
+    v = zsub(v, zmul(zconj(zb), zconj(H2[j + ldh * i])));
+    if (i == j) v.x += zabs2(zb) + shift;
+    Hz[(int64_t)b * sHz + i + (int64_t)pr * j] = v;
+}
+
+void launch_form_hz(hipStream_t st, const zd* H1, const zd* H2, int64_t ldh, int pr, const zd* z,
+                    double shift, zd* Hz, int64_t sHz, int batch) {
+    hipLaunchKernelGGL(k_form_hz, dim3(cdiv((int64_t)pr * pr, 256), batch), dim3(256), 0, st, H1, H2,
+                       ldh, pr, z, shift, Hz, sHz);
+    LAUNCH_CHECK();
+}
+
+// Coefficients of CzX_ = (J5 - z V) G1 = [J5 V] [G1; -z G1] for shift b, written at column
+// offset coff[b] of a (2pr x ktot) matrix C1 (ld 2pr); G1 = first rr[b] columns of G_b.
+__global__ void k_recy_coef1(const zd* __restrict__ G, int64_t ldg, int64_t sG, const int* __restrict__ rr,
+                             const int* __restrict__ coff, const zd* __restrict__ z, int pr,
+                             zd* __restrict__ C1) {
+    const int b = blockIdx.y;
+    const int k = rr[b];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= pr * k) return;
+    const int i = idx % pr, j = idx / pr;
+    const zd g = G[(int64_t)b * sG + i + ldg * j];
+    zd* c = C1 + (int64_t)(2 * pr) * (coff[b] + j);
+    c[i] = g;
+    c[pr + i] = zneg(zmul(z[b], g));
+}
+
+void launch_recy_coef1(hipStream_t st, const zd* G, int64_t ldg, int64_t sG, const int* rr,
+                       const int* coff, const zd* z, int pr, int rmax, zd* C1, int batch) {
+    hipLaunchKernelGGL(k_recy_coef1, dim3(cdiv((int64_t)pr * rmax, 256), batch), dim3(256), 0, st, G, ldg,
+                       sG, rr, coff, z, pr, C1);
+    LAUNCH_CHECK();
+}
+
+// Per shift b (one workgroup): G_total = G1 G2 (r_ < pr) or G2 (r_ == pr, as the reference
+// does, fastRaylRitz.jl:196-202), and the coefficients of
+//   W = (J1 - z J2 - conj(z) J5 + |z|^2 V) G_total - V G_total diag(sig^2)
+//     = [J1 J2 J5 V] [G; -z G; -conj(z) G; G (|z|^2 - sig^2)]       (fastRaylRitz.jl:203-216)
+// written at columns b*r of CW (4pr x batch*r, ld 4pr) and G_total at columns b*r of CG
+// (pr x batch*r, ld pr).
+__global__ void __launch_bounds__(256)
+k_recy_coef2(const zd* __restrict__ G, int64_t ldg, int64_t sG, const int* __restrict__ rr,
+             const zd* __restrict__ G2, int64_t ldg2, int64_t sG2, const double* __restrict__ sig,
+             int64_t sS, const zd* __restrict__ z, int pr, int r, zd* __restrict__ CW,
+             zd* __restrict__ CG) {
+    const int b = blockIdx.x;
+    const int k = rr[b];
+    const zd zb = z[b];
+    const double az2 = zabs2(zb);
+    const zd* Gb = G + (int64_t)b * sG;
+    const zd* G2b = G2 + (int64_t)b * sG2;
+    for (int idx = threadIdx.x; idx < pr * r; idx += blockDim.x) {
+        const int i = idx % pr, j = idx / pr;
+        zd g;
+        if (k < pr) {
+            g = zzero();
+            for (int m = 0; m < k; ++m) g = zfma(Gb[i + ldg * m], G2b[m + ldg2 * j], g);
+        } else {
+            g = G2b[i + ldg2 * j];
+        }
+        const double s2 = sig[(int64_t)b * sS + j] * sig[(int64_t)b * sS + j];
+        zd* c = CW + (int64_t)(4 * pr) * (b * r + j);
+        c[i] = g;
+        c[pr + i] = zneg(zmul(zb, g));
+        c[2 * pr + i] = zneg(zmul(zconj(zb), g));
+        c[3 * pr + i] = zscale(g, az2 - s2);
+        CG[(int64_t)pr * (b * r + j) + i] = g;
+    }
+}
+
+void launch_recy_coef2(hipStream_t st, const zd* G, int64_t ldg, int64_t sG, const int* rr,
+                       const zd* G2, int64_t ldg2, int64_t sG2, const double* sig, int64_t sS,
+                       const zd* z, int pr, int r, zd* CW, zd* CG, int batch) {
+    hipLaunchKernelGGL(k_recy_coef2, dim3(batch), dim3(256), 0, st, G, ldg, sG, rr, G2, ldg2, sG2, sig,
+                       sS, z, pr, r, CW, CG);
+    LAUNCH_CHECK();
+}
+
+// Full Householder QR of the 2r x r matrix Y in LDS; writes U = Q^H (2r x 2r) into Uo.
+__device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo, double* red) {
+    const int m = 2 * r;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < m * m; idx += blockDim.x) Q[idx] = (idx % m == idx / m) ? zmk(1.0, 0.0) : zzero();
+    __shared__ zd sv[32];
+    __shared__ zd s_tau;
+    __syncthreads();
+    for (int j = 0; j < r; ++j) {
+        // reflector for Y[j:m, j]
+        if (tid == 0) {
+            zd alpha = Y[j + m * j];
+            double xn2 = 0.0;
+            for (int i = j + 1; i < m; ++i) xn2 += zabs2(Y[i + m * j]);
+            if (xn2 == 0.0 && alpha.y == 0.0) {
+                s_tau = zzero();
+                for (int i = 0; i < m; ++i) sv[i] = (i == j) ? zmk(1.0, 0.0) : zzero();
+            } else {
+                double nrm = sqrt(zabs2(alpha) + xn2);
+                double beta = alpha.x >= 0 ? -nrm : nrm;
+                s_tau = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+                zd sc = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+                for (int i = 0; i < m; ++i) sv[i] = (i < j) ? zzero() : (i == j ? zmk(1.0, 0.0) : zmul(Y[i + m * j], sc));
+            }
+        }
+        __syncthreads();
+        const zd t = s_tau;
+        // Y <- H^H Y for columns >= j  (H = I - t v v^H)
+        for (int c = j + tid; c < r; c += blockDim.x) {
+            zd s = zzero();
+            for (int i = j; i < m; ++i) s = zcfma(sv[i], Y[i + m * c], s);
+            s = zmul(zconj(t), s);
+            for (int i = j; i < m; ++i) Y[i + m * c] = zsub(Y[i + m * c], zmul(sv[i], s));
+        }
+        // Q <- Q H (accumulate rows)
+        for (int row = tid; row < m; row += blockDim.x) {
+            zd s = zzero();
+            for (int i = j; i < m; ++i) s = zfma(Q[row + m * i], sv[i], s);
+            s = zmul(s, t);
+            for (int i = j; i < m; ++i) Q[row + m * i] = zsub(Q[row + m * i], zmul(s, zconj(sv[i])));
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+        const int a = idx % m, c = idx / m;
+        Uo[a + m * c] = zconj(Q[c + m * a]);
+    }
+    __syncthreads();
+    (void)red;
+}
+
+// Small side of _block_unitary_update! (recySpace.jl:226-286): for i = 0..p-2, U_i from the
+// QR of [L[pb,pb]; L[i,pb]] (rls_block_unitary_rot!, RLSLinearAlgebra.jl:164-191), then
+// L rows (pb,i) <- U_i [L_pb; L_i], H <- U_i H U_i^H on blocks (pb,i) for H1 and H2.
+// Writes the U_i sequence to Uout (p-1 matrices of 2r x 2r) for the column pass.
+__global__ void __launch_bounds__(256)
+k_block_rot_small(zd* __restrict__ L, zd* __restrict__ H1, zd* __restrict__ H2, int64_t ld, int p,
+                  int r, zd* __restrict__ Uout) {
+    __shared__ zd Y[32 * 16];
+    __shared__ zd Q[32 * 32];
+    __shared__ double red[4];
+    const int tid = threadIdx.x;
+    const int pb = p - 1;
+    const int pr = p * r;
+    const int m = 2 * r;
+    for (int i = 0; i < pb; ++i) {
+        for (int idx = tid; idx < m * r; idx += blockDim.x) {
+            int a = idx % m, c = idx / m;
+            int row = (a < r) ? pb * r + a : i * r + (a - r);
+            Y[a + m * c] = L[row + ld * (pb * r + c)];
+        }
+        __syncthreads();
+        zd* U = Uout + (int64_t)i * m * m;
+        lds_block_unitary(Y, r, Q, U, red);
+        __syncthreads();
+        // lmul! on L rows (pb, i), all pr columns
+        for (int c = tid; c < pr; c += blockDim.x) {
+            zd a1[16], a2[16];
+            for (int s = 0; s < r; ++s) { a1[s] = L[pb * r + s + ld * c]; a2[s] = L[i * r + s + ld * c]; }
+            for (int t = 0; t < r; ++t) {
+                zd s1 = zzero(), s2 = zzero();
+                for (int s = 0; s < r; ++s) {
+                    s1 = zfma(U[t + m * s], a1[s], s1);
+                    s1 = zfma(U[t + m * (r + s)], a2[s], s1);
+                    s2 = zfma(U[(r + t) + m * s], a1[s], s2);
+                    s2 = zfma(U[(r + t) + m * (r + s)], a2[s], s2);
+                }
+                L[pb * r + t + ld * c] = s1;
+                L[i * r + t + ld * c] = s2;
+            }
+        }
+        __syncthreads();
+        for (int hsel = 0; hsel < 2; ++hsel) {
+            zd* H = hsel == 0 ? H1 : H2;
+            // rmul!(H, adjoint(bu)): columns (pb, i) <- [H_pb H_i] U^H
+            for (int row = tid; row < pr; row += blockDim.x) {
+                zd a1[16], a2[16];
+                for (int s = 0; s < r; ++s) { a1[s] = H[row + ld * (pb * r + s)]; a2[s] = H[row + ld * (i * r + s)]; }
+                for (int t = 0; t < r; ++t) {
+                    zd s1 = zzero(), s2 = zzero();
+                    for (int s = 0; s < r; ++s) {
+                        s1 = zfma(a1[s], zconj(U[t + m * s]), s1);
+                        s1 = zfma(a2[s], zconj(U[t + m * (r + s)]), s1);
+                        s2 = zfma(a1[s], zconj(U[(r + t) + m * s]), s2);
+                        s2 = zfma(a2[s], zconj(U[(r + t) + m * (r + s)]), s2);
+                    }
+                    H[row + ld * (pb * r + t)] = s1;
+                    H[row + ld * (i * r + t)] = s2;
+                }
+            }
+            __syncthreads();
+            // lmul!(bu, H): rows (pb, i) <- U [H_pb; H_i]
+            for (int c = tid; c < pr; c += blockDim.x) {
+                zd a1[16], a2[16];
+                for (int s = 0; s < r; ++s) { a1[s] = H[pb * r + s + ld * c]; a2[s] = H[i * r + s + ld * c]; }
+                for (int t = 0; t < r; ++t) {
+                    zd s1 = zzero(), s2 = zzero();
+                    for (int s = 0; s < r; ++s) {
+                        s1 = zfma(U[t + m * s], a1[s], s1);
+                        s1 = zfma(U[t + m * (r + s)], a2[s], s1);
+                        s2 = zfma(U[(r + t) + m * s], a1[s], s2);
+                        s2 = zfma(U[(r + t) + m * (r + s)], a2[s], s2);
+                    }
+                    H[pb * r + t + ld * c] = s1;
+                    H[i * r + t + ld * c] = s2;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, int p, int r, zd* U) {
+    if (p <= 1) return;
+    if (r > 16) throw RlsError{-1, "block rotation: r > 16 unsupported"};
+    hipLaunchKernelGGL(k_block_rot_small, dim3(1), dim3(256), 0, st, L, H1, H2, ld, p, r, U);
+    LAUNCH_CHECK();
+}
+
+// D[i + ldd*j] = conj(S[j + lds*i])  (m x k block of D from the k x m block of S)
+__global__ void k_conj_transpose(int m, int k, const zd* __restrict__ S, int64_t lds,
+                                 zd* __restrict__ D, int64_t ldd) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * k) return;
+    const int i = idx % m, j = idx / m;
+    D[i + ldd * j] = zconj(S[j + lds * i]);
+}
+
+void launch_conj_transpose(hipStream_t st, int m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd) {
+    hipLaunchKernelGGL(k_conj_transpose, dim3(cdiv((int64_t)m * k, 256)), dim3(256), 0, st, m, k, S, lds,
+                       D, ldd);
+    LAUNCH_CHECK();
+}
+
+// Generic strided 2-D copy of an m x k block (ld src / dst), used for block moves.
+__global__ void k_copy2d(int64_t m, int k, const zd* __restrict__ S, int64_t lds, zd* __restrict__ D,
+                         int64_t ldd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) D[i + ldd * j] = S[i + lds * j];
+}
+
+void launch_copy2d(hipStream_t st, int64_t m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd) {
+    if (m <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_copy2d, dim3(cdiv(m, 256), k), dim3(256), 0, st, m, k, S, lds, D, ldd);
+    LAUNCH_CHECK();
+}
+
+// Y = -X (m x k) ; used to negate Gram-Schmidt coefficients
+__global__ void k_neg2d(int m, int k, const zd* __restrict__ S, int64_t lds, zd* __restrict__ D, int64_t ldd) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * k) return;
+    const int i = idx % m, j = idx / m;
+    D[i + ldd * j] = zneg(S[i + lds * j]);
+}
+
+void launch_neg2d(hipStream_t st, int m, int k, const zd* S, int64_t lds, zd* D, int64_t ldd) {
+    hipLaunchKernelGGL(k_neg2d, dim3(cdiv((int64_t)m * k, 256)), dim3(256), 0, st, m, k, S, lds, D, ldd);
+    LAUNCH_CHECK();
+}
+
+// Small complex GEMM C (m x n) = alpha A (m x k) B (k x n) + beta C, one workgroup (small sizes).
+__global__ void k_small_gemm(int m, int n, int k, const zd* __restrict__ A, int64_t lda,
+                             const zd* __restrict__ B, int64_t ldb, zd* __restrict__ C, int64_t ldc,
+                             double beta) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < m * n; idx += gridDim.x * blockDim.x) {
+        const int i = idx % m, j = idx / m;
+        zd s = zzero();
+        for (int q = 0; q < k; ++q) s = zfma(A[i + lda * q], B[q + ldb * j], s);
+        C[i + ldc * j] = (beta != 0.0) ? zadd(zscale(C[i + ldc * j], beta), s) : s;
+    }
+}
+
+void launch_small_gemm(hipStream_t st, int m, int n, int k, const zd* A, int64_t lda, const zd* B,
+                       int64_t ldb, zd* C, int64_t ldc, double beta) {
+    hipLaunchKernelGGL(k_small_gemm, dim3(cdiv((int64_t)m * n, 256)), dim3(256), 0, st, m, n, k, A, lda,
+                       B, ldb, C, ldc, beta);
+    LAUNCH_CHECK();
+}
+
+// R = Rinv = I when the input of this Cholesky-QR pass was already orthonormal
+// (||X^H X - I||_F < tol): the pass then leaves X unchanged, as the early exit of
+// _rls_qr_shiftchol! does (RLSLinearAlgebra.jl:779-782).
+__global__ void k_identity_if_orth(int k, const double* __restrict__ offI, double tol,
+                                   zd* __restrict__ R, zd* __restrict__ Rinv) {
+    if (!(offI[0] < tol)) return;
+    for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
+        zd v = (idx % k == idx / k) ? zmk(1.0, 0.0) : zzero();
+        R[idx] = v;
+        Rinv[idx] = v;
+    }
+}
+
+void launch_identity_if_orth(hipStream_t st, int k, const double* offI, double tol, zd* R, zd* Rinv) {
+    hipLaunchKernelGGL(k_identity_if_orth, dim3(1), dim3(256), 0, st, k, offI, tol, R, Rinv);
+    LAUNCH_CHECK();
+}

@@ -1,0 +1,536 @@
+// Tall-skinny FP64 kernels over n-length columns (the HBM-streaming part of the solver):
+// multi-operand tall GEMM (recycle-space combines of fastRaylRitz.jl:183-216, basisUpdate!
+// lobpcgSvd.jl:195-220, Gram-Schmidt corrections RLSLinearAlgebra.jl:1141-1149), Gram
+// products A^H B, column norms, dense triangular multiply / shifted solve with the Schur
+// factor (psTriRecy.jl:87-99, 797 via RLSMatrices.jl:390-436), and the column side of the
+// block unitary rotations (recySpace.jl:239-280).
+#include "rls_common.h"
+#include "rls_kernels.h"
+
+namespace {
+
+constexpr int TG_ROWS = 64;      // rows per workgroup (one per lane)
+constexpr int TG_WAVES = 4;      // waves split the contraction dimension
+
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_tall_gemm(int64_t n, TallOperands ops, const zd* __restrict__ B, int64_t ldb, int64_t sB, int k,
+            zd* __restrict__ Y, int64_t ldy, int64_t sY, double beta, zd* __restrict__ Y2,
+            int64_t ldy2, int64_t sY2, int q2, const zd* __restrict__ B2, int64_t ldb2) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t row = (int64_t)blockIdx.x * TG_ROWS + lane;
+    const int c0 = blockIdx.y * KC;
+    const int kc = min(KC, k - c0);
+    const int b = blockIdx.z;
+    const zd* Bb = B + (int64_t)b * sB + ldb * c0;
+    const zd* B2b = B2 ? B2 + (int64_t)b * sB + ldb2 * c0 : nullptr;
+    zd acc[KC], acc2[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) { acc[j] = zzero(); acc2[j] = zzero(); }
+    const bool valid = row < n;
+    int off = 0;
+    for (int q = 0; q < ops.nops; ++q) {
+        const zd* A = ops.A[q];
+        const int64_t lda = ops.lda[q];
+        const int Kq = ops.K[q];
+        const bool second = (q == q2) && (Y2 != nullptr);
+        for (int kk = w; kk < Kq; kk += TG_WAVES) {
+            zd a = valid ? A[row + lda * kk] : zzero();
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) acc[j] = zfma(a, Bb[off + kk + ldb * j], acc[j]);
+            if (second) {
+#pragma unroll
+                for (int j = 0; j < KC; ++j)
+                    if (j < kc) acc2[j] = zfma(a, B2b[kk + ldb2 * j], acc2[j]);
+            }
+        }
+        off += Kq;
+    }
+    __shared__ zd red[TG_WAVES - 1][TG_ROWS][KC];
+    const bool two = (Y2 != nullptr);
+    for (int pass = 0; pass < (two ? 2 : 1); ++pass) {
+        zd* ac = pass == 0 ? acc : acc2;
+        if (w > 0) {
+#pragma unroll
+            for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = ac[j];
+        }
+        __syncthreads();
+        if (w == 0 && valid) {
+            for (int j = 0; j < kc; ++j) {
+                zd s = ac[j];
+                for (int v = 0; v < TG_WAVES - 1; ++v) s = zadd(s, red[v][lane][j]);
+                if (pass == 0) {
+                    zd* y = Y + (int64_t)b * sY + row + ldy * (c0 + j);
+                    *y = (beta != 0.0) ? zadd(zscale(*y, beta), s) : s;
+                } else {
+                    Y2[(int64_t)b * sY2 + row + ldy2 * (c0 + j)] = s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Gram partials: part[chunk][a + ka*b] = sum_{rows in chunk} conj(A[i,a]) B[i,b]
+constexpr int GR_T = 32;
+__global__ void __launch_bounds__(256)
+k_gram_partial(int64_t n, int64_t rows_per_chunk, const zd* __restrict__ A, int64_t lda, int ka,
+               const zd* __restrict__ B, int64_t ldb, int kb, zd* __restrict__ part) {
+    __shared__ zd As[64][GR_T + 1];
+    __shared__ zd Bs[64][GR_T + 1];
+    const int tid = threadIdx.x;
+    const int a0 = blockIdx.y * GR_T, b0 = blockIdx.z * GR_T;
+    const int kat = min(GR_T, ka - a0), kbt = min(GR_T, kb - b0);
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = min(n, r0 + rows_per_chunk);
+    zd acc[4];
+    for (int e = 0; e < 4; ++e) acc[e] = zzero();
+    for (int64_t t0 = r0; t0 < r1; t0 += 64) {
+        const int rows = (int)min((int64_t)64, r1 - t0);
+        for (int idx = tid; idx < 64 * GR_T; idx += 256) {
+            int i = idx & 63, c = idx >> 6;
+            As[i][c] = (i < rows && c < kat) ? A[t0 + i + lda * (a0 + c)] : zzero();
+            Bs[i][c] = (i < rows && c < kbt) ? B[t0 + i + ldb * (b0 + c)] : zzero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;
+            int a = idx % GR_T, bb = idx / GR_T;
+            if (a < kat && bb < kbt) {
+                zd s = acc[e];
+                for (int i = 0; i < rows; ++i) s = zcfma(As[i][a], Bs[i][bb], s);
+                acc[e] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int idx = tid + 256 * e;
+        int a = idx % GR_T, bb = idx / GR_T;
+        if (a < kat && bb < kbt)
+            part[(int64_t)blockIdx.x * ka * kb + (a0 + a) + (int64_t)ka * (b0 + bb)] = acc[e];
+    }
+}
+
+__global__ void k_gram_reduce(int nchunk, int ka, int kb, const zd* __restrict__ part,
+                              zd* __restrict__ C, int64_t ldc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ka * kb) return;
+    const int a = idx % ka, b = idx / ka;
+    zd s = zzero();
+    for (int c = 0; c < nchunk; ++c) s = zadd(s, part[(int64_t)c * ka * kb + idx]);
+    C[a + ldc * b] = s;
+}
+
+__global__ void __launch_bounds__(256)
+k_colnorm2_partial(int64_t n, int64_t rows_per_chunk, const zd* __restrict__ A, int64_t lda, int k,
+                   double* __restrict__ part) {
+    __shared__ double red[4];
+    const int j = blockIdx.y;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = min(n, r0 + rows_per_chunk);
+    double s = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) s += zabs2(A[i + lda * j]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * k + j] = s;
+}
+
+__global__ void k_colnorm2_reduce(int nchunk, int k, const double* __restrict__ part,
+                                  double* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * k + j];
+    out[j] = s;
+}
+
+__global__ void k_scale_sub(int64_t n, zd* __restrict__ Y, int64_t ldy, const zd* __restrict__ X,
+                            int64_t ldx, const double* __restrict__ s, const double* __restrict__ t,
+                            int mode) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= n) return;
+    double sj = s ? s[j] : 1.0, tj = t ? t[j] : 0.0;
+    if (mode == 1) { tj = 1.0; }         // Y = Y*s - X      (residual, side 'L')
+    else if (mode == 2) { tj = sj; sj = 1.0; }   // Y = Y - X*s      (residual, side 'S')
+    zd y = Y[i + ldy * j];
+    zd x = X[i + ldx * j];
+    Y[i + ldy * j] = zsub(zscale(y, sj), zscale(x, tj));
+}
+
+// Y = T X for upper-triangular T: lane = row, waves split the column range.
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ X,
+             int64_t ldx, int k, zd* __restrict__ Y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int64_t row = r0 + lane;
+    const int c0 = blockIdx.y * KC;
+    const int kc = min(KC, k - c0);
+    zd acc[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[j] = zzero();
+    const bool valid = row < n;
+    for (int64_t col = r0 + w; col < n; col += 4) {
+        zd t = (valid && col >= row) ? T[row + ldt * col] : zzero();
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+            if (j < kc) acc[j] = zfma(t, X[col + ldx * (c0 + j)], acc[j]);
+    }
+    __shared__ zd red[3][64][KC];
+    if (w > 0) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = acc[j];
+    }
+    __syncthreads();
+    if (w == 0 && valid) {
+        for (int j = 0; j < kc; ++j) {
+            zd s = acc[j];
+            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
+            Y[row + ldy * (c0 + j)] = s;
+        }
+    }
+}
+
+// Y = T^H X: one wave per output row i (= column i of T), lanes over j <= i.
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_trmmh_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ X,
+              int64_t ldx, int k, zd* __restrict__ Y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int c0 = blockIdx.y * KC;
+    const int kc = min(KC, k - c0);
+    if (i >= n) return;
+    zd acc[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[j] = zzero();
+    const zd* Tc = T + ldt * i;
+    for (int64_t r = lane; r <= i; r += 64) {
+        zd t = Tc[r];
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+            if (j < kc) acc[j] = zcfma(t, X[r + ldx * (c0 + j)], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        double re = wave_sum(acc[j].x), im = wave_sum(acc[j].y);
+        if (lane == 0 && j < kc) Y[i + ldy * (c0 + j)] = zmk(re, im);
+    }
+}
+
+// Inverse of each 64 x 64 diagonal block of T - zI (upper triangular).
+__global__ void __launch_bounds__(64)
+k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __restrict__ Dinv) {
+    __shared__ zd D[64][65];
+    const int jb = blockIdx.x;
+    const int64_t o = (int64_t)jb * 64;
+    const int m = (int)min((int64_t)64, n - o);
+    const int c = threadIdx.x;
+    for (int a = 0; a < 64; ++a) {
+        zd v;
+        if (a < m && c < m) {
+            v = (a <= c) ? T[(o + a) + ldt * (o + c)] : zzero();
+            if (a == c) v = zsub(v, z);
+        } else {
+            v = (a == c) ? zmk(1.0, 0.0) : zzero();
+        }
+        D[a][c] = v;
+    }
+    __syncthreads();
+    // column c of the inverse: back substitution
+    zd x[64];
+    for (int a = 0; a < 64; ++a) x[a] = zzero();
+    for (int a = c; a >= 0; --a) {
+        zd s = (a == c) ? zmk(1.0, 0.0) : zzero();
+        for (int q = a + 1; q <= c; ++q) s = zsub(s, zmul(D[a][q], x[q]));
+        x[a] = zdiv(s, D[a][a]);
+    }
+    zd* out = Dinv + (int64_t)jb * 64 * 64;
+    for (int a = 0; a < 64; ++a) out[a + 64 * c] = x[a];
+}
+
+// One step of the blocked back substitution for (T - zI) X = B (upper):
+// x_jb = Dinv_jb * X_jb (into tmp), and X_i -= T[i, jb] x_jb for the row block of this WG.
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_trsm_u_step(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
+              zd* __restrict__ X, int64_t ldx, int k, zd* __restrict__ tmp, int jb) {
+    __shared__ zd xs[64][KC];
+    const int tid = threadIdx.x;
+    const int64_t o = (int64_t)jb * 64;
+    const int m = (int)min((int64_t)64, n - o);
+    const zd* Di = Dinv + (int64_t)jb * 4096;
+    // x_jb
+    for (int idx = tid; idx < 64 * KC; idx += 256) {
+        int a = idx & 63, j = idx >> 6;
+        zd s = zzero();
+        if (a < m && j < k) {
+            for (int c = a; c < m; ++c) s = zfma(Di[a + 64 * c], X[(o + c) + ldx * j], s);
+        }
+        xs[a][j] = s;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int idx = tid; idx < 64 * KC; idx += 256) {
+            int a = idx & 63, j = idx >> 6;
+            if (a < m && j < k) tmp[(o + a) + ldx * j] = xs[a][j];
+        }
+    }
+    if (jb == 0) return;
+    // update row block ib = blockIdx.x (< jb)
+    const int ib = blockIdx.x;
+    const int64_t ro = (int64_t)ib * 64;
+    const int lane = tid & 63, w = tid >> 6;
+    zd acc[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[j] = zzero();
+    for (int c = w; c < m; c += 4) {
+        zd t = T[(ro + lane) + ldt * (o + c)];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zfma(t, xs[c][j], acc[j]);
+    }
+    __shared__ zd red[3][64][KC];
+    if (w > 0) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = acc[j];
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int j = 0; j < k; ++j) {
+            zd s = acc[j];
+            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
+            zd* xp = X + (ro + lane) + ldx * j;
+            *xp = zsub(*xp, s);
+        }
+    }
+}
+
+// One step of the blocked forward substitution for (T - zI)^H X = B:
+// x_jb = Dinv_jb^H * X_jb (into tmp), and X_i -= (T[jb, i])^H x_jb for row block i > jb.
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_trsm_h_step(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
+              zd* __restrict__ X, int64_t ldx, int k, zd* __restrict__ tmp, int jb, int nblk) {
+    __shared__ zd xs[64][KC];
+    const int tid = threadIdx.x;
+    const int64_t o = (int64_t)jb * 64;
+    const int m = (int)min((int64_t)64, n - o);
+    const zd* Di = Dinv + (int64_t)jb * 4096;
+    for (int idx = tid; idx < 64 * KC; idx += 256) {
+        int a = idx & 63, j = idx >> 6;
+        zd s = zzero();
+        if (a < m && j < k) {
+            // (Dinv^H)[a, c] = conj(Dinv[c, a]), nonzero for c <= a
+            for (int c = 0; c <= a; ++c) s = zcfma(Di[c + 64 * a], X[(o + c) + ldx * j], s);
+        }
+        xs[a][j] = s;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int idx = tid; idx < 64 * KC; idx += 256) {
+            int a = idx & 63, j = idx >> 6;
+            if (a < m && j < k) tmp[(o + a) + ldx * j] = xs[a][j];
+        }
+    }
+    const int ib = jb + 1 + blockIdx.x;
+    if (ib >= nblk) return;
+    const int64_t ro = (int64_t)ib * 64;
+    const int mi = (int)min((int64_t)64, n - ro);
+    const int lane = tid & 63, w = tid >> 6;
+    zd acc[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[j] = zzero();
+    // X_i[a] -= sum_c conj(T[o + c, ro + a]) x[c]
+    if (lane < mi) {
+        const zd* Tc = T + ldt * (ro + lane) + o;
+        for (int c = w; c < m; c += 4) {
+            zd t = Tc[c];
+#pragma unroll
+            for (int j = 0; j < KC; ++j) acc[j] = zcfma(t, xs[c][j], acc[j]);
+        }
+    }
+    __shared__ zd red[3][64][KC];
+    if (w > 0) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = acc[j];
+    }
+    __syncthreads();
+    if (w == 0 && lane < mi) {
+        for (int j = 0; j < k; ++j) {
+            zd s = acc[j];
+            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
+            zd* xp = X + (ro + lane) + ldx * j;
+            *xp = zsub(*xp, s);
+        }
+    }
+}
+
+__global__ void k_copy_cols(int64_t n, const zd* __restrict__ S, int64_t lds, zd* __restrict__ D,
+                            int64_t ldd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n) D[i + ldd * j] = S[i + lds * j];
+}
+
+// Column side of the block unitary rotations, one thread per row of one array.
+constexpr int ROT_RMAX = 16;
+struct RotArrays { zd* a[8]; };
+__global__ void __launch_bounds__(256)
+k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int r, int p, const zd* __restrict__ U) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    zd* A = arr.a[blockIdx.y] + row;
+    const int pb = p - 1;
+    zd ap[ROT_RMAX], ai[ROT_RMAX], np_[ROT_RMAX];
+    for (int c = 0; c < r; ++c) ap[c] = A[ld * (pb * r + c)];
+    const int r2 = 2 * r;
+    for (int i = 0; i < pb; ++i) {
+        const zd* Ui = U + (int64_t)i * r2 * r2;   // U (2r x 2r), we apply U^H from the right
+        for (int c = 0; c < r; ++c) ai[c] = A[ld * (i * r + c)];
+        // [np ni] = [ap ai] * U^H ; (U^H)[s, t] = conj(U[t, s])
+        for (int t = 0; t < r; ++t) {
+            zd s1 = zzero(), s2 = zzero();
+            for (int s = 0; s < r; ++s) {
+                s1 = zfma(ap[s], zconj(Ui[t + r2 * s]), s1);
+                s1 = zfma(ai[s], zconj(Ui[t + r2 * (r + s)]), s1);
+                s2 = zfma(ap[s], zconj(Ui[(r + t) + r2 * s]), s2);
+                s2 = zfma(ai[s], zconj(Ui[(r + t) + r2 * (r + s)]), s2);
+            }
+            np_[t] = s1;
+            A[ld * (i * r + t)] = s2;
+        }
+        for (int c = 0; c < r; ++c) ap[c] = np_[c];
+    }
+    for (int c = 0; c < r; ++c) A[ld * (pb * r + c)] = ap[c];
+}
+
+}  // namespace
+
+void launch_tall_gemm(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
+                      int64_t sB, int k, zd* Y, int64_t ldy, int64_t sY, double beta, int batch,
+                      zd* Y2, int64_t ldy2, int64_t sY2, int q2, const zd* B2, int64_t ldb2) {
+    if (k <= 0 || n <= 0 || batch <= 0) return;
+    const int gx = cdiv(n, TG_ROWS);
+    if (k <= 8) {
+        hipLaunchKernelGGL(k_tall_gemm<8>, dim3(gx, 1, batch), dim3(256), 0, st, n, ops, B, ldb, sB,
+                           k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2);
+    } else {
+        hipLaunchKernelGGL(k_tall_gemm<16>, dim3(gx, cdiv(k, 16), batch), dim3(256), 0, st, n, ops,
+                           B, ldb, sB, k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2);
+    }
+    LAUNCH_CHECK();
+}
+
+static int gram_chunks(int64_t n) {
+    int64_t c = (n + 63) / 64;
+    return (int)(c < 256 ? c : 256);
+}
+
+size_t gram_partial_elems(int64_t n, int ka, int kb) {
+    return (size_t)gram_chunks(n) * ka * kb;
+}
+
+void launch_gram(hipStream_t st, int64_t n, const zd* A, int64_t lda, int ka, const zd* B,
+                 int64_t ldb, int kb, zd* C, int64_t ldc, zd* part) {
+    const int nch = gram_chunks(n);
+    int64_t rpc = ((n + nch - 1) / nch + 63) / 64 * 64;
+    const int nch_eff = (int)((n + rpc - 1) / rpc);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nch_eff, cdiv(ka, GR_T), cdiv(kb, GR_T)), dim3(256), 0,
+                       st, n, rpc, A, lda, ka, B, ldb, kb, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gram_reduce, dim3(cdiv(ka * kb, 256)), dim3(256), 0, st, nch_eff, ka, kb,
+                       part, C, ldc);
+    LAUNCH_CHECK();
+}
+
+void launch_colnorm2(hipStream_t st, int64_t n, const zd* A, int64_t lda, int k, double* out,
+                     zd* part) {
+    const int nch = gram_chunks(n);
+    int64_t rpc = ((n + nch - 1) / nch + 63) / 64 * 64;
+    const int nch_eff = (int)((n + rpc - 1) / rpc);
+    double* dp = reinterpret_cast<double*>(part);
+    hipLaunchKernelGGL(k_colnorm2_partial, dim3(nch_eff, k), dim3(256), 0, st, n, rpc, A, lda, k, dp);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colnorm2_reduce, dim3(cdiv(k, 256)), dim3(256), 0, st, nch_eff, k, dp, out);
+    LAUNCH_CHECK();
+}
+
+void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X, int64_t ldx,
+                      int k, const double* s, const double* t, int mode) {
+    hipLaunchKernelGGL(k_scale_sub, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, Y, ldy, X, ldx, s, t,
+                       mode);
+    LAUNCH_CHECK();
+}
+
+void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
+                       int64_t ldx, int k, zd* Y, int64_t ldy) {
+    if (k <= 8)
+        hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), 1), dim3(256), 0, st, n, T, ldt, X, ldx,
+                           k, Y, ldy);
+    else
+        hipLaunchKernelGGL(k_trmm_upper<16>, dim3(cdiv(n, 64), cdiv(k, 16)), dim3(256), 0, st, n, T,
+                           ldt, X, ldx, k, Y, ldy);
+    LAUNCH_CHECK();
+}
+
+void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
+                        int64_t ldx, int k, zd* Y, int64_t ldy) {
+    if (k <= 8)
+        hipLaunchKernelGGL(k_trmmh_upper<8>, dim3(cdiv(n, 4), 1), dim3(256), 0, st, n, T, ldt, X, ldx,
+                           k, Y, ldy);
+    else
+        hipLaunchKernelGGL(k_trmmh_upper<16>, dim3(cdiv(n, 4), cdiv(k, 16)), dim3(256), 0, st, n, T,
+                           ldt, X, ldx, k, Y, ldy);
+    LAUNCH_CHECK();
+}
+
+void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* Dinv) {
+    hipLaunchKernelGGL(k_tri_diag_inv, dim3(cdiv(n, 64)), dim3(64), 0, st, n, T, ldt, z, Dinv);
+    LAUNCH_CHECK();
+}
+
+void launch_trsm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
+                       zd* X, int64_t ldx, int k, zd* tmp) {
+    if (k > 8) throw RlsError{-1, "trsm: k > 8 unsupported"};
+    const int nblk = cdiv(n, 64);
+    for (int jb = nblk - 1; jb >= 0; --jb) {
+        hipLaunchKernelGGL(k_trsm_u_step<8>, dim3(jb > 0 ? jb : 1), dim3(256), 0, st, n, T, ldt, Dinv,
+                           X, ldx, k, tmp, jb);
+    }
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_cols, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, tmp, ldx, X, ldx);
+    LAUNCH_CHECK();
+}
+
+void launch_trsm_upper_h(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
+                         zd* X, int64_t ldx, int k, zd* tmp) {
+    if (k > 8) throw RlsError{-1, "trsm: k > 8 unsupported"};
+    const int nblk = cdiv(n, 64);
+    for (int jb = 0; jb < nblk; ++jb) {
+        int below = nblk - 1 - jb;
+        hipLaunchKernelGGL(k_trsm_h_step<8>, dim3(below > 0 ? below : 1), dim3(256), 0, st, n, T, ldt,
+                           Dinv, X, ldx, k, tmp, jb, nblk);
+    }
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_cols, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, tmp, ldx, X, ldx);
+    LAUNCH_CHECK();
+}
+
+void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int narr, int64_t ld,
+                           int r, int p, const zd* U) {
+    if (p <= 1) return;
+    if (r > ROT_RMAX) throw RlsError{-1, "block rotation: r > 16 unsupported"};
+    if (narr > 8) throw RlsError{-1, "block rotation: too many arrays"};
+    RotArrays ra;
+    for (int i = 0; i < narr; ++i) ra.a[i] = arrays[i];
+    hipLaunchKernelGGL(k_block_rot_cols, dim3(cdiv(n, 256), narr), dim3(256), 0, st, n, ra, ld, r, p,
+                       U);
+    LAUNCH_CHECK();
+}
