@@ -54,6 +54,7 @@ typedef struct rls_tri_control {
     int32_t mode;                     /* RLS_MODE_PLAIN_FP64 or RLS_MODE_BATCHED */
     int32_t window;                   /* max shifts per speculative window (batched mode), 0 = auto */
     uint64_t seed;                    /* seed of the initial block (reference: rand!(X)) */
+    int32_t verbose;                  /* >= 1: progress lines on stderr every ~5% of the grid */
 } rls_tri_control;
 
 #define RLS_MODE_PLAIN_FP64 0   /* one shift at a time, FP64 everywhere                        */

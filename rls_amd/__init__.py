@@ -148,6 +148,7 @@ class PsTriRecyControl:
     window: int = 0
     seed: int = 0
     maxit: int = 2000
+    progress: int = 0
 
     def __post_init__(self):
         if self.shift is None:
@@ -246,6 +247,7 @@ def _c_control(control: PsTriRecyControl, rec: Recursion) -> _capi.TriControl:
     c.mode = _capi.RLS_MODE_BATCHED if control.mode == "batched" else _capi.RLS_MODE_PLAIN_FP64
     c.window = control.window
     c.seed = control.seed
+    c.verbose = int(control.progress)
     return c
 
 
@@ -280,6 +282,22 @@ class TriSolver:
                                n_it=st.n_it, n_pts=st.n_pts, lobpcg_it=arrs["lobpcg_it"],
                                n_windows=st.n_windows, n_window_shifts=st.n_window_shifts)
         return sigma, stats
+
+
+def initial_block(n: int, r: int, seed: int = 0) -> np.ndarray:
+    """The library's start block (the reference's rand!(X), psTriRecy.jl:607): uniform [0,1)
+    real/imaginary parts from a counter-based splitmix64 stream, element e = i + n j uses
+    counters 2e and 2e+1.  Returned n x r, column-major order."""
+    e = np.arange(n * r, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        def u(cnt):
+            z = cnt + (np.uint64(seed) << np.uint64(32)) + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        vals = u(2 * e) + 1j * u(2 * e + 1)
+    return vals.reshape(r, n).T.copy()
 
 
 def _print_startup(io, rec: Recursion, control: PsTriRecyControl):

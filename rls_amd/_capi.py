@@ -49,6 +49,7 @@ class TriControl(C.Structure):
         ("mode", C.c_int32),
         ("window", C.c_int32),
         ("seed", C.c_uint64),
+        ("verbose", C.c_int32),
     ]
 
 

@@ -504,6 +504,14 @@ void rls_ctx::launch_chol_rinv_cond(double skip_tol, double shift_scale, int k) 
 // --------------------------------------------------------------------------------------
 namespace {
 
+double u01_splitmix(uint64_t seed, uint64_t counter) {
+    uint64_t z = counter + (seed << 32) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
 double resolve_fail_dist(const rls_tri_control& c) {
     if (std::isfinite(c.recycle_fail_dist_thresh)) {
         if (c.recycle_fail_dist_thresh < 0)
@@ -534,12 +542,13 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     for (int64_t i = 0; i < nZ; ++i) sigma[i] = std::numeric_limits<double>::quiet_NaN();
     auto set_stat = [&](int64_t* arr, int64_t i, int64_t v) { if (arr) arr[i] = v; };
 
-    // initial block: uniform [0,1) real and imaginary parts (rand!(X), psTriRecy.jl:607)
+    // initial block: uniform [0,1) real and imaginary parts (rand!(X), psTriRecy.jl:607),
+    // drawn from a counter-based splitmix64 stream so hosts can reproduce it exactly:
+    // element e = i + n j (column-major) uses counters 2e (re) and 2e+1 (im).
     {
-        std::mt19937_64 gen(c.seed);
-        std::uniform_real_distribution<double> U01(0.0, 1.0);
         std::vector<cplx> X0((size_t)n * r);
-        for (auto& v : X0) { double a = U01(gen); double b = U01(gen); v = cplx(a, b); }
+        for (size_t e = 0; e < X0.size(); ++e)
+            X0[e] = cplx(u01_splitmix(c.seed, 2 * e), u01_splitmix(c.seed, 2 * e + 1));
         ctx->h2d(ctx->S.p, reinterpret_cast<zd*>(X0.data()), X0.size());
     }
     int scaled_remaining = 0;
@@ -550,6 +559,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     const bool batched = (c.mode == RLS_MODE_BATCHED);
     int window = c.window > 0 ? c.window : 64;
     int run_hint = 8;
+    int64_t next_report = std::max<int64_t>(nZ / 20, 1);
 
     int64_t i = 0;
     while (i < nZ) {
@@ -625,6 +635,13 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
         set_stat(stats ? stats->M_vec_apps_no_recycle : nullptr, k, m_apps);
         set_stat(stats ? stats->R_vec_apps : nullptr, k, r_apps);
         i = k + 1;
+        if (c.verbose > 0 && i >= next_report) {
+            std::fprintf(stderr, "[psTriRecy] %lld/%lld refined=%lld it=%lld p=%d  recycle %.2fs refine %.2fs update %.2fs\n",
+                         (long long)i, (long long)nZ, (long long)n_pts, (long long)n_it, ctx->p,
+                         t_rec * 1e-9, t_ref * 1e-9, t_upd * 1e-9);
+            std::fflush(stderr);
+            next_report = i + std::max<int64_t>(nZ / 20, 1);
+        }
     }
     ctx->sync();
     ctx->lastX.resize((size_t)n * r);

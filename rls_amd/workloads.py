@@ -22,7 +22,8 @@ import scipy.sparse as sp
 
 def toeplitz_real(n: int) -> np.ndarray:
     k = np.arange(-(n - 1), n)
-    t = np.where(k < 0, ((-1.0) ** np.abs(k)) / (1.0 + np.abs(k)), 2.0 / (1.0 + k) ** 2)
+    ak = np.abs(k).astype(np.float64)
+    t = np.where(k < 0, ((-1.0) ** ak) / (1.0 + ak), 2.0 / (1.0 + ak) ** 2)
     i = np.arange(n)
     return t[(i[:, None] - i[None, :]) + (n - 1)].astype(np.float64)
 

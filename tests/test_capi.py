@@ -1,0 +1,77 @@
+"""C-ABI boundary checks that run without a GPU: the library builds, loads and exports every
+function declared in include/*.h; the ctypes structs match the C structs' layout."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import rls_amd
+from rls_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            src = open(os.path.join(inc, f)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(rls_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librls_hip.so not built")
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    decl = declared_functions()
+    assert decl, "no declarations found"
+    for name in decl:
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+    assert set(_capi.EXPORTED) <= set(decl)
+
+
+def test_control_defaults_match_reference():
+    """rls_tri_control_default mirrors PsTriRecyControl's defaults (psTriRecy.jl:350-370)."""
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librls_hip.so not built")
+    lib = _capi.load()
+    c = _capi.TriControl()
+    lib.rls_tri_control_default(ctypes.byref(c), 3.0)
+    py = rls_amd.PsTriRecyControl(normC=3.0)
+    assert c.tol == py.tol == 1e-3
+    assert c.shift == pytest.approx(py.shift) and py.shift == pytest.approx(9.0 * 2.220446049250313e-16 * 100)
+    assert (c.gap_tau, c.gap_tau_scale, c.recy_tau) == (py.gap_tau, py.gap_tau_scale, py.recy_tau)
+    assert (c.r, c.p, c.p_min) == (py.r, py.p, py.p_min) == (6, 50, 1)
+    assert c.lobpcg_tol_scale == py.lobpcg_tol_scale and c.lobpcg_side == b"L"
+    assert c.maxit == 2000 and c.implict_recycle == 1 and c.implict_update == 0
+
+
+def test_struct_sizes_match_header():
+    # offsets computed by the C compiler for the header layout (x86-64 SysV)
+    assert ctypes.sizeof(_capi.TriControl) == 144
+    assert ctypes.sizeof(_capi.TriStats) == 128
+
+
+def test_grid_mirror_matches_oracle():
+    from oracle import rls_oracle as O
+    import numpy as np
+    for args in [(-4.0, 6.5, -5.0, 2.0, 200, 200, 20), (0.45, 0.55, 0.0, 0.1, 20, 20, 20), (0, 1, 0, 1, 5, 6, 3)]:
+        a, b = rls_amd.recursion_grid(*args), O.recursion_grid(*args)
+        assert np.array_equal(a.Z, b.Z)
+        assert np.array_equal(rls_amd.recursion_natural_permutation(a), O.recursion_natural_permutation(b))
+    with pytest.raises(ValueError):
+        rls_amd.recursion_grid(0, 1, 0, 1, 10, 10, 3)
+
+
+def test_context_without_gpu_fails_loudly():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librls_hip.so not built")
+    import subprocess, sys
+    code = "import rls_amd._capi as c; c.Context(0)"
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       env={**os.environ, "HIP_VISIBLE_DEVICES": "-1"})
+    assert r.returncode != 0 and "rls_ctx_create" in r.stderr

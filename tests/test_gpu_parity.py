@@ -1,0 +1,137 @@
+"""GPU parity: the HIP psTriRecy path against the CPU oracle and FP64 ground truth.
+
+Every case prints its name and sizes before touching the GPU.  Sizes are chosen so the
+oracle (a pure numpy restatement) finishes in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import rls_amd as R
+from oracle import rls_oracle as O
+from oracle import truth
+from rls_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _window(which, nx, ny, x0, y0, k):
+    g = WL.GRIDS[which]
+    dx = (g[1] - g[0]) / 199
+    dy = (g[3] - g[2]) / 199
+    return (x0, x0 + (nx - 1) * dx, y0, y0 + (ny - 1) * dy, nx, ny, k)
+
+
+CASES = {
+    # name: (matrix builder, n, grid window, p)
+    "toeplitz300": (lambda n: WL.toeplitz_real(n).astype(complex), 300,
+                    _window("toeplitz", 12, 20, 0.0, -1.0, 10), 20),
+    "laser300": (lambda n: WL.laser(n), 300, _window("laser", 10, 20, -0.2, -0.3, 10), 20),
+    "basor240": (lambda n: WL.basor(n), 240, _window("toeplitz", 8, 12, 1.0, -1.5, 6), 15),
+}
+
+
+def _setup(name):
+    build, n, win, p = CASES[name]
+    C = build(n)
+    normC = float(sla.svdvals(C)[0])
+    T = R.schur_preprocess(C)
+    return C, T, normC, win, p
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_matches_truth_and_oracle(name):
+    C, T, normC, win, p = _setup(name)
+    _say(f"{name} n={C.shape[0]} grid={win[4]}x{win[5]} p={p}")
+    rec = R.recursion_grid(*win)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="batched")
+    sig, st = R.TriSolver(T).solve(rec.Z, ctl, rec)
+    tr = truth.svd_sigma_min(C, rec.Z)
+    rel = np.abs(sig - tr) / tr
+    assert not np.isnan(sig).any()
+    assert rel.max() <= 1e-4, f"max rel err {rel.max():.2e} vs FP64 SVD"
+    # oracle: the same algorithm in numpy FP64, same start block
+    orec = O.recursion_grid(*win)
+    octl = O.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6)
+    X0 = _initial_block(T.shape[0], ctl.r, ctl.seed)
+    osig, ost, _ = O.ps_tri_recy(O.dense_tri_problem(T), orec, octl, X_init=X0)
+    orel = np.abs(sig - osig) / osig
+    assert orel.max() <= 1e-4, f"max rel diff vs oracle {orel.max():.2e}"
+    # the trajectories agree: same refined points, up to borderline acceptance decisions
+    gref = st.lobpcg_it > 0
+    mism = int((gref != ost.refined).sum())
+    assert mism <= max(2, len(sig) // 50), f"{mism} refine decisions differ from the oracle"
+    assert st.refine_time_ns > 0
+
+
+def _initial_block(n, r, seed):
+    return R.initial_block(n, r, seed)
+
+
+def test_plain_and_batched_identical_trajectory():
+    C, T, normC, win, p = _setup("laser300")
+    _say("plain-vs-batched laser300")
+    rec = R.recursion_grid(*win)
+    solver = R.TriSolver(T)
+    out = {}
+    for mode in ("plain", "batched"):
+        ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode=mode)
+        out[mode] = solver.solve(rec.Z, ctl, rec)
+    (s1, st1), (s2, st2) = out["plain"], out["batched"]
+    assert np.array_equal(st1.lobpcg_it > 0, st2.lobpcg_it > 0)
+    assert np.max(np.abs(s1 - s2) / s1) < 1e-10
+    assert st2.n_windows > 0 and st2.n_window_shifts >= len(s2) - st2.n_pts
+
+
+def test_diagonal_closed_form_gpu():
+    _say("diagonal n=80")
+    d = np.linspace(-1, 1, 80) + 0.3j * np.sin(np.arange(80))
+    C = np.diag(d)
+    rec = R.recursion_grid(-0.5, 0.5, -0.2, 0.6, 6, 6, 3)
+    ctl = R.PsTriRecyControl(tol=1e-6, normC=float(np.abs(d).max()), p=5, r=3, recy_tau=1.6)
+    sig, st = R.TriSolver(C).solve(rec.Z, ctl, rec)
+    for i, z in enumerate(rec.Z):
+        dist = np.abs(d - z)
+        assert np.min(np.abs(dist - sig[i]) / dist.min()) < 1e-6   # a genuine singular value
+        if st.lobpcg_it[i] > 0:
+            assert abs(sig[i] - dist.min()) / dist.min() < 1e-6      # refined: the smallest
+
+
+def test_stats_bookkeeping():
+    C, T, normC, win, p = _setup("toeplitz300")
+    _say("stats toeplitz300")
+    rec = R.recursion_grid(*win)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, r=6)
+    sig, st = R.TriSolver(T).solve(rec.Z, ctl, rec)
+    refined = st.lobpcg_it > 0
+    assert st.n_pts == refined.sum() and st.n_it == st.lobpcg_it.sum()
+    it = st.lobpcg_it[refined]
+    # R_vec_apps = 2r + it r + (it-1) r  (psTriRecy.jl:811-813)
+    assert np.array_equal(st.R_vec_apps[refined], 2 * 6 + it * 6 + (it - 1) * 6)
+    # M_vec_apps = 3r per update, no update after the last point (psTriRecy.jl:822-844)
+    upd = refined.copy()
+    upd[-1] = False
+    assert np.array_equal(st.M_vec_apps[upd], np.full(upd.sum(), 18))
+    assert np.all(st.M_vec_apps[~refined] == 0)
+    assert np.all(st.p_count <= p) and st.p_count[0] == 0
+    np.testing.assert_allclose(st.pts[: st.n_pts], rec.Z[refined])
+
+
+def test_errors_are_reported():
+    _say("error paths")
+    T = np.triu(np.ones((64, 64), dtype=complex))
+    solver = R.TriSolver(T)
+    rec = R.recursion_grid(0, 1, 0, 1, 4, 4, 2)
+    with pytest.raises(R._capi.RlsError, match="p\\*r"):
+        solver.solve(rec.Z, R.PsTriRecyControl(p=200, r=6), rec)
+    with pytest.raises(R._capi.RlsError, match="p_min"):
+        solver.solve(rec.Z, R.PsTriRecyControl(p=4, p_min=5), rec)
+    with pytest.raises(R._capi.RlsError, match="recycle_fail_dist_thresh"):
+        solver.solve(rec.Z, R.PsTriRecyControl(recycle_fail_dist_thresh=-1.0), rec)
