@@ -68,7 +68,7 @@ def main():
     t_pre = time.perf_counter() - t0
     log(f"schur n={n} {t_pre:.1f}s, normC={normC:.6g}")
     solver = R.TriSolver(T)
-    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched")
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched", progress=1)
     nZ = len(rec.Z)
 
     for w in range(args.warmup):
@@ -88,7 +88,7 @@ def main():
 
     # plain FP64 one-shift-at-a-time solver on the leading points of the same traversal
     npl = min(args.plain_points, nZ)
-    ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain")
+    ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain", progress=1)
     t0 = time.perf_counter()
     sig_p, st_p = solver.solve(rec.Z[:npl], ctl_plain, rec)
     t_plain = time.perf_counter() - t0

@@ -79,8 +79,9 @@ struct rls_ctx {
     DArr<int> info;
     // recycle RR workspaces
     DArr<zd> Hz, heevW, Gev, C1, CzX, Gc, V2, CW, CG, Wr, Xr, zdev;
-    DArr<double> lamev, sigr, tausd;
-    DArr<int> rr, coff;
+    DArr<double> lamev, sigr, tausd, lamev2;
+    DArr<int> rr, coff, rr2;
+    DArr<zd> heevW2;
     // last X (Ritz vectors) of the final state, for callers
     std::vector<cplx> lastX;
 
@@ -89,7 +90,12 @@ struct rls_ctx {
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
                           &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev};
         for (auto* a : zs) a->release();
-        DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd};
+        DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd, &lamev2};
+        tri_ctr.release();
+        tri_flags.release();
+        tri_err.release();
+        heevW2.release();
+        rr2.release();
         for (auto* a : ds) a->release();
         info.release();
         rr.release();
@@ -116,10 +122,10 @@ struct rls_ctx {
 
     // Y = beta Y + A B  (A: n x K)
     void gemm1(const zd* A, int64_t lda, int K, const zd* B, int64_t ldb, int k, zd* Y, int64_t ldy,
-               double beta) {
+               double beta, double alpha = 1.0) {
         TallOperands ops{};
         ops.A[0] = A; ops.lda[0] = lda; ops.K[0] = K; ops.nops = 1;
-        launch_tall_gemm(st, n, ops, B, ldb, 0, k, Y, ldy, 0, beta, 1);
+        launch_tall_gemm(st, n, ops, B, ldb, 0, k, Y, ldy, 0, beta, 1, nullptr, 0, 0, -1, nullptr, 0, alpha);
     }
 
     // Cholesky-QR passes in place on X (n x k), accumulating R = R_m ... R_1 into Racc
@@ -137,7 +143,8 @@ struct rls_ctx {
         const double conv_tol = 2.0 * std::sqrt((double)k) * EPS64;
         for (int pass = 0; pass < 3; ++pass) {
             gram_op(X, ldx, k, X, ldx, k, gram.p, k);
-            launch_chol_rinv_cond(pass == 0 ? -1.0 : conv_tol, shift_scale, k);
+            launch_chol_rinv(st, gram.p, 0, k, shift_scale, Rm.p, Rinv.p, 0, offI.p, info.p, 1,
+                             pass == 0 ? -1.0 : conv_tol);
             gemm1(X, ldx, k, Rinv.p, k, k, X, ldx, 0.0);
             if (want_r) {
                 if (pass == 0) {
@@ -149,7 +156,6 @@ struct rls_ctx {
             }
         }
     }
-    void launch_chol_rinv_cond(double skip_tol, double shift_scale, int k);
 
     // rls_orth!(V, X): two passes of projection + QR (RLSLinearAlgebra.jl:1127-1167)
     void orth(const zd* Vb, int64_t ldv, int kV, zd* X, int64_t ldx, int kX) {
@@ -157,8 +163,7 @@ struct rls_ctx {
         Wneg.ensure((size_t)kV * kX);
         for (int pass = 0; pass < 2; ++pass) {
             gram_op(Vb, ldv, kV, X, ldx, kX, Wk.p, kV);
-            launch_neg2d(st, kV, kX, Wk.p, kV, Wneg.p, kV);
-            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+            gemm1(Vb, ldv, kV, Wk.p, kV, kX, X, ldx, 1.0, -1.0);
             tall_qr(X, ldx, kX, false);
         }
     }
@@ -175,8 +180,7 @@ struct rls_ctx {
         // pass 1
         if (kV > 0) {
             gram_op(Vb, ldv, kV, X, ldx, kX, Wtot.p, kV);
-            launch_neg2d(st, kV, kX, Wtot.p, kV, Wneg.p, kV);
-            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+            gemm1(Vb, ldv, kV, Wtot.p, kV, kX, X, ldx, 1.0, -1.0);
         }
         tall_qr(X, ldx, kX, true);
         // R1 <- Racc
@@ -185,8 +189,7 @@ struct rls_ctx {
         // pass 2
         if (kV > 0) {
             gram_op(Vb, ldv, kV, X, ldx, kX, Wk.p, kV);
-            launch_neg2d(st, kV, kX, Wk.p, kV, Wneg.p, kV);
-            gemm1(Vb, ldv, kV, Wneg.p, kV, kX, X, ldx, 1.0);
+            gemm1(Vb, ldv, kV, Wk.p, kV, kX, X, ldx, 1.0, -1.0);
         }
         tall_qr(X, ldx, kX, true);
         // Wtot += W2 R1 ; Rtot = R2 R1
@@ -223,18 +226,37 @@ struct rls_ctx {
     }
 
     // operator applications (S = I, Schur factor)
-    void apply_C(zd* X, int k) {   // X <- (T - zI)^{-H} X   (mul!(CX, InvMat(Rz'), X))
+    DArr<unsigned int> tri_ctr, tri_flags;
+    DArr<int> tri_err;
+    unsigned int tri_epoch = 0;
+    void tri_solve(zd* X, int k, bool upper) {
+        const int nblk = cdiv(n, 64);
+        if (tri_flags.n < (size_t)nblk + 1 || !tri_ctr.p) {
+            tri_flags.ensure((size_t)nblk + 1);
+            tri_ctr.ensure(1);
+            tri_err.ensure(1);
+            HIP_TRY(hipMemsetAsync(tri_flags.p, 0, sizeof(unsigned int) * tri_flags.n, st));
+            HIP_TRY(hipMemsetAsync(tri_ctr.p, 0, sizeof(unsigned int), st));
+            HIP_TRY(hipMemsetAsync(tri_err.p, 0, sizeof(int), st));
+            tri_epoch = 0;
+            tri_ticket_base_ok = true;
+        }
         for (int c0 = 0; c0 < k; c0 += 8) {
             int kc = std::min(8, k - c0);
-            launch_trsm_upper_h(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, trsm_tmp.p);
+            ++tri_epoch;
+            launch_trsm_persistent(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
+                                   tri_flags.p, tri_epoch, tri_err.p);
         }
     }
-    void apply_CH(zd* X, int k) {  // X <- (T - zI)^{-1} X
-        for (int c0 = 0; c0 < k; c0 += 8) {
-            int kc = std::min(8, k - c0);
-            launch_trsm_upper(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, trsm_tmp.p);
-        }
+    bool tri_ticket_base_ok = false;
+    void check_tri_err() {
+        if (!tri_err.p) return;
+        int e = 0;
+        d2h(&e, tri_err.p, 1);
+        if (e) throw RlsError{RLS_E_HIP, "triangular solve: dependency wait exceeded its bound"};
     }
+    void apply_C(zd* X, int k) { tri_solve(X, k, false); }   // X <- (T - zI)^{-H} X
+    void apply_CH(zd* X, int k) { tri_solve(X, k, true); }   // X <- (T - zI)^{-1} X
 
     // svdRaylRitz! + basisUpdate! on the first nS columns of (S, CS): new X|P (2r or r cols)
     void svd_rr_update(int nS, bool side_L, int keep) {
@@ -249,8 +271,8 @@ struct rls_ctx {
         // XP' = S G[:, :keep], CXP' = CS G[:, :keep]
         gemm1(S.p, n, nS, Gs.p, nS, keep, S2.p, n, 0.0);
         gemm1(CS.p, n, nS, Gs.p, nS, keep, CS2.p, n, 0.0);
-        launch_copy2d(st, n, keep, S2.p, n, S.p, n);
-        launch_copy2d(st, n, keep, CS2.p, n, CS.p, n);
+        std::swap(S.p, S2.p);
+        std::swap(CS.p, CS2.p);
     }
 
     // residual!(C, X, CX, W, lam, side) into S[:, 2r:3r] (or P slot for the first call)
@@ -276,31 +298,34 @@ struct rls_ctx {
         if (!side_L) throw RlsError{RLS_E_ARG, "lobpcg_side 'S' is not supported on the Schur path"};
         const double tol = c.lobpcg_tol_scale * c.tol;
         launch_tri_diag_inv(st, n, T.p, n, zmk(z.real(), z.imag()), Dinv.p);
-        zd* X = S.p;
-        zd* P = S.p + (int64_t)n * r;
-        zd* W = S.p + (int64_t)n * 2 * r;
+        // S = [X P W], CS = [CX CP CW]; svd_rr_update swaps the double buffers, so block
+        // pointers are re-derived after every call.
+        auto Xb = [&]() { return S.p; };
+        auto Pb = [&]() { return S.p + (int64_t)n * r; };
+        auto Wb = [&]() { return S.p + (int64_t)n * 2 * r; };
         // iteration 1
-        tall_qr(X, n, r, false);
-        launch_copy2d(st, n, r, X, n, CS.p, n);
+        tall_qr(Xb(), n, r, false);
+        launch_copy2d(st, n, r, Xb(), n, CS.p, n);
         apply_C(CS.p, r);
         svd_rr_update(r, side_L, r);
-        residual_into(P, side_L);
-        orth(X, n, r, P, n, r);
-        launch_copy2d(st, n, r, P, n, CS.p + (int64_t)n * r, n);
+        residual_into(Pb(), side_L);
+        orth(Xb(), n, r, Pb(), n, r);
+        launch_copy2d(st, n, r, Pb(), n, CS.p + (int64_t)n * r, n);
         apply_C(CS.p + (int64_t)n * r, r);
         int it = c.maxit;
         for (int i = 1; i <= c.maxit; ++i) {
             if (i == 1) svd_rr_update(2 * r, side_L, 2 * r);
             else svd_rr_update(3 * r, side_L, 2 * r);
-            residual_into(W, side_L);
-            double res = colnorm(W, n);
+            residual_into(Wb(), side_L);
+            double res = colnorm(Wb(), n);
             if (res < tol) { it = i; break; }
-            orth(S.p, n, 2 * r, W, n, r);
-            launch_copy2d(st, n, r, W, n, CS.p + (int64_t)n * 2 * r, n);
+            orth(S.p, n, 2 * r, Wb(), n, r);
+            launch_copy2d(st, n, r, Wb(), n, CS.p + (int64_t)n * 2 * r, n);
             apply_C(CS.p + (int64_t)n * 2 * r, r);
         }
         double s0 = 0;
         d2h(&s0, sig.p, 1);
+        check_tri_err();
         return {s0, it};
     }
 
@@ -411,7 +436,7 @@ struct rls_ctx {
                     std::vector<int>& rr_out) {
         const int B = (int)zs.size();
         const int pr = p * r;
-        const int nvec = std::min(pr, 64);
+        const int nvec = pr;
         zdev.ensure(B);
         tausd.ensure(B);
         h2d(zdev.p, reinterpret_cast<const zd*>(zs.data()), B);
@@ -426,23 +451,18 @@ struct rls_ctx {
                        (int64_t)pr * nvec, rr.p, tausd.p, r, nvec, 0);
         std::vector<int> rrh(B);
         d2h(rrh.data(), rr.p, B);
-        // shifts whose Ritz dimension exceeds the on-device limit are reported as failed
-        std::vector<int> rrc(B), coffh(B);
+        std::vector<int> coffh(B);
         int tot = 0, rmax = 0;
         for (int b = 0; b < B; ++b) {
-            rrc[b] = std::min(rrh[b], nvec);
             coffh[b] = tot;
-            tot += rrc[b];
-            rmax = std::max(rmax, rrc[b]);
+            tot += rrh[b];
+            rmax = std::max(rmax, rrh[b]);
         }
         rr_out = rrh;
         coff.ensure(B);
-        DArr<int>& rrd = info;   // clamped rr on device
-        rrd.ensure(B);
         h2d(coff.p, coffh.data(), B);
-        h2d(rrd.p, rrc.data(), B);
         C1.ensure((size_t)2 * pr * tot);
-        launch_recy_coef1(st, Gev.p, pr, (int64_t)pr * nvec, rrd.p, coff.p, zdev.p, pr, rmax, C1.p, B);
+        launch_recy_coef1(st, Gev.p, pr, (int64_t)pr * nvec, rr.p, coff.p, zdev.p, pr, rmax, C1.p, B);
         CzX.ensure((size_t)n * tot);
         {
             TallOperands ops{};
@@ -451,20 +471,41 @@ struct rls_ctx {
             ops.nops = 2;
             launch_tall_gemm(st, n, ops, C1.p, 2 * pr, 0, tot, CzX.p, n, 0, 0.0, 1);
         }
-        // per-shift Gram of CzX_ (rr_b x rr_b, padded to ld rmax)
+        // per-shift Gram of CzX_ (rr_b x rr_b inside rmax x rmax)
         Gc.ensure((size_t)B * rmax * rmax);
-        for (int b = 0; b < B; ++b)
-            gram_op(CzX.p + (int64_t)n * coffh[b], n, rrc[b], CzX.p + (int64_t)n * coffh[b], n, rrc[b],
-                    Gc.p + (int64_t)b * rmax * rmax, rmax);
+        if (rmax <= 32) {
+            part.ensure(gram_self_batched_elems(n, B));
+            launch_gram_self_batched(st, n, CzX.p, n, coff.p, rr.p, B, Gc.p, rmax, (int64_t)rmax * rmax,
+                                     part.p);
+        } else {
+            for (int b = 0; b < B; ++b)
+                gram_op(CzX.p + (int64_t)n * coffh[b], n, rrh[b], CzX.p + (int64_t)n * coffh[b], n, rrh[b],
+                        Gc.p + (int64_t)b * rmax * rmax, rmax);
+        }
+        // SVD of the r_ x r_ factor: the smallest r singular triplets of CzX_ (rls_svd!,
+        // ascending).  Small r_: Cholesky + one-sided Jacobi; large r_: Hermitian eigensolve of
+        // the padded Gram.
         V2.ensure((size_t)B * rmax * rmax);
         sigr.ensure((size_t)B * rmax);
-        const double shift_scale = 11.0 * ((double)n * rmax + (double)rmax * (rmax + 1));
-        launch_gram_svd_asc(st, Gc.p, (int64_t)rmax * rmax, rrd.p, rmax, shift_scale, V2.p,
-                            (int64_t)rmax * rmax, sigr.p, rmax, B);
+        int64_t ldv2 = rmax, sv2 = (int64_t)rmax * rmax, ssig = rmax;
+        if (rmax <= 64) {
+            const double shift_scale = 11.0 * ((double)n * rmax + (double)rmax * (rmax + 1));
+            launch_gram_svd_asc(st, Gc.p, (int64_t)rmax * rmax, rr.p, rmax, shift_scale, V2.p,
+                                (int64_t)rmax * rmax, sigr.p, rmax, B);
+        } else {
+            launch_pad_gram(st, Gc.p, rmax, rr.p, B);
+            heevW2.ensure((size_t)B * heev_workspace_elems(rmax, r));
+            lamev2.ensure((size_t)B * rmax);
+            rr2.ensure(B);
+            launch_heev_rr(st, Gc.p, rmax, (int64_t)rmax * rmax, rmax, B, heevW2.p, lamev2.p, rmax, V2.p,
+                           rmax, (int64_t)rmax * r, rr2.p, tausd.p, r, r, r);
+            launch_lam_to_sig(st, lamev2.p, rmax, sigr.p, rmax, r, B);
+            sv2 = (int64_t)rmax * r;
+        }
         CW.ensure((size_t)4 * pr * B * r);
         CG.ensure((size_t)pr * B * r);
-        launch_recy_coef2(st, Gev.p, pr, (int64_t)pr * nvec, rrd.p, V2.p, rmax, (int64_t)rmax * rmax,
-                          sigr.p, rmax, zdev.p, pr, r, CW.p, CG.p, B);
+        launch_recy_coef2(st, Gev.p, pr, (int64_t)pr * nvec, rr.p, V2.p, ldv2, sv2, sigr.p, ssig, zdev.p,
+                          pr, r, CW.p, CG.p, B);
         Wr.ensure((size_t)n * B * r);
         Xr.ensure((size_t)n * B * r);
         {
@@ -488,16 +529,9 @@ struct rls_ctx {
         for (int b = 0; b < B; ++b) {
             res_out[b] = std::sqrt(nh[(size_t)b * r]);
             for (int j = 0; j < r; ++j) sig_out[(size_t)b * r + j] = sh[(size_t)b * rmax + j];
-            if (rrh[b] > nvec) res_out[b] = std::numeric_limits<double>::infinity();
         }
     }
 };
-
-void rls_ctx::launch_chol_rinv_cond(double skip_tol, double shift_scale, int k) {
-    // skip_tol < 0: always factor; otherwise R = Rinv = I when ||G - I||_F < skip_tol
-    launch_chol_rinv(st, gram.p, 0, k, shift_scale, Rm.p, Rinv.p, 0, offI.p, info.p, 1);
-    if (skip_tol >= 0) launch_identity_if_orth(st, k, offI.p, skip_tol, Rm.p, Rinv.p);
-}
 
 // --------------------------------------------------------------------------------------
 // psTriRecy driver
@@ -736,7 +770,10 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
     return guarded(ctx, [&]() -> int {
         if (!T || n <= 0 || ld < n) throw RlsError{RLS_E_ARG, "rls_set_tri_dense: bad T / n / ld"};
         HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
         ctx->n = n;
+        ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
+        ctx->tri_ctr.release();
         ctx->T.ensure((size_t)n * n);
         HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
                                  hipMemcpyHostToDevice, ctx->st));
@@ -752,6 +789,84 @@ int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_contr
         HIP_TRY(hipSetDevice(ctx->device));
         return solve_tri(ctx, reinterpret_cast<const cplx*>(Z), nZ, *control, sigma_out, stats);
     });
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------------------
+// Kernel-level test hooks (not part of the public ABI in include/): each runs one device
+// primitive on host data through a private context so the GPU test suite can compare it
+// with numpy/scipy.
+// --------------------------------------------------------------------------------------
+extern "C" {
+
+int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_fixed, double* lam,
+                  double* G, int* rr_out) {
+    rls_ctx* ctx = nullptr;
+    int rc = rls_ctx_create(0, &ctx);
+    if (rc) return rc;
+    rc = guarded(ctx, [&]() -> int {
+        DArr<zd> dA, dW, dG;
+        DArr<double> dl, dt;
+        DArr<int> drr;
+        dA.ensure((size_t)n * n);
+        dW.ensure(heev_workspace_elems(n, nvec));
+        dG.ensure((size_t)n * nvec);
+        dl.ensure(n);
+        dt.ensure(1);
+        drr.ensure(1);
+        ctx->h2d(dA.p, reinterpret_cast<const zd*>(A), (size_t)n * n);
+        ctx->h2d(dt.p, &tau, 1);
+        launch_heev_rr(ctx->st, dA.p, n, 0, n, 1, dW.p, dl.p, n, dG.p, n, 0, drr.p, dt.p, r, nvec, mode_fixed);
+        ctx->d2h(lam, dl.p, n);
+        ctx->d2h(reinterpret_cast<zd*>(G), dG.p, (size_t)n * nvec);
+        ctx->d2h(rr_out, drr.p, 1);
+        dA.release(); dW.release(); dG.release(); dl.release(); dt.release(); drr.release();
+        return RLS_OK;
+    });
+    rls_ctx_destroy(ctx);
+    return rc;
+}
+
+int rls_test_trsm(int64_t n, const double* T, double zr, double zi, int k, int upper, double* X) {
+    rls_ctx* ctx = nullptr;
+    int rc = rls_ctx_create(0, &ctx);
+    if (rc) return rc;
+    rc = guarded(ctx, [&]() -> int {
+        if (rls_set_tri_dense(ctx, T, n, n) != RLS_OK) throw RlsError{RLS_E_ARG, ctx->err};
+        DArr<zd> dX;
+        dX.ensure((size_t)n * k);
+        ctx->Dinv.ensure((size_t)cdiv(n, 64) * 4096);
+        ctx->h2d(dX.p, reinterpret_cast<const zd*>(X), (size_t)n * k);
+        launch_tri_diag_inv(ctx->st, n, ctx->T.p, n, zmk(zr, zi), ctx->Dinv.p);
+        // two solves in a row exercise the epoch/ticket bookkeeping
+        ctx->tri_solve(dX.p, k, upper != 0);
+        ctx->d2h(reinterpret_cast<zd*>(X), dX.p, (size_t)n * k);
+        ctx->check_tri_err();
+        dX.release();
+        return RLS_OK;
+    });
+    rls_ctx_destroy(ctx);
+    return rc;
+}
+
+int rls_test_tall_qr(int64_t n, int k, double* X, double* R) {
+    rls_ctx* ctx = nullptr;
+    int rc = rls_ctx_create(0, &ctx);
+    if (rc) return rc;
+    rc = guarded(ctx, [&]() -> int {
+        ctx->n = n;
+        DArr<zd> dX;
+        dX.ensure((size_t)n * k);
+        ctx->h2d(dX.p, reinterpret_cast<const zd*>(X), (size_t)n * k);
+        ctx->tall_qr(dX.p, n, k, true);
+        ctx->d2h(reinterpret_cast<zd*>(X), dX.p, (size_t)n * k);
+        ctx->d2h(reinterpret_cast<zd*>(R), ctx->Racc.p, (size_t)k * k);
+        dX.release();
+        return RLS_OK;
+    });
+    rls_ctx_destroy(ctx);
+    return rc;
 }
 
 }  // extern "C"

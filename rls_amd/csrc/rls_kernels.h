@@ -17,7 +17,7 @@ struct TallOperands {
 void launch_tall_gemm(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
                       int64_t sB, int k, zd* Y, int64_t ldy, int64_t sY, double beta, int batch,
                       zd* Y2 = nullptr, int64_t ldy2 = 0, int64_t sY2 = 0, int q2 = -1,
-                      const zd* B2 = nullptr, int64_t ldb2 = 0);
+                      const zd* B2 = nullptr, int64_t ldb2 = 0, double alpha = 1.0);
 
 // C (ka x kb, ld ldc) = A^H B with A n x ka, B n x kb.  `part` is a workspace of at least
 // gram_partial_elems(n, ka, kb) complex values.
@@ -64,7 +64,8 @@ void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, i
                         double shift_scale, zd* G, int64_t sG, double* sig, double* lam,
                         int64_t sSig, int* info, int batch);
 void launch_chol_rinv(hipStream_t st, const zd* Gram, int64_t sGram, int k, double shift_scale,
-                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch);
+                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch,
+                      double skip_tol = -1.0);
 void launch_gram_svd_asc(hipStream_t st, const zd* Gram, int64_t sGram, const int* kk, int kmax,
                          double shift_scale, zd* V, int64_t sV, double* sig, int64_t sS, int batch);
 void launch_form_hz(hipStream_t st, const zd* H1, const zd* H2, int64_t ldh, int pr, const zd* z,
@@ -81,3 +82,14 @@ void launch_neg2d(hipStream_t st, int m, int k, const zd* S, int64_t lds, zd* D,
 void launch_small_gemm(hipStream_t st, int m, int n, int k, const zd* A, int64_t lda, const zd* B,
                        int64_t ldb, zd* C, int64_t ldc, double beta);
 void launch_identity_if_orth(hipStream_t st, int k, const double* offI, double tol, zd* R, zd* Rinv);
+size_t gram_self_batched_elems(int64_t n, int B);
+void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t lda, const int* coff,
+                              const int* kk, int B, zd* C, int64_t ldc, int64_t sC, zd* part);
+void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B);
+void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
+                       int B);
+// Persistent single-launch shifted triangular solve (see rls_tall.hip); counter/flags are
+// device words owned by the caller (flags: one per 64-row block), epoch increases per call.
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv, zd* X,
+                            int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
+                            unsigned int epoch, int* err);

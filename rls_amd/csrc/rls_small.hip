@@ -20,7 +20,7 @@
 // ---------------------------------------------------------------------------------------
 namespace {
 
-constexpr int HEEV_THREADS = 256;
+constexpr int HEEV_THREADS = 512;
 
 __device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
     // number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2
@@ -132,9 +132,9 @@ __device__ void multisect(const double* d, const double* e2, int n, const int* i
 }  // namespace
 
 // A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
-//    batch stride sA (elements).  W: workspace, per problem n*n complex + 8n doubles.
-// Outputs: lam (n per problem, ascending), G (ldg x nvec_max complex per problem, stride sG),
-//          rr (per problem: the Ritz dimension chosen by the gap rule, see below).
+//    batch stride sA (elements).  Wk: per-problem workspace (heev_workspace_elems).
+// Outputs: lam (n per problem, ascending; entries not needed by the rule stay NaN),
+//          G (ldg x nvec_max complex per problem, stride sG), rr (Ritz dimension).
 // Gap rule (find_rr_dim!): gap = lam_max / tau * eps; windows n_chk = n_init, n_init+2r, ...
 // (<= n); r_ = first i with |lam_i - lam_{i+1}| > gap inside the largest window, then
 // max(r_, r); if none, r_ = n.  Eigenvectors 1..min(r_, nvec_max) are returned.
@@ -146,6 +146,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
           int nvec_max, int mode_fixed) {
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
     const zd* Ab = A + (int64_t)b * sA;
     zd* M = Wk + (int64_t)b * sW;                  // n x n full Hermitian copy, ld = n
     double* dws = reinterpret_cast<double*>(M + (int64_t)n * n);
@@ -157,85 +158,127 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     double* Y = reinterpret_cast<double*>(tau + n);   // n x nvec_max real eigenvectors of T
     double* scratch = Y + (int64_t)n * nvec_max;     // per-vector tri-solve workspace 4n x nvec
     __shared__ double red[HEEV_THREADS / 64];
-    __shared__ zd sv[1024];   // Householder vector (n <= 1024)
-    __shared__ zd sw[1024];   // w
+    __shared__ zd sv[1024];    // current Householder vector v_k
+    __shared__ zd sw[1024];    // w_k
+    __shared__ zd pv[1024];    // pending update vectors (v_{k-1}, w_{k-1})
+    __shared__ zd pw[1024];
     __shared__ int s_rr;
+    __shared__ double s_part[HEEV_THREADS];
 
     // 1. copy with Hermitian completion from the upper triangle
-    for (int64_t idx = tid; idx < (int64_t)n * n; idx += blockDim.x) {
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += nthr) {
         int i = (int)(idx % n), j = (int)(idx / n);
         zd v = (i <= j) ? Ab[i + lda * j] : zconj(Ab[j + lda * i]);
         if (i == j) v.y = 0.0;
         M[i + (int64_t)n * j] = v;
     }
+    for (int i = tid; i < 1024; i += nthr) { pv[i] = zzero(); pw[i] = zzero(); }
     __syncthreads();
 
-    // 2. tridiagonalisation, lower form
+    // 2. tridiagonalisation (zhetd2 lower form).  The rank-2 update of step k-1 is applied
+    //    lazily inside the matrix-vector pass of step k (one read + one write per element
+    //    per step); the column of step k gets it on the fly.
+    bool pending = false;
     for (int k = 0; k < n - 1; ++k) {
-        const int m = n - k - 1;                 // length of x = M[k+1:n, k]
+        const int m = n - k - 1;                 // x = A[k+1:n, k]
         zd* x = M + (k + 1) + (int64_t)n * k;
-        // xnorm^2 of x[1:]
+        // x' = x - (pending update of step k-1 restricted to column k, rows k+1..)
+        // in the k-1 frame, row k+1+i <-> index i+1, column k <-> index 0
+        if (pending) {
+            zd w0c = zconj(pw[0]), v0c = zconj(pv[0]);
+            for (int i = tid; i < m; i += nthr) {
+                zd a = x[i];
+                a = zsub(a, zmul(pv[i + 1], w0c));
+                a = zsub(a, zmul(pw[i + 1], v0c));
+                x[i] = a;
+            }
+            if (tid == 0) {
+                zd dk = M[k + (int64_t)n * k];
+                dk.x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+                M[k + (int64_t)n * k] = dk;
+            }
+        }
+        __syncthreads();
         double part = 0.0;
-        for (int i = 1 + tid; i < m; i += blockDim.x) part += zabs2(x[i]);
+        for (int i = 1 + tid; i < m; i += nthr) part += zabs2(x[i]);
         double xn2 = block_sum(part, red);
         zd alpha = x[0];
         zd t = zzero();
         double beta;
         if (xn2 == 0.0 && alpha.y == 0.0) {
             beta = alpha.x;
-            t = zzero();
-            for (int i = tid; i < m; i += blockDim.x) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zzero();
+            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zzero();
         } else {
             double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
             beta = (alpha.x >= 0.0) ? -nrm : nrm;
             t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
             zd scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
-            for (int i = tid; i < m; i += blockDim.x) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zmul(x[i], scal);
+            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zmul(x[i], scal);
         }
         __syncthreads();
         if (tid == 0) { e[k] = beta; tau[k] = t; d[k] = M[k + (int64_t)n * k].x; }
-        // store v below the subdiagonal (x[1:] <- v[1:]) for the back-transformation
-        for (int i = 1 + tid; i < m; i += blockDim.x) x[i] = sv[i];
-        __syncthreads();
-        if (t.x == 0.0 && t.y == 0.0) continue;
-        // w = tau * A22 v   (A22 = M[k+1:, k+1:])
+        for (int i = 1 + tid; i < m; i += nthr) x[i] = sv[i];   // keep v for the back-transform
+        // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
-        for (int i = tid; i < m; i += blockDim.x) {
-            zd acc = zzero();
-            for (int j = 0; j < m; ++j) acc = zfma(A22[i + (int64_t)n * j], sv[j], acc);
-            sw[i] = zmul(t, acc);
+        const bool has_t = !(t.x == 0.0 && t.y == 0.0);
+        for (int i = tid; i < m; i += nthr) {
+            zd acc0 = zzero(), acc1 = zzero();
+            const zd pvi = pending ? pv[i + 1] : zzero();
+            const zd pwi = pending ? pw[i + 1] : zzero();
+            int j = 0;
+            for (; j + 1 < m; j += 2) {
+                zd a0 = A22[i + (int64_t)n * j];
+                zd a1 = A22[i + (int64_t)n * (j + 1)];
+                if (pending) {
+                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                    a1 = zsub(a1, zadd(zmul(pvi, zconj(pw[j + 2])), zmul(pwi, zconj(pv[j + 2]))));
+                    A22[i + (int64_t)n * j] = a0;
+                    A22[i + (int64_t)n * (j + 1)] = a1;
+                }
+                acc0 = zfma(a0, sv[j], acc0);
+                acc1 = zfma(a1, sv[j + 1], acc1);
+            }
+            for (; j < m; ++j) {
+                zd a0 = A22[i + (int64_t)n * j];
+                if (pending) {
+                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                    A22[i + (int64_t)n * j] = a0;
+                }
+                acc0 = zfma(a0, sv[j], acc0);
+            }
+            sw[i] = zmul(t, zadd(acc0, acc1));
         }
         __syncthreads();
-        // alpha2 = -1/2 tau (w^H v)
+        if (!has_t) {
+            pending = false;
+            __syncthreads();
+            continue;
+        }
+        // alpha2 = -1/2 tau (w^H v); w += alpha2 v
         double pr = 0.0, pi = 0.0;
-        for (int i = tid; i < m; i += blockDim.x) {
+        for (int i = tid; i < m; i += nthr) {
             zd c = zcmul(sw[i], sv[i]);
             pr += c.x; pi += c.y;
         }
         double whv_r = block_sum(pr, red);
         double whv_i = block_sum(pi, red);
         zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
-        for (int i = tid; i < m; i += blockDim.x) sw[i] = zfma(a2, sv[i], sw[i]);
-        __syncthreads();
-        // A22 -= v w^H + w v^H
-        for (int i = tid; i < m; i += blockDim.x) {
-            zd vi = sv[i], wi = sw[i];
-            for (int j = 0; j < m; ++j) {
-                zd a = A22[i + (int64_t)n * j];
-                a = zsub(a, zmul(vi, zconj(sw[j])));
-                a = zsub(a, zmul(wi, zconj(sv[j])));
-                A22[i + (int64_t)n * j] = a;
-            }
+        for (int i = tid; i < m; i += nthr) {
+            pw[i] = zfma(a2, sv[i], sw[i]);
+            pv[i] = sv[i];
         }
+        pending = true;
         __syncthreads();
     }
     if (tid == 0) {
-        d[n - 1] = M[(n - 1) + (int64_t)n * (n - 1)].x;
+        zd dl = M[(n - 1) + (int64_t)n * (n - 1)];
+        if (pending) dl.x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+        d[n - 1] = dl.x;
         e[n - 1] = 0.0;
         tau[n - 1] = zzero();
     }
     __syncthreads();
-    for (int i = tid; i < n; i += blockDim.x) e2[i] = e[i] * e[i];
+    for (int i = tid; i < n; i += nthr) e2[i] = e[i] * e[i];
     __syncthreads();
 
     // 3. Gershgorin interval
@@ -251,17 +294,15 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     gl -= 2.0 * eps * tnorm * n + pivmin;
     gu += 2.0 * eps * tnorm * n + pivmin;
     double* lo_ = lam_out + (int64_t)b * sL;
-    for (int i = tid; i < n; i += blockDim.x) lam[i] = nan("");
+    for (int i = tid; i < n; i += nthr) lam[i] = nan("");
     __syncthreads();
 
-    // 4. eigenvalues on demand (multisection) + Ritz-dimension rule of find_rr_dim!:
-    //    gap = lam_max / tau * eps; windows n_chk = n_init, n_init + 2r, ... (<= n);
-    //    r_ = first i with |lam_i - lam_{i+1}| > gap, then max(r_, r); none -> n.
+    // 4. eigenvalues on demand (multisection) + Ritz-dimension rule of find_rr_dim!
     {
         __shared__ int s_idx[64];
         __shared__ int s_k, s_kc;
         __shared__ int s_done;
-        int have = 0;   // eigenvalues 0..have-1 computed
+        int have = 0;
         bool have_max = false;
         if (tid == 0) { s_rr = 0; s_done = 0; }
         __syncthreads();
@@ -269,22 +310,21 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
                                             : ((2 * r < n) ? 2 * r : n);
         int target = n_init;
         while (true) {
-            // wanted indices: have..target-1 and n-1 (once)
             if (tid == 0) {
-                int k = 0, kc = 0;
-                if (!have_max && mode_fixed <= 0) s_idx[k++] = n - 1;
-                for (int i = have; i < target && k < 64; ++i) {
+                int kq = 0, kc = 0;
+                if (!have_max && mode_fixed <= 0) s_idx[kq++] = n - 1;
+                for (int i = have; i < target && kq < 64; ++i) {
                     if (i == n - 1 && !have_max && mode_fixed <= 0) { ++kc; continue; }
-                    s_idx[k++] = i; ++kc;
+                    s_idx[kq++] = i; ++kc;
                 }
-                s_k = k; s_kc = kc;
+                s_k = kq; s_kc = kc;
             }
             __syncthreads();
             multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
             __syncthreads();
             have_max = true;
             have += s_kc;
-            if (have < target) continue;   // more than 64 requested: loop
+            if (have < target) continue;
             if (tid == 0) {
                 if (mode_fixed > 0) {
                     s_rr = n_init; s_done = 1;
@@ -310,13 +350,12 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
             target = target + 2 * r;
             __syncthreads();
         }
-        // eigenvalues needed for the eigenvectors 0..rr-1
         int need = s_rr < nvec_max ? s_rr : nvec_max;
         while (have < need) {
             if (tid == 0) {
-                int k = 0;
-                for (int i = have; i < need && k < 64; ++i) s_idx[k++] = i;
-                s_k = k; s_kc = k;
+                int kq = 0;
+                for (int i = have; i < need && kq < 64; ++i) s_idx[kq++] = i;
+                s_k = kq; s_kc = kq;
             }
             __syncthreads();
             multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
@@ -326,79 +365,105 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         if (tid == 0) rr_out[b] = s_rr;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += blockDim.x) lo_[i] = lam[i];
+    for (int i = tid; i < n; i += nthr) lo_[i] = lam[i];
     const int nv = s_rr < nvec_max ? s_rr : nvec_max;
 
-    // 5. inverse iteration for eigenvectors 0..nv-1 (thread j owns vector j)
-    const double ortol = 1e-3 * tnorm;
+    // 5. inverse iteration for eigenvectors 0..nv-1 (thread j owns vector j); vectors whose
+    //    eigenvalues are closer than ortol are re-orthogonalised (one wave per cluster).
+    const double ortol = 1e-9 * tnorm;
     const double floorv = eps * tnorm + pivmin;
+    const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
     for (int it = 0; it < 3; ++it) {
-        for (int j = tid; j < nv; j += blockDim.x) {
+        for (int j = tid; j < nv; j += nthr) {
             double* y = Y + (int64_t)j * n;
             if (it == 0) {
-                // deterministic start vector
                 uint32_t s = 0x9E3779B9u * (uint32_t)(j + 1);
                 for (int i = 0; i < n; ++i) {
                     s ^= s << 13; s ^= s >> 17; s ^= s << 5;
                     y[i] = (double)(s & 0xFFFFFF) / 16777216.0 - 0.5;
                 }
             }
-            // small perturbation keeps (T - lam I) nonsingular for repeated eigenvalues
-            double lj = lam[j];
-            tri_shifted_solve(d, e, n, lj, y, scratch + (int64_t)j * 4 * n, floorv);
+            tri_shifted_solve(d, e, n, lam[j], y, scratch + (int64_t)j * 4 * n, floorv);
             double nr = 0.0;
             for (int i = 0; i < n; ++i) nr += y[i] * y[i];
             nr = 1.0 / sqrt(nr);
             for (int i = 0; i < n; ++i) y[i] *= nr;
         }
         __syncthreads();
-        // modified Gram-Schmidt inside clusters (eigenvalues closer than ortol), in index order
-        for (int j = 1; j < nv; ++j) {
-            for (int q = 0; q < j; ++q) {
-                if (fabs(lam[j] - lam[q]) > ortol) continue;
-                double* yj = Y + (int64_t)j * n;
-                const double* yq = Y + (int64_t)q * n;
-                double part2 = 0.0;
-                for (int i = tid; i < n; i += blockDim.x) part2 += yj[i] * yq[i];
-                double dot = block_sum(part2, red);
-                for (int i = tid; i < n; i += blockDim.x) yj[i] -= dot * yq[i];
-                __syncthreads();
+        // clusters = maximal runs with lam[j] - lam[j-1] <= ortol; cluster c starts at j0
+        int c = 0;
+        for (int j0 = 0; j0 < nv;) {
+            int j1 = j0 + 1;
+            while (j1 < nv && lam[j1] - lam[j1 - 1] <= ortol) ++j1;
+            if (j1 - j0 > 1 && (c % nw) == wv) {
+                for (int j = j0 + 1; j < j1; ++j) {
+                    double* yj = Y + (int64_t)j * n;
+                    for (int q = j0; q < j; ++q) {
+                        const double* yq = Y + (int64_t)q * n;
+                        double s = 0.0;
+                        for (int i = lane; i < n; i += 64) s += yj[i] * yq[i];
+                        s = wave_sum(s);
+                        for (int i = lane; i < n; i += 64) yj[i] -= s * yq[i];
+                    }
+                    double s2 = 0.0;
+                    for (int i = lane; i < n; i += 64) s2 += yj[i] * yj[i];
+                    s2 = 1.0 / sqrt(wave_sum(s2));
+                    for (int i = lane; i < n; i += 64) yj[i] *= s2;
+                }
             }
-            double* yj = Y + (int64_t)j * n;
-            double part3 = 0.0;
-            for (int i = tid; i < n; i += blockDim.x) part3 += yj[i] * yj[i];
-            double nr = block_sum(part3, red);
-            nr = 1.0 / sqrt(nr);
-            for (int i = tid; i < n; i += blockDim.x) yj[i] *= nr;
-            __syncthreads();
+            if (j1 - j0 > 1) ++c;
+            j0 = j1;
         }
+        __syncthreads();
     }
 
-    // 6. back-transformation G = H(0) H(1) ... H(n-2) y, applied as y <- H(k) y for k descending
+    // 6. back-transformation G = H(0) H(1) ... H(n-2) y, y <- H(k) y for k descending.
+    //    Dots of all vectors with v_k in one pass: thread (j, slice) partial sums.
     zd* Gb = G + (int64_t)b * sG;
-    for (int j = 0; j < nv; ++j) {
-        const double* y = Y + (int64_t)j * n;
-        for (int i = tid; i < n; i += blockDim.x) Gb[i + ldg * j] = zmk(y[i], 0.0);
+    for (int64_t idx = tid; idx < (int64_t)n * nv; idx += nthr) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        Gb[i + ldg * j] = zmk(Y[(int64_t)j * n + i], 0.0);
     }
     __syncthreads();
-    for (int k = n - 2; k >= 0; --k) {
-        zd t = tau[k];
-        if (t.x == 0.0 && t.y == 0.0) continue;
-        const int m = n - k - 1;
-        const zd* x = M + (k + 1) + (int64_t)n * k;   // v[0] = 1 implicit, v[1:] stored
-        for (int j = 0; j < nv; ++j) {
-            zd* g = Gb + (k + 1) + ldg * j;
+    for (int j0 = 0; j0 < nv; j0 += nthr) {
+        const int nvc = min(nv - j0, nthr);
+        const int S = nthr / nvc;                 // row slices per vector
+        const int jj = tid % nvc, sl = tid / nvc;
+        for (int k = n - 2; k >= 0; --k) {
+            const zd t = tau[k];
+            if (t.x == 0.0 && t.y == 0.0) continue;
+            const int m = n - k - 1;
+            const zd* x = M + (k + 1) + (int64_t)n * k;   // v[0] = 1 implicit, v[1:] stored
+            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : x[i];
+            __syncthreads();
             double pr = 0.0, pi = 0.0;
-            for (int i = tid; i < m; i += blockDim.x) {
-                zd vi = (i == 0) ? zmk(1.0, 0.0) : x[i];
-                zd c = zcmul(vi, g[i]);
-                pr += c.x; pi += c.y;
+            if (sl < S) {
+                const zd* g = Gb + (k + 1) + ldg * (j0 + jj);
+                for (int i = sl; i < m; i += S) {
+                    zd c2 = zcmul(sv[i], g[i]);
+                    pr += c2.x; pi += c2.y;
+                }
             }
-            double sr = block_sum(pr, red), si = block_sum(pi, red);
-            zd f = zmul(t, zmk(sr, si));   // H = I - tau v v^H
-            for (int i = tid; i < m; i += blockDim.x) {
-                zd vi = (i == 0) ? zmk(1.0, 0.0) : x[i];
-                g[i] = zsub(g[i], zmul(vi, f));
+            s_part[tid] = pr;
+            __syncthreads();
+            if (tid < nvc) {
+                double sr = 0.0;
+                for (int q = 0; q < S; ++q) sr += s_part[q * nvc + tid];
+                sw[tid].x = sr;
+            }
+            __syncthreads();
+            s_part[tid] = pi;
+            __syncthreads();
+            if (tid < nvc) {
+                double si = 0.0;
+                for (int q = 0; q < S; ++q) si += s_part[q * nvc + tid];
+                sw[tid] = zmul(t, zmk(sw[tid].x, si));   // H = I - tau v v^H
+            }
+            __syncthreads();
+            for (int idx = tid; idx < m * nvc; idx += nthr) {
+                const int i = idx % m, j = idx / m;
+                zd* g = Gb + (k + 1) + ldg * (j0 + j);
+                g[i] = zsub(g[i], zmul(sv[i], sw[j]));
             }
             __syncthreads();
         }
@@ -414,7 +479,7 @@ size_t heev_workspace_elems(int n, int nvec_max) {
 void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
                     zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
                     const double* taus, int r, int nvec_max, int mode_fixed) {
-    if (n > 1024) throw RlsError{-1, "heev: n > 1024 unsupported"};
+    if (n > 1023) throw RlsError{-1, "heev: n > 1023 unsupported"};
     int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
     hipLaunchKernelGGL(k_heev_rr, dim3(batch), dim3(HEEV_THREADS), 0, st, A, lda, sA, n, Wk, sW,
                        lam, sL, G, ldg, sG, rr, taus, r, nvec_max, mode_fixed);
@@ -692,7 +757,7 @@ void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, i
 __global__ void __launch_bounds__(SMALL_THREADS)
 k_chol_rinv(const zd* __restrict__ Gram, int64_t sGram, int k, double shift_scale,
             zd* __restrict__ Rout, zd* __restrict__ Rinv, int64_t sR, double* __restrict__ offI,
-            int* __restrict__ info) {
+            int* __restrict__ info, double skip_tol) {
     extern __shared__ zd lds[];
     const int b = blockIdx.x, tid = threadIdx.x;
     zd* A = lds;
@@ -710,6 +775,18 @@ k_chol_rinv(const zd* __restrict__ Gram, int64_t sGram, int k, double shift_scal
         part += zabs2(v);
     }
     double fro = block_sum(part, red);
+    if (skip_tol >= 0.0 && sqrt(fro) < skip_tol) {
+        // already orthonormal: this pass is the identity (RLSLinearAlgebra.jl:779-782)
+        zd* Ro = Rout + (int64_t)b * sR;
+        zd* Rio = Rinv + (int64_t)b * sR;
+        for (int idx = tid; idx < k * k; idx += blockDim.x) {
+            zd v = (idx % k == idx / k) ? zmk(1.0, 0.0) : zzero();
+            Ro[idx] = v;
+            Rio[idx] = v;
+        }
+        if (tid == 0) { info[b] = -2; offI[b] = sqrt(fro); }
+        return;
+    }
     const double eps = 2.220446049250313e-16;
     int nshift = 0;
     double shift = 0.0;
@@ -735,11 +812,12 @@ k_chol_rinv(const zd* __restrict__ Gram, int64_t sGram, int k, double shift_scal
 }
 
 void launch_chol_rinv(hipStream_t st, const zd* Gram, int64_t sGram, int k, double shift_scale,
-                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch) {
+                      zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch,
+                      double skip_tol) {
     if (k > 64) throw RlsError{-1, "chol_rinv: k > 64 unsupported"};
     size_t lds = (size_t)2 * k * k * sizeof(zd);
     hipLaunchKernelGGL(k_chol_rinv, dim3(batch), dim3(SMALL_THREADS), lds, st, Gram, sGram, k,
-                       shift_scale, R, Rinv, sR, offI, info);
+                       shift_scale, R, Rinv, sR, offI, info, skip_tol);
     LAUNCH_CHECK();
 }
 
@@ -1104,5 +1182,44 @@ __global__ void k_identity_if_orth(int k, const double* __restrict__ offI, doubl
 
 void launch_identity_if_orth(hipStream_t st, int k, const double* offI, double tol, zd* R, zd* Rinv) {
     hipLaunchKernelGGL(k_identity_if_orth, dim3(1), dim3(256), 0, st, k, offI, tol, R, Rinv);
+    LAUNCH_CHECK();
+}
+
+// Pad the per-shift Gram blocks (k_b x k_b inside kmax x kmax) so that a Hermitian
+// eigensolve of the kmax x kmax matrix returns the k_b-block's smallest eigenpairs: zero the
+// off-diagonal padding and put a value above every eigenvalue on the padded diagonal.
+__global__ void k_pad_gram(zd* __restrict__ Gc, int kmax, const int* __restrict__ kk) {
+    const int b = blockIdx.y;
+    const int k = kk[b];
+    zd* G = Gc + (int64_t)b * kmax * kmax;
+    __shared__ double big;
+    if (threadIdx.x == 0) {
+        double tr = 0.0;
+        for (int j = 0; j < k; ++j) tr += fabs(G[j + kmax * j].x);
+        big = 1e3 * (tr + 1.0);
+    }
+    __syncthreads();
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < kmax * kmax; idx += gridDim.x * blockDim.x) {
+        const int i = idx % kmax, j = idx / kmax;
+        if (i < k && j < k) continue;
+        G[idx] = (i == j) ? zmk(big, 0.0) : zzero();
+    }
+}
+
+void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B) {
+    hipLaunchKernelGGL(k_pad_gram, dim3(cdiv((int64_t)kmax * kmax, 256), B), dim3(256), 0, st, Gc, kmax, kk);
+    LAUNCH_CHECK();
+}
+
+// sig = sqrt(max(lam, 0)) for the first r entries of each problem
+__global__ void k_lam_to_sig(const double* __restrict__ lam, int64_t sL, double* __restrict__ sig,
+                             int64_t sS, int r) {
+    const int b = blockIdx.x, j = threadIdx.x;
+    if (j < r) sig[(int64_t)b * sS + j] = sqrt(fmax(lam[(int64_t)b * sL + j], 0.0));
+}
+
+void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
+                       int B) {
+    hipLaunchKernelGGL(k_lam_to_sig, dim3(B), dim3(64), 0, st, lam, sL, sig, sS, r);
     LAUNCH_CHECK();
 }

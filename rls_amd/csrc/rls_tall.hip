@@ -16,7 +16,7 @@ template <int KC>
 __global__ void __launch_bounds__(256)
 k_tall_gemm(int64_t n, TallOperands ops, const zd* __restrict__ B, int64_t ldb, int64_t sB, int k,
             zd* __restrict__ Y, int64_t ldy, int64_t sY, double beta, zd* __restrict__ Y2,
-            int64_t ldy2, int64_t sY2, int q2, const zd* __restrict__ B2, int64_t ldb2) {
+            int64_t ldy2, int64_t sY2, int q2, const zd* __restrict__ B2, int64_t ldb2, double alpha) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t row = (int64_t)blockIdx.x * TG_ROWS + lane;
@@ -63,6 +63,7 @@ k_tall_gemm(int64_t n, TallOperands ops, const zd* __restrict__ B, int64_t ldb, 
                 for (int v = 0; v < TG_WAVES - 1; ++v) s = zadd(s, red[v][lane][j]);
                 if (pass == 0) {
                     zd* y = Y + (int64_t)b * sY + row + ldy * (c0 + j);
+                    s = zscale(s, alpha);
                     *y = (beta != 0.0) ? zadd(zscale(*y, beta), s) : s;
                 } else {
                     Y2[(int64_t)b * sY2 + row + ldy2 * (c0 + j)] = s;
@@ -415,15 +416,16 @@ k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int r, int p, const zd* _
 
 void launch_tall_gemm(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
                       int64_t sB, int k, zd* Y, int64_t ldy, int64_t sY, double beta, int batch,
-                      zd* Y2, int64_t ldy2, int64_t sY2, int q2, const zd* B2, int64_t ldb2) {
+                      zd* Y2, int64_t ldy2, int64_t sY2, int q2, const zd* B2, int64_t ldb2,
+                      double alpha) {
     if (k <= 0 || n <= 0 || batch <= 0) return;
     const int gx = cdiv(n, TG_ROWS);
     if (k <= 8) {
         hipLaunchKernelGGL(k_tall_gemm<8>, dim3(gx, 1, batch), dim3(256), 0, st, n, ops, B, ldb, sB,
-                           k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2);
+                           k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2, alpha);
     } else {
         hipLaunchKernelGGL(k_tall_gemm<16>, dim3(gx, cdiv(k, 16), batch), dim3(256), 0, st, n, ops,
-                           B, ldb, sB, k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2);
+                           B, ldb, sB, k, Y, ldy, sY, beta, Y2, ldy2, sY2, q2, B2, ldb2, alpha);
     }
     LAUNCH_CHECK();
 }
@@ -532,5 +534,206 @@ void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int nar
     for (int i = 0; i < narr; ++i) ra.a[i] = arrays[i];
     hipLaunchKernelGGL(k_block_rot_cols, dim3(cdiv(n, 256), narr), dim3(256), 0, st, n, ra, ld, r, p,
                        U);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched Gram of column blocks: C_b = A_b^H A_b with A_b = A + lda * coff[b] (n x k_b,
+// k_b <= 32), written to C + b*sC (ld ldc); entries beyond k_b are left untouched.
+// ---------------------------------------------------------------------------------------
+namespace {
+__global__ void __launch_bounds__(256)
+k_gram_self_batched_partial(int64_t n, int64_t rows_per_chunk, const zd* __restrict__ A, int64_t lda,
+                            const int* __restrict__ coff, const int* __restrict__ kk,
+                            zd* __restrict__ part) {
+    __shared__ zd As[64][33];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.y;
+    const int k = kk[b];
+    const zd* Ab = A + lda * coff[b];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = min(n, r0 + rows_per_chunk);
+    zd acc[4];
+    for (int e = 0; e < 4; ++e) acc[e] = zzero();
+    for (int64_t t0 = r0; t0 < r1; t0 += 64) {
+        const int rows = (int)min((int64_t)64, r1 - t0);
+        for (int idx = tid; idx < 64 * 32; idx += 256) {
+            int i = idx & 63, c = idx >> 6;
+            As[i][c] = (i < rows && c < k) ? Ab[t0 + i + lda * c] : zzero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = tid + 256 * e;
+            int a = idx & 31, bb = idx >> 5;
+            if (a < k && bb < k) {
+                zd s = acc[e];
+                for (int i = 0; i < rows; ++i) s = zcfma(As[i][a], As[i][bb], s);
+                acc[e] = s;
+            }
+        }
+        __syncthreads();
+    }
+    zd* pp = part + ((int64_t)blockIdx.x * gridDim.y + b) * 1024;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pp[tid + 256 * e] = acc[e];
+}
+
+__global__ void k_gram_self_batched_reduce(int nchunk, int B, const int* __restrict__ kk,
+                                           const zd* __restrict__ part, zd* __restrict__ C,
+                                           int64_t ldc, int64_t sC) {
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = kk[b];
+    if (idx >= 1024) return;
+    const int a = idx & 31, bb = idx >> 5;
+    if (a >= k || bb >= k) return;
+    zd s = zzero();
+    for (int c = 0; c < nchunk; ++c) s = zadd(s, part[((int64_t)c * B + b) * 1024 + idx]);
+    C[(int64_t)b * sC + a + ldc * bb] = s;
+}
+}  // namespace
+
+size_t gram_self_batched_elems(int64_t n, int B) { return (size_t)gram_chunks(n) * B * 1024; }
+
+void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t lda, const int* coff,
+                              const int* kk, int B, zd* C, int64_t ldc, int64_t sC, zd* part) {
+    const int nch = gram_chunks(n);
+    int64_t rpc = ((n + nch - 1) / nch + 63) / 64 * 64;
+    const int nch_eff = (int)((n + rpc - 1) / rpc);
+    hipLaunchKernelGGL(k_gram_self_batched_partial, dim3(nch_eff, B), dim3(256), 0, st, n, rpc, A, lda,
+                       coff, kk, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gram_self_batched_reduce, dim3(4, B), dim3(256), 0, st, nch_eff, B, kk, part, C,
+                       ldc, sC);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Persistent blocked triangular solve with the shifted Schur factor (one launch per solve).
+//   upper = 1:  X <- (T - zI)^{-1} X        (back substitution, blocks last to first)
+//   upper = 0:  X <- (T - zI)^{-H} X        (forward substitution with the lower factor)
+// One workgroup per 64-row block.  Blocks are claimed through a ticket counter in start
+// order, so a workgroup only ever waits for blocks claimed by workgroups that are already
+// running.  Block j publishes its solution by storing `epoch` into flag[j] after a device
+// fence; readers spin (bounded) on an agent-scope atomic load, then fence.  The counter is
+// never reset: tickets of solve e are [e*nblk, (e+1)*nblk) because launches on one stream
+// run in order.  A spin that exceeds its bound sets *err and the kernel still terminates.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int PT_KC = 8;
+__global__ void __launch_bounds__(256)
+k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
+                  zd* X, int64_t ldx, int k, int upper, unsigned int* counter, unsigned int* flags,
+                  unsigned int epoch, int* err) {
+    __shared__ zd Ts[64][65];
+    __shared__ zd Ds[64][65];
+    __shared__ zd xs[64][PT_KC];
+    __shared__ zd bj[64][PT_KC];
+    __shared__ unsigned int s_ticket;
+    __shared__ int s_bail;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int nblk = (int)((n + 63) / 64);
+    if (tid == 0) {
+        s_ticket = atomicAdd(counter, 1u);
+        s_bail = 0;
+    }
+    __syncthreads();
+    const int t = (int)(s_ticket % (unsigned)nblk);
+    const int j = upper ? nblk - 1 - t : t;
+    const int64_t o = (int64_t)j * 64;
+    const int m = (int)min((int64_t)64, n - o);
+    // diagonal-block inverse (upper: Dinv_j, lower: Dinv_j^H) and right-hand side
+    const zd* Di = Dinv + (int64_t)j * 4096;
+    for (int idx = tid; idx < 4096; idx += 256) {
+        const int a = idx & 63, c = idx >> 6;
+        Ds[a][c] = upper ? Di[a + 64 * c] : zconj(Di[c + 64 * a]);
+    }
+    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
+        const int a = idx & 63, c = idx >> 6;
+        bj[a][c] = (a < m && c < k) ? X[(o + a) + ldx * c] : zzero();
+    }
+    zd acc[PT_KC];
+#pragma unroll
+    for (int c = 0; c < PT_KC; ++c) acc[c] = zzero();
+    // dependencies: upper -> blocks j+1..nblk-1 (descending), lower -> blocks 0..j-1
+    const int ndep = upper ? (nblk - 1 - j) : j;
+    for (int q = 0; q < ndep; ++q) {
+        const int kb = upper ? nblk - 1 - q : q;
+        const int64_t ko = (int64_t)kb * 64;
+        const int mk = (int)min((int64_t)64, n - ko);
+        __syncthreads();
+        // T block (independent of other workgroups): upper reads T[j, kb]; lower reads
+        // (T[kb, j])^H, i.e. Ts[a][c] = conj(T[ko + c, o + a]).
+        for (int idx = tid; idx < 4096; idx += 256) {
+            const int a = idx & 63, c = idx >> 6;
+            if (upper) {
+                Ts[a][c] = (a < m && c < mk) ? T[(o + a) + ldt * (ko + c)] : zzero();
+            } else {
+                // coalesced over c: element (row ko+c', col o+a') with c' = idx & 63
+                const int cc = idx & 63, aa = idx >> 6;
+                Ts[aa][cc] = (aa < m && cc < mk) ? zconj(T[(ko + cc) + ldt * (o + aa)]) : zzero();
+            }
+        }
+        if (tid == 0) {
+            unsigned int spins = 0;
+            while (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 24)) { s_bail = 1; atomicExch(err, 1); break; }
+            }
+            __threadfence();
+        }
+        __syncthreads();
+        if (s_bail) break;
+        for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
+            const int a = idx & 63, c = idx >> 6;
+            xs[a][c] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
+        }
+        __syncthreads();
+        // acc[lane][c] += sum over this wave's quarter of the block columns
+        for (int cc = w; cc < 64; cc += 4) {
+            const zd tv = Ts[lane][cc];
+#pragma unroll
+            for (int c = 0; c < PT_KC; ++c) acc[c] = zfma(tv, xs[cc][c], acc[c]);
+        }
+    }
+    // reduce the 4 waves' partial sums into bj (bj -= acc)
+    __syncthreads();
+    for (int v = 0; v < 4; ++v) {
+        if (w == v) {
+#pragma unroll
+            for (int c = 0; c < PT_KC; ++c) bj[lane][c] = zsub(bj[lane][c], acc[c]);
+        }
+        __syncthreads();
+    }
+    // x_j = Ds * bj
+    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
+        const int a = idx & 63, c = idx >> 6;
+        zd s = zzero();
+        if (a < m && c < k)
+            for (int q = 0; q < m; ++q) s = zfma(Ds[a][q], bj[q][c], s);
+        xs[a][c] = s;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
+        const int a = idx & 63, c = idx >> 6;
+        if (a < m && c < k) X[(o + a) + ldx * c] = xs[a][c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(&flags[j], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+}  // namespace
+
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv, zd* X,
+                            int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
+                            unsigned int epoch, int* err) {
+    if (k > PT_KC) throw RlsError{-1, "trsm: k > 8 unsupported"};
+    const int nblk = cdiv(n, 64);
+    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(256), 0, st, n, T, ldt, Dinv, X, ldx, k,
+                       upper ? 1 : 0, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
