@@ -87,7 +87,7 @@ def main():
     value = nZ * len(times) / total
 
     # plain FP64 one-shift-at-a-time solver on the leading points of the same traversal
-    npl = min(args.plain_points, nZ)
+    npl = max(1, min(args.plain_points, nZ))
     ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain", progress=1)
     t0 = time.perf_counter()
     sig_p, st_p = solver.solve(rec.Z[:npl], ctl_plain, rec)

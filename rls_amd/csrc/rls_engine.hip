@@ -469,7 +469,8 @@ struct rls_ctx {
             ops.A[0] = J5.p; ops.lda[0] = n; ops.K[0] = pr;
             ops.A[1] = V.p; ops.lda[1] = n; ops.K[1] = pr;
             ops.nops = 2;
-            launch_tall_gemm(st, n, ops, C1.p, 2 * pr, 0, tot, CzX.p, n, 0, 0.0, 1);
+            if (tot >= 32) launch_zgemm_tiled(st, n, ops, C1.p, 2 * pr, tot, CzX.p, n);
+            else launch_tall_gemm(st, n, ops, C1.p, 2 * pr, 0, tot, CzX.p, n, 0, 0.0, 1);
         }
         // per-shift Gram of CzX_ (rr_b x rr_b inside rmax x rmax)
         Gc.ensure((size_t)B * rmax * rmax);
@@ -515,8 +516,15 @@ struct rls_ctx {
             ops.A[2] = J5.p; ops.lda[2] = n; ops.K[2] = pr;
             ops.A[3] = V.p; ops.lda[3] = n; ops.K[3] = pr;
             ops.nops = 4;
-            launch_tall_gemm(st, n, ops, CW.p, 4 * pr, 0, B * r, Wr.p, n, 0, 0.0, 1, Xr.p, n, 0, 3, CG.p,
-                             pr);
+            if (B * r >= 32) {
+                launch_zgemm_tiled(st, n, ops, CW.p, 4 * pr, B * r, Wr.p, n);
+                TallOperands ov{};
+                ov.A[0] = V.p; ov.lda[0] = n; ov.K[0] = pr; ov.nops = 1;
+                launch_zgemm_tiled(st, n, ov, CG.p, pr, B * r, Xr.p, n);
+            } else {
+                launch_tall_gemm(st, n, ops, CW.p, 4 * pr, 0, B * r, Wr.p, n, 0, 0.0, 1, Xr.p, n, 0, 3, CG.p,
+                                 pr);
+            }
         }
         nrm2.ensure((size_t)B * r);
         part.ensure(gram_partial_elems(n, B * r, 1) + 8);
@@ -809,6 +817,7 @@ int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_
         DArr<zd> dA, dW, dG;
         DArr<double> dl, dt;
         DArr<int> drr;
+        (void)hipGetLastError();
         dA.ensure((size_t)n * n);
         dW.ensure(heev_workspace_elems(n, nvec));
         dG.ensure((size_t)n * nvec);
@@ -824,6 +833,7 @@ int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_
         dA.release(); dW.release(); dG.release(); dl.release(); dt.release(); drr.release();
         return RLS_OK;
     });
+    if (rc) fprintf(stderr, "rls_test_heev: %s\n", ctx->err.c_str());
     rls_ctx_destroy(ctx);
     return rc;
 }
@@ -870,3 +880,45 @@ int rls_test_tall_qr(int64_t n, int k, double* X, double* R) {
 }
 
 }  // extern "C"
+
+extern "C" int rls_test_heev_batch_time(int n, const double* A, int batch, int reps, int r, int nvec,
+                                        int mode_fixed, double* ms_out) {
+    rls_ctx* ctx = nullptr;
+    int rc = rls_ctx_create(0, &ctx);
+    if (rc) return rc;
+    rc = guarded(ctx, [&]() -> int {
+        DArr<zd> dA, dW, dG;
+        DArr<double> dl, dt;
+        DArr<int> drr;
+        dA.ensure((size_t)n * n * batch);
+        dW.ensure(heev_workspace_elems(n, nvec) * batch);
+        dG.ensure((size_t)n * nvec * batch);
+        dl.ensure((size_t)n * batch);
+        dt.ensure(batch);
+        drr.ensure(batch);
+        for (int b = 0; b < batch; ++b) ctx->h2d(dA.p + (size_t)b * n * n, reinterpret_cast<const zd*>(A), (size_t)n * n);
+        std::vector<double> taus(batch, 1e-10);
+        ctx->h2d(dt.p, taus.data(), batch);
+        launch_heev_rr(ctx->st, dA.p, n, (int64_t)n * n, n, batch, dW.p, dl.p, n, dG.p, n, (int64_t)n * nvec, drr.p, dt.p, r, nvec, mode_fixed);
+        ctx->sync();
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, ctx->st));
+        for (int it = 0; it < reps; ++it)
+            launch_heev_rr(ctx->st, dA.p, n, (int64_t)n * n, n, batch, dW.p, dl.p, n, dG.p, n, (int64_t)n * nvec, drr.p, dt.p, r, nvec, mode_fixed);
+        HIP_TRY(hipEventRecord(e1, ctx->st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *ms_out = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        dA.release(); dW.release(); dG.release(); dl.release(); dt.release(); drr.release();
+        return RLS_OK;
+    });
+    rls_ctx_destroy(ctx);
+    return rc;
+}
+
+extern "C" void rls_test_heev_phases(long long* out) { read_heev_phases(out); }

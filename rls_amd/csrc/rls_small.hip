@@ -10,6 +10,7 @@
 //     src/types/RLSLinearAlgebra.jl:164-191, and the LQ of svdRaylRitz!, lobpcgSvd.jl:171-179)
 #include "rls_common.h"
 #include "rls_kernels.h"
+#include <algorithm>
 
 // ---------------------------------------------------------------------------------------
 // Hermitian eigensolver.
@@ -20,7 +21,7 @@
 // ---------------------------------------------------------------------------------------
 namespace {
 
-constexpr int HEEV_THREADS = 512;
+constexpr int HEEV_THREADS = 1024;
 
 __device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
     // number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2
@@ -85,11 +86,12 @@ __device__ void tri_shifted_solve(const double* d, const double* e, int n, doubl
 // Eigenvalues with indices idx[0..k) (0-based, ascending order) of the symmetric tridiagonal
 // (d, e2 = e^2) by multisection: each wanted index gets blockDim/k threads that evaluate
 // Sturm counts at equally spaced interior points of its bracket every round.
+template <int NT = HEEV_THREADS>
 __device__ void multisect(const double* d, const double* e2, int n, const int* idx, int k,
                           double* lam, double gl, double gu, double pivmin) {
     __shared__ double s_lo[64], s_hi[64];
-    __shared__ double s_x[HEEV_THREADS];
-    __shared__ int s_c[HEEV_THREADS];
+    __shared__ double s_x[NT];
+    __shared__ int s_c[NT];
     __shared__ int s_active;
     const double eps = 2.220446049250313e-16;
     const int tid = threadIdx.x;
@@ -131,6 +133,10 @@ __device__ void multisect(const double* d, const double* e2, int n, const int* i
 
 }  // namespace
 
+// Phase timestamps of problem 0 of the last k_heev_rr launch (s_memrealtime ticks, 100 MHz):
+// [0] start, [1] tridiagonal done, [2] eigenvalues done, [3] inverse iteration done, [4] end.
+__device__ long long g_heev_phase[8];
+
 // A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
 //    batch stride sA (elements).  Wk: per-problem workspace (heev_workspace_elems).
 // Outputs: lam (n per problem, ascending; entries not needed by the rule stay NaN),
@@ -164,7 +170,9 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __shared__ zd pw[1024];
     __shared__ int s_rr;
     __shared__ double s_part[HEEV_THREADS];
+    __shared__ zd s_acc[HEEV_THREADS / 256][256];
 
+    if (b == 0 && tid == 0) g_heev_phase[0] = wall_clock64();
     // 1. copy with Hermitian completion from the upper triangle
     for (int64_t idx = tid; idx < (int64_t)n * n; idx += nthr) {
         int i = (int)(idx % n), j = (int)(idx / n);
@@ -221,32 +229,49 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
         const bool has_t = !(t.x == 0.0 && t.y == 0.0);
-        for (int i = tid; i < m; i += nthr) {
-            zd acc0 = zzero(), acc1 = zzero();
-            const zd pvi = pending ? pv[i + 1] : zzero();
-            const zd pwi = pending ? pw[i + 1] : zzero();
-            int j = 0;
-            for (; j + 1 < m; j += 2) {
-                zd a0 = A22[i + (int64_t)n * j];
-                zd a1 = A22[i + (int64_t)n * (j + 1)];
-                if (pending) {
-                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
-                    a1 = zsub(a1, zadd(zmul(pvi, zconj(pw[j + 2])), zmul(pwi, zconj(pv[j + 2]))));
-                    A22[i + (int64_t)n * j] = a0;
-                    A22[i + (int64_t)n * (j + 1)] = a1;
+        // 2-D split: thread (row gi, column group gg); groups reduce through LDS
+        {
+            const int RG = 256;
+            const int NG = nthr / RG;
+            const int gi = tid % RG, gg = tid / RG;
+            for (int i0 = 0; i0 < m; i0 += RG) {
+                const int i = i0 + gi;
+                zd acc0 = zzero(), acc1 = zzero();
+                if (i < m) {
+                    const zd pvi = pending ? pv[i + 1] : zzero();
+                    const zd pwi = pending ? pw[i + 1] : zzero();
+                    int j = gg;
+                    for (; j + NG < m; j += 2 * NG) {
+                        const int j1 = j + NG;
+                        zd a0 = A22[i + (int64_t)n * j];
+                        zd a1 = A22[i + (int64_t)n * j1];
+                        if (pending) {
+                            a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                            a1 = zsub(a1, zadd(zmul(pvi, zconj(pw[j1 + 1])), zmul(pwi, zconj(pv[j1 + 1]))));
+                            A22[i + (int64_t)n * j] = a0;
+                            A22[i + (int64_t)n * j1] = a1;
+                        }
+                        acc0 = zfma(a0, sv[j], acc0);
+                        acc1 = zfma(a1, sv[j1], acc1);
+                    }
+                    for (; j < m; j += NG) {
+                        zd a0 = A22[i + (int64_t)n * j];
+                        if (pending) {
+                            a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                            A22[i + (int64_t)n * j] = a0;
+                        }
+                        acc0 = zfma(a0, sv[j], acc0);
+                    }
                 }
-                acc0 = zfma(a0, sv[j], acc0);
-                acc1 = zfma(a1, sv[j + 1], acc1);
-            }
-            for (; j < m; ++j) {
-                zd a0 = A22[i + (int64_t)n * j];
-                if (pending) {
-                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
-                    A22[i + (int64_t)n * j] = a0;
+                s_acc[gg][gi] = zadd(acc0, acc1);
+                __syncthreads();
+                if (gg == 0 && i < m) {
+                    zd sacc = s_acc[0][gi];
+                    for (int g2 = 1; g2 < NG; ++g2) sacc = zadd(sacc, s_acc[g2][gi]);
+                    sw[i] = zmul(t, sacc);
                 }
-                acc0 = zfma(a0, sv[j], acc0);
+                __syncthreads();
             }
-            sw[i] = zmul(t, zadd(acc0, acc1));
         }
         __syncthreads();
         if (!has_t) {
@@ -280,6 +305,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __syncthreads();
     for (int i = tid; i < n; i += nthr) e2[i] = e[i] * e[i];
     __syncthreads();
+    if (b == 0 && tid == 0) g_heev_phase[1] = wall_clock64();
 
     // 3. Gershgorin interval
     double gl = 1e300, gu = -1e300;
@@ -367,6 +393,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __syncthreads();
     for (int i = tid; i < n; i += nthr) lo_[i] = lam[i];
     const int nv = s_rr < nvec_max ? s_rr : nvec_max;
+    if (b == 0 && tid == 0) g_heev_phase[2] = wall_clock64();
 
     // 5. inverse iteration for eigenvectors 0..nv-1 (thread j owns vector j); vectors whose
     //    eigenvalues are closer than ortol are re-orthogonalised (one wave per cluster).
@@ -417,6 +444,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         __syncthreads();
     }
 
+    if (b == 0 && tid == 0) g_heev_phase[3] = wall_clock64();
     // 6. back-transformation G = H(0) H(1) ... H(n-2) y, y <- H(k) y for k descending.
     //    Dots of all vectors with v_k in one pass: thread (j, slice) partial sums.
     zd* Gb = G + (int64_t)b * sG;
@@ -468,18 +496,58 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
             __syncthreads();
         }
     }
+    if (b == 0 && tid == 0) g_heev_phase[4] = wall_clock64();
 }
 
-size_t heev_workspace_elems(int n, int nvec_max) {
-    // complex elements per problem
-    size_t doubles = (size_t)8 * n + (size_t)n * nvec_max + (size_t)4 * n * nvec_max;
-    return (size_t)n * n + (doubles + 1) / 2 + 8;
+void read_heev_phases(long long* out) {
+    (void)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_heev_phase), sizeof(long long) * 5);
 }
+
+size_t heev_wave_workspace_elems(int n, int nvec_max);
+
+size_t heev_workspace_elems(int n, int nvec_max) {
+    // complex elements per problem (large-n kernel layout; covers the wave kernel's too)
+    size_t doubles = (size_t)8 * n + (size_t)n * nvec_max + (size_t)4 * n * nvec_max;
+    size_t big = (size_t)n * n + (doubles + 1) / 2 + 8;
+    size_t wave = heev_wave_workspace_elems(n, nvec_max);
+    return big > wave ? big : wave;
+}
+
+__global__ void __launch_bounds__(256)
+k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restrict__ Wk,
+            int64_t sW, double* __restrict__ lam_out, int64_t sL, zd* __restrict__ G, int64_t ldg,
+            int64_t sG, int* __restrict__ rr_out, const double* __restrict__ taus, int r,
+            int nvec_max, int mode_fixed);
+size_t heev_wave_workspace_elems(int n, int nvec_max);
 
 void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
                     zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
                     const double* taus, int r, int nvec_max, int mode_fixed) {
     if (n > 1023) throw RlsError{-1, "heev: n > 1023 unsupported"};
+    if (n <= 128) {
+        // wave-per-matrix LDS kernel; its workspace fits inside the general one
+        int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
+        size_t lds = (size_t)n * (n + 1) / 2 * sizeof(zd);               // packed matrix
+        lds = std::max(lds, (size_t)2 * n * 64 * sizeof(double));       // Y and D chunks
+        lds = std::max(lds, (size_t)(n + 1) * 64 * sizeof(zd));          // G chunk
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipFuncAttributes fa;
+            size_t stat = 0;
+            if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_heev_wave)) == hipSuccess)
+                stat = fa.sharedSizeBytes;
+            const size_t want = (size_t)128 * 129 / 2 * sizeof(zd);
+            if (stat + want > 160 * 1024) throw RlsError{-2, "heev: LDS budget exceeded"};
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_wave),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
+            (void)hipGetLastError();   // an unsupported attribute must not poison later checks
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(k_heev_wave, dim3(batch), dim3(256), lds, st, A, lda, sA, n, Wk, sW, lam, sL,
+                           G, ldg, sG, rr, taus, r, nvec_max, mode_fixed);
+        LAUNCH_CHECK();
+        return;
+    }
     int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
     hipLaunchKernelGGL(k_heev_rr, dim3(batch), dim3(HEEV_THREADS), 0, st, A, lda, sA, n, Wk, sW,
                        lam, sL, G, ldg, sG, rr, taus, r, nvec_max, mode_fixed);
@@ -1018,92 +1086,86 @@ __device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo, double* red) {
     (void)red;
 }
 
-// Small side of _block_unitary_update! (recySpace.jl:226-286): for i = 0..p-2, U_i from the
-// QR of [L[pb,pb]; L[i,pb]] (rls_block_unitary_rot!, RLSLinearAlgebra.jl:164-191), then
-// L rows (pb,i) <- U_i [L_pb; L_i], H <- U_i H U_i^H on blocks (pb,i) for H1 and H2.
-// Writes the U_i sequence to Uout (p-1 matrices of 2r x 2r) for the column pass.
+// Small side of _block_unitary_update! (recySpace.jl:226-286).  Rotation i only touches
+// block rows/columns (pb, i), and row block i of the newest block column is untouched by
+// rotations before i, so the U_i sequence depends only on the evolving r x r block
+// L[pb, pb]: U_i = Q^H from the QR of [L[pb,pb]; L[i,pb]] (rls_block_unitary_rot!,
+// RLSLinearAlgebra.jl:164-191) and L[pb,pb] <- R.  One workgroup computes the sequence;
+// applying it to L, H1, H2 and the tall arrays is done by the row/column passes.
 __global__ void __launch_bounds__(256)
-k_block_rot_small(zd* __restrict__ L, zd* __restrict__ H1, zd* __restrict__ H2, int64_t ld, int p,
-                  int r, zd* __restrict__ Uout) {
+k_block_rot_seq(const zd* __restrict__ L, int64_t ld, int p, int r, zd* __restrict__ Uout) {
     __shared__ zd Y[32 * 16];
     __shared__ zd Q[32 * 32];
+    __shared__ zd rend[16 * 16];
     __shared__ double red[4];
     const int tid = threadIdx.x;
     const int pb = p - 1;
-    const int pr = p * r;
     const int m = 2 * r;
+    for (int idx = tid; idx < r * r; idx += blockDim.x) {
+        const int a = idx % r, c = idx / r;
+        rend[a + r * c] = L[(pb * r + a) + ld * (pb * r + c)];
+    }
+    __syncthreads();
     for (int i = 0; i < pb; ++i) {
         for (int idx = tid; idx < m * r; idx += blockDim.x) {
-            int a = idx % m, c = idx / m;
-            int row = (a < r) ? pb * r + a : i * r + (a - r);
-            Y[a + m * c] = L[row + ld * (pb * r + c)];
+            const int a = idx % m, c = idx / m;
+            Y[a + m * c] = (a < r) ? rend[a + r * c] : L[(i * r + a - r) + ld * (pb * r + c)];
         }
         __syncthreads();
         zd* U = Uout + (int64_t)i * m * m;
         lds_block_unitary(Y, r, Q, U, red);
-        __syncthreads();
-        // lmul! on L rows (pb, i), all pr columns
-        for (int c = tid; c < pr; c += blockDim.x) {
-            zd a1[16], a2[16];
-            for (int s = 0; s < r; ++s) { a1[s] = L[pb * r + s + ld * c]; a2[s] = L[i * r + s + ld * c]; }
-            for (int t = 0; t < r; ++t) {
-                zd s1 = zzero(), s2 = zzero();
-                for (int s = 0; s < r; ++s) {
-                    s1 = zfma(U[t + m * s], a1[s], s1);
-                    s1 = zfma(U[t + m * (r + s)], a2[s], s1);
-                    s2 = zfma(U[(r + t) + m * s], a1[s], s2);
-                    s2 = zfma(U[(r + t) + m * (r + s)], a2[s], s2);
-                }
-                L[pb * r + t + ld * c] = s1;
-                L[i * r + t + ld * c] = s2;
-            }
+        // Y now holds [R; ~0] (Householder applied in place): rend <- R
+        for (int idx = tid; idx < r * r; idx += blockDim.x) {
+            const int a = idx % r, c = idx / r;
+            rend[a + r * c] = (a <= c) ? Y[a + m * c] : zzero();
         }
         __syncthreads();
-        for (int hsel = 0; hsel < 2; ++hsel) {
-            zd* H = hsel == 0 ? H1 : H2;
-            // rmul!(H, adjoint(bu)): columns (pb, i) <- [H_pb H_i] U^H
-            for (int row = tid; row < pr; row += blockDim.x) {
-                zd a1[16], a2[16];
-                for (int s = 0; s < r; ++s) { a1[s] = H[row + ld * (pb * r + s)]; a2[s] = H[row + ld * (i * r + s)]; }
-                for (int t = 0; t < r; ++t) {
-                    zd s1 = zzero(), s2 = zzero();
-                    for (int s = 0; s < r; ++s) {
-                        s1 = zfma(a1[s], zconj(U[t + m * s]), s1);
-                        s1 = zfma(a2[s], zconj(U[t + m * (r + s)]), s1);
-                        s2 = zfma(a1[s], zconj(U[(r + t) + m * s]), s2);
-                        s2 = zfma(a2[s], zconj(U[(r + t) + m * (r + s)]), s2);
-                    }
-                    H[row + ld * (pb * r + t)] = s1;
-                    H[row + ld * (i * r + t)] = s2;
-                }
-            }
-            __syncthreads();
-            // lmul!(bu, H): rows (pb, i) <- U [H_pb; H_i]
-            for (int c = tid; c < pr; c += blockDim.x) {
-                zd a1[16], a2[16];
-                for (int s = 0; s < r; ++s) { a1[s] = H[pb * r + s + ld * c]; a2[s] = H[i * r + s + ld * c]; }
-                for (int t = 0; t < r; ++t) {
-                    zd s1 = zzero(), s2 = zzero();
-                    for (int s = 0; s < r; ++s) {
-                        s1 = zfma(U[t + m * s], a1[s], s1);
-                        s1 = zfma(U[t + m * (r + s)], a2[s], s1);
-                        s2 = zfma(U[(r + t) + m * s], a1[s], s2);
-                        s2 = zfma(U[(r + t) + m * (r + s)], a2[s], s2);
-                    }
-                    H[pb * r + t + ld * c] = s1;
-                    H[i * r + t + ld * c] = s2;
-                }
-            }
-            __syncthreads();
-        }
     }
+}
+
+// Left application, column by column: for each column c < ncols and i = 0..p-2,
+// [A_pb; A_i](:, c) <- U_i [A_pb; A_i](:, c)   (lmul!(bu, A), RLSLinearAlgebra.jl:60-92).
+struct SmallArrays { zd* a[4]; };
+__global__ void __launch_bounds__(256)
+k_block_rot_rows(int ncols, SmallArrays arr, int64_t ld, int r, int p, const zd* __restrict__ U) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    zd* A = arr.a[blockIdx.y] + ld * c;
+    const int pb = p - 1, m = 2 * r;
+    zd ap[16], ai[16], np_[16];
+    for (int s2 = 0; s2 < r; ++s2) ap[s2] = A[pb * r + s2];
+    for (int i = 0; i < pb; ++i) {
+        const zd* Ui = U + (int64_t)i * m * m;
+        for (int s2 = 0; s2 < r; ++s2) ai[s2] = A[i * r + s2];
+        for (int t = 0; t < r; ++t) {
+            zd s1 = zzero(), sb = zzero();
+            for (int q = 0; q < r; ++q) {
+                s1 = zfma(Ui[t + m * q], ap[q], s1);
+                s1 = zfma(Ui[t + m * (r + q)], ai[q], s1);
+                sb = zfma(Ui[(r + t) + m * q], ap[q], sb);
+                sb = zfma(Ui[(r + t) + m * (r + q)], ai[q], sb);
+            }
+            np_[t] = s1;
+            A[i * r + t] = sb;
+        }
+        for (int q = 0; q < r; ++q) ap[q] = np_[q];
+    }
+    for (int s2 = 0; s2 < r; ++s2) A[pb * r + s2] = ap[s2];
 }
 
 void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, int p, int r, zd* U) {
     if (p <= 1) return;
     if (r > 16) throw RlsError{-1, "block rotation: r > 16 unsupported"};
-    hipLaunchKernelGGL(k_block_rot_small, dim3(1), dim3(256), 0, st, L, H1, H2, ld, p, r, U);
+    hipLaunchKernelGGL(k_block_rot_seq, dim3(1), dim3(256), 0, st, L, ld, p, r, U);
     LAUNCH_CHECK();
+    const int pr = p * r;
+    SmallArrays sa{};
+    sa.a[0] = L; sa.a[1] = H1; sa.a[2] = H2;
+    hipLaunchKernelGGL(k_block_rot_rows, dim3(cdiv(pr, 256), 3), dim3(256), 0, st, pr, sa, ld, r, p, U);
+    LAUNCH_CHECK();
+    // right application to H1, H2 (H <- H U^H), same kernel as the tall arrays
+    zd* hs[2] = {H1, H2};
+    launch_block_rot_cols(st, pr, hs, 2, ld, r, p, U);
 }
 
 // D[i + ldd*j] = conj(S[j + lds*i])  (m x k block of D from the k x m block of S)
@@ -1222,4 +1284,392 @@ void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* si
                        int B) {
     hipLaunchKernelGGL(k_lam_to_sig, dim3(B), dim3(64), 0, st, lam, sL, sig, sS, r);
     LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Wave-per-matrix Hermitian eigensolver for n <= 128 (the common size of the recycled
+// Rayleigh-Ritz problem).  One 64-lane workgroup per matrix; the lower triangle lives in LDS
+// (packed by columns), so the tridiagonalisation runs at LDS latency with wave-level
+// reductions and single-wave barriers.  Same contract as k_heev_rr.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int HW_NMAX = 128;
+
+__device__ __forceinline__ int poff(int n, int j) { return j * n - (j * (j - 1)) / 2; }
+
+__device__ int sturm_count_rcp(const double* d, const double* e2, int n, double x, double pivmin) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0);
+    for (int i = 1; i < n; ++i) {
+        double rq = __builtin_amdgcn_rcp(q);
+        rq = rq * fma(-q, rq, 2.0);            // one Newton step
+        q = (d[i] - x) - e2[i - 1] * rq;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += (q < 0);
+    }
+    return cnt;
+}
+
+// wave multisection for wanted indices idx[0..k) (k <= 64)
+__device__ void multisect_wave(const double* d, const double* e2, int n, const int* idx, int k,
+                               double* lam, double gl, double gu, double pivmin, double* s_lo,
+                               double* s_hi, double* s_x, int* s_c) {
+    const double eps = 2.220446049250313e-16;
+    const int lane = threadIdx.x;
+    const int G = 64 / k;
+    if (lane < k) { s_lo[lane] = gl; s_hi[lane] = gu; }
+    __syncthreads();
+    for (int round = 0; round < 400; ++round) {
+        const int j = lane / G, t = lane % G;
+        if (j < k) {
+            double lo = s_lo[j], hi = s_hi[j];
+            double x = lo + (hi - lo) * (double)(t + 1) / (double)(G + 1);
+            s_x[lane] = x;
+            s_c[lane] = (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin)
+                            ? -1 : sturm_count_rcp(d, e2, n, x, pivmin);
+        }
+        __syncthreads();
+        int active = 0;
+        if (lane < k) {
+            const int base = lane * G;
+            if (s_c[base] >= 0) {
+                double lo = s_lo[lane], hi = s_hi[lane];
+                for (int q = 0; q < G; ++q) {
+                    if (s_c[base + q] <= idx[lane]) lo = s_x[base + q];
+                    else { hi = s_x[base + q]; break; }
+                }
+                s_lo[lane] = lo; s_hi[lane] = hi;
+                active = 1;
+            }
+        }
+        active = __any(active);
+        __syncthreads();
+        if (!active) break;
+    }
+    if (lane < k) lam[idx[lane]] = 0.5 * (s_lo[lane] + s_hi[lane]);
+    __syncthreads();
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restrict__ Wk,
+            int64_t sW, double* __restrict__ lam_out, int64_t sL, zd* __restrict__ G, int64_t ldg,
+            int64_t sG, int* __restrict__ rr_out, const double* __restrict__ taus, int r,
+            int nvec_max, int mode_fixed) {
+    extern __shared__ zd Lp[];                       // packed lower triangle, n(n+1)/2
+    __shared__ zd sv[HW_NMAX], pv[HW_NMAX + 1], pw[HW_NMAX + 1], tau[HW_NMAX];
+    __shared__ double d[HW_NMAX], e[HW_NMAX], e2[HW_NMAX], lam[HW_NMAX];
+    __shared__ int s_c[64], s_idx[64];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int tid0 = threadIdx.x;
+    const zd* Ab = A + (int64_t)b * sA;
+    // workspace: packed reflectors (n(n+1)/2 complex), then Y (n x nvec_max real)
+    zd* Hh = Wk + (int64_t)b * sW;
+    double* Y = reinterpret_cast<double*>(Hh + (int64_t)n * (n + 1) / 2);
+
+    if (b == 0 && tid0 == 0) g_heev_phase[0] = wall_clock64();
+    // load the lower triangle from the upper one (heevr uplo='U'): column i of A holds
+    // A[j, i], j <= i, contiguous; L[i, j] = conj(A[j, i])
+    for (int i = 0; i < n; ++i) {
+        for (int j = tid0; j <= i; j += 256) {
+            zd v = zconj(Ab[j + lda * i]);
+            if (i == j) v.y = 0.0;
+            Lp[poff(n, j) + (i - j)] = v;
+        }
+    }
+    if (tid0 == 0) { pv[0] = zzero(); pw[0] = zzero(); }
+    __syncthreads();
+    // 4 waves: thread (row lane + 64 q, column group wv) for the trailing-matrix passes
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6;
+    __shared__ zd s_acc4[4][HW_NMAX];
+    __shared__ double s_red[4];
+    bool pending = false;
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;
+        const int ok = poff(n, k);
+        if (pending) {
+            const zd w0c = zconj(pw[0]), v0c = zconj(pv[0]);
+            for (int i = tid; i < m; i += 256) {
+                zd a = Lp[ok + 1 + i];
+                a = zsub(a, zadd(zmul(pv[i + 1], w0c), zmul(pw[i + 1], v0c)));
+                Lp[ok + 1 + i] = a;
+            }
+            if (tid == 0) Lp[ok].x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+            for (int q = 0; q < 2; ++q) {
+                const int i = lane + 64 * q;
+                if (i >= m) continue;
+                const zd pvi = pv[i + 1], pwi = pw[i + 1];
+                int j = wv;
+                for (; j + 4 <= i; j += 8) {
+                    const int o0 = poff(n, k + 1 + j) + (i - j);
+                    const int o1 = poff(n, k + 5 + j) + (i - j - 4);
+                    zd a0 = Lp[o0], a1 = Lp[o1];
+                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                    a1 = zsub(a1, zadd(zmul(pvi, zconj(pw[j + 5])), zmul(pwi, zconj(pv[j + 5]))));
+                    if (i == j + 4) a1.y = 0.0;
+                    Lp[o0] = a0;
+                    Lp[o1] = a1;
+                }
+                for (; j <= i; j += 4) {
+                    const int o0 = poff(n, k + 1 + j) + (i - j);
+                    zd a0 = Lp[o0];
+                    a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                    if (i == j) a0.y = 0.0;
+                    Lp[o0] = a0;
+                }
+            }
+        }
+        __syncthreads();
+        double part = 0.0;
+        for (int i = 1 + tid; i < m; i += 256) part += zabs2(Lp[ok + 1 + i]);
+        const double xn2 = block_sum(part, s_red);
+        const zd alpha = Lp[ok + 1];
+        zd t = zzero();
+        double beta;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            beta = alpha.x;
+            for (int i = tid; i < m; i += 256) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zzero();
+        } else {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = (alpha.x >= 0.0) ? -nrm : nrm;
+            t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+            const zd scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+            for (int i = tid; i < m; i += 256) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zmul(Lp[ok + 1 + i], scal);
+        }
+        __syncthreads();
+        if (tid == 0) { e[k] = beta; tau[k] = t; d[k] = Lp[ok].x; }
+        for (int i = 1 + tid; i < m; i += 256) {
+            Lp[ok + 1 + i] = sv[i];
+            Hh[ok + 1 + i] = sv[i];                  // reflectors kept for the back-transform
+        }
+        __syncthreads();
+        if (t.x == 0.0 && t.y == 0.0) { pending = false; __syncthreads(); continue; }
+        // w = tau * A22 v: partial sums over columns j = wv (mod 4), two accumulators each
+        for (int q = 0; q < 2; ++q) {
+            const int i = lane + 64 * q;
+            zd acc0 = zzero(), acc1 = zzero();
+            if (i < m) {
+                int j = wv;
+                for (; j + 4 < m; j += 8) {
+                    const int j1 = j + 4;
+                    const zd a0 = (j <= i) ? Lp[poff(n, k + 1 + j) + (i - j)]
+                                           : zconj(Lp[poff(n, k + 1 + i) + (j - i)]);
+                    const zd a1 = (j1 <= i) ? Lp[poff(n, k + 1 + j1) + (i - j1)]
+                                            : zconj(Lp[poff(n, k + 1 + i) + (j1 - i)]);
+                    acc0 = zfma(a0, sv[j], acc0);
+                    acc1 = zfma(a1, sv[j1], acc1);
+                }
+                for (; j < m; j += 4) {
+                    const zd a0 = (j <= i) ? Lp[poff(n, k + 1 + j) + (i - j)]
+                                           : zconj(Lp[poff(n, k + 1 + i) + (j - i)]);
+                    acc0 = zfma(a0, sv[j], acc0);
+                }
+            }
+            if (i < HW_NMAX) s_acc4[wv][i] = zadd(acc0, acc1);
+        }
+        __syncthreads();
+        double pr = 0.0, pi = 0.0;
+        zd wloc = zzero();
+        if (tid < m) {
+            wloc = zadd(zadd(s_acc4[0][tid], s_acc4[1][tid]), zadd(s_acc4[2][tid], s_acc4[3][tid]));
+            wloc = zmul(t, wloc);
+            const zd c = zcmul(wloc, sv[tid]);
+            pr = c.x; pi = c.y;
+        }
+        pr = block_sum(pr, s_red);
+        pi = block_sum(pi, s_red);
+        const zd a2 = zscale(zmul(t, zmk(pr, pi)), -0.5);
+        if (tid < m) { pw[tid] = zfma(a2, sv[tid], wloc); pv[tid] = sv[tid]; }
+        pending = true;
+        __syncthreads();
+    }
+    if (tid0 == 0) {
+        zd dl = Lp[poff(n, n - 1)];
+        if (pending) dl.x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+        d[n - 1] = dl.x;
+        e[n - 1] = 0.0;
+        tau[n - 1] = zzero();
+    }
+    __syncthreads();
+    for (int i = tid0; i < n; i += 256) { e2[i] = e[i] * e[i]; lam[i] = nan(""); }
+    __syncthreads();
+    if (b == 0 && tid0 == 0) g_heev_phase[1] = wall_clock64();
+
+    // eigenvalues on demand + Ritz-dimension rule (as in k_heev_rr)
+    double gl = 1e300, gu = -1e300;
+    for (int i = 0; i < n; ++i) {
+        const double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        gl = fmin(gl, d[i] - off);
+        gu = fmax(gu, d[i] + off);
+    }
+    const double tnorm = fmax(fabs(gl), fabs(gu));
+    const double eps = 2.220446049250313e-16;
+    const double pivmin = fmax(1e-300, tnorm * 1e-300);
+    gl -= 2.0 * eps * tnorm * n + pivmin;
+    gu += 2.0 * eps * tnorm * n + pivmin;
+    int rr = 0;
+    {
+        int have = 0;
+        bool have_max = false;
+        const int n_init = (mode_fixed > 0) ? (mode_fixed < n ? mode_fixed : n)
+                                            : ((2 * r < n) ? 2 * r : n);
+        int target = n_init;
+        while (true) {
+            int kq = 0, kc = 0;
+            if (tid0 == 0) {
+                if (!have_max && mode_fixed <= 0) s_idx[kq++] = n - 1;
+                for (int i = have; i < target && kq < 64; ++i) {
+                    if (i == n - 1 && !have_max && mode_fixed <= 0) { ++kc; continue; }
+                    s_idx[kq++] = i; ++kc;
+                }
+                s_c[0] = kq; s_c[1] = kc;
+            }
+            __syncthreads();
+            kq = s_c[0]; kc = s_c[1];
+            __syncthreads();
+            multisect<256>(d, e2, n, s_idx, kq, lam, gl, gu, pivmin);
+            have_max = true;
+            have += kc;
+            if (have < target) continue;
+            bool done;
+            if (mode_fixed > 0) {
+                rr = n_init; done = true;
+            } else {
+                const double gap = lam[n - 1] / taus[b] * eps;
+                int first = 0;
+                for (int i = 0; i + 1 < target; ++i)
+                    if (fabs(lam[i] - lam[i + 1]) > gap) { first = i + 1; break; }
+                if (first > 0) { rr = first > r ? first : r; if (rr > n) rr = n; done = true; }
+                else if (target + 2 * r <= n) done = false;
+                else { rr = n; done = true; }
+            }
+            if (done) break;
+            target += 2 * r;
+        }
+        const int need = rr < nvec_max ? rr : nvec_max;
+        while (have < need) {
+            int kq = 0;
+            if (tid0 == 0) {
+                for (int i = have; i < need && kq < 64; ++i) s_idx[kq++] = i;
+                s_c[0] = kq;
+            }
+            __syncthreads();
+            kq = s_c[0];
+            __syncthreads();
+            multisect<256>(d, e2, n, s_idx, kq, lam, gl, gu, pivmin);
+            have += kq;
+        }
+    }
+    if (tid0 == 0) rr_out[b] = rr;
+    double* lo_ = lam_out + (int64_t)b * sL;
+    for (int i = tid0; i < n; i += 256) lo_[i] = lam[i];
+    const int nv = rr < nvec_max ? rr : nvec_max;
+    if (b == 0 && tid0 == 0) g_heev_phase[2] = wall_clock64();
+    const bool wave0 = (tid0 < 64);
+
+    // Eigenvectors in chunks of 64 (lane = vector), with the LDS of the packed matrix reused:
+    //   inverse iteration on T - lam I = L D L^T (D from the Sturm recurrence, no pivoting;
+    //   tiny pivots are floored) with Y and D in LDS, cluster re-orthogonalisation per wave;
+    //   then the back-transformation G = H(0)...H(n-2) Y with the chunk of G in LDS.
+    const double ortol = 1e-9 * tnorm;
+    const double floorv = eps * tnorm + pivmin;
+    zd* Gb = G + (int64_t)b * sG;
+    double* Ys = reinterpret_cast<double*>(Lp);       // n x 64 (ld n)
+    double* Ds = Ys + (int64_t)n * 64;                // n x 64
+    for (int c0 = 0; c0 < nv; c0 += 64) {
+        const int nc = min(64, nv - c0);
+        __syncthreads();
+        const int j = lane;
+        double* y = Ys + (int64_t)j * n;
+        double* D = Ds + (int64_t)j * n;
+        if (wave0 && j < nc) {
+            const double lj = lam[c0 + j];
+            // D_i of L D L^T for T - lj I
+            double q = d[0] - lj;
+            if (fabs(q) < floorv) q = (q < 0 ? -floorv : floorv);
+            D[0] = q;
+            for (int i = 1; i < n; ++i) {
+                q = (d[i] - lj) - e2[i - 1] / q;
+                if (fabs(q) < floorv) q = (q < 0 ? -floorv : floorv);
+                D[i] = q;
+            }
+            uint32_t sd = 0x9E3779B9u * (uint32_t)(c0 + j + 1);
+            for (int i = 0; i < n; ++i) {
+                sd ^= sd << 13; sd ^= sd >> 17; sd ^= sd << 5;
+                y[i] = (double)(sd & 0xFFFFFF) / 16777216.0 - 0.5;
+            }
+        }
+        for (int it = 0; it < 3; ++it) {
+            if (wave0 && j < nc) {
+                // forward: z_i = y_i - (e_{i-1}/D_{i-1}) z_{i-1};  backward: x_i = z_i/D_i - (e_i/D_i) x_{i+1}
+                for (int i = 1; i < n; ++i) y[i] -= (e[i - 1] / D[i - 1]) * y[i - 1];
+                y[n - 1] /= D[n - 1];
+                for (int i = n - 2; i >= 0; --i) y[i] = (y[i] - e[i] * y[i + 1]) / D[i];
+                double nr = 0.0;
+                for (int i = 0; i < n; ++i) nr += y[i] * y[i];
+                nr = 1.0 / sqrt(nr);
+                for (int i = 0; i < n; ++i) y[i] *= nr;
+            }
+            __syncthreads();
+            for (int j0 = 0; wave0 && j0 < nc;) {
+                int j1 = j0 + 1;
+                while (j1 < nc && lam[c0 + j1] - lam[c0 + j1 - 1] <= ortol) ++j1;
+                for (int jj = j0 + 1; jj < j1; ++jj) {
+                    double* yj = Ys + (int64_t)jj * n;
+                    for (int qq = j0; qq < jj; ++qq) {
+                        const double* yq = Ys + (int64_t)qq * n;
+                        double sacc = 0.0;
+                        for (int i = lane; i < n; i += 64) sacc += yj[i] * yq[i];
+                        sacc = wave_sum(sacc);
+                        for (int i = lane; i < n; i += 64) yj[i] -= sacc * yq[i];
+                    }
+                    double s2 = 0.0;
+                    for (int i = lane; i < n; i += 64) s2 += yj[i] * yj[i];
+                    s2 = 1.0 / sqrt(wave_sum(s2));
+                    for (int i = lane; i < n; i += 64) yj[i] *= s2;
+                }
+                j0 = j1;
+            }
+            __syncthreads();
+        }
+        if (b == 0 && tid0 == 0) g_heev_phase[3] = wall_clock64();
+        // Y chunk -> global (coalesced), then G chunk in LDS (ld n+1)
+        for (int idx = tid0; idx < n * nc; idx += 256) Y[idx] = Ys[idx];
+        __syncthreads();
+        const int ldl = n + 1;
+        zd* Gs = Lp;
+        for (int idx = tid0; idx < n * nc; idx += 256) {
+            const int i = idx % n, jj = idx / n;
+            Gs[i + ldl * jj] = zmk(Y[idx], 0.0);
+        }
+        __syncthreads();
+        if (wave0 && j < nc) {
+            zd* g = Gs + ldl * j;
+            for (int k = n - 2; k >= 0; --k) {
+                const zd tk = tau[k];
+                if (tk.x == 0.0 && tk.y == 0.0) continue;
+                const int m = n - k - 1;
+                const zd* hv = Hh + poff(n, k) + 1;      // v[0] = 1, v[1:] at hv[1..m)
+                zd dot = g[k + 1];
+                for (int i = 1; i < m; ++i) dot = zcfma(hv[i], g[k + 1 + i], dot);
+                const zd f = zmul(tk, dot);
+                g[k + 1] = zsub(g[k + 1], f);
+                for (int i = 1; i < m; ++i) g[k + 1 + i] = zsub(g[k + 1 + i], zmul(hv[i], f));
+            }
+        }
+        __syncthreads();
+        for (int idx = tid0; idx < n * nc; idx += 256) {
+            const int i = idx % n, jj = idx / n;
+            Gb[i + ldg * (c0 + jj)] = Gs[i + ldl * jj];
+        }
+    }
+    if (b == 0 && tid0 == 0) g_heev_phase[4] = wall_clock64();
+}
+
+size_t heev_wave_workspace_elems(int n, int nvec_max) {
+    return (size_t)n * (n + 1) / 2 + ((size_t)n * 64 + 1) / 2 + 8;
 }

@@ -163,7 +163,10 @@ __global__ void k_scale_sub(int64_t n, zd* __restrict__ Y, int64_t ldy, const zd
     Y[i + ldy * j] = zsub(zscale(y, sj), zscale(x, tj));
 }
 
-// Y = T X for upper-triangular T: lane = row, waves split the column range.
+// Y = T X for upper-triangular T: workgroup = 64 rows x one 512-column slice of T (lane =
+// row, waves split the slice); slices accumulate into the pre-zeroed Y with FP64 atomics so
+// that ~n^2/(64*512) workgroups stream T in parallel.
+constexpr int TRMM_SLICE = 512;
 template <int KC>
 __global__ void __launch_bounds__(256)
 k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ X,
@@ -172,17 +175,19 @@ k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restr
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t r0 = (int64_t)blockIdx.x * 64;
     const int64_t row = r0 + lane;
-    const int c0 = blockIdx.y * KC;
-    const int kc = min(KC, k - c0);
+    const int64_t cs = (int64_t)blockIdx.y * TRMM_SLICE;
+    const int64_t ce = min(n, cs + TRMM_SLICE);
+    if (ce <= r0) return;                       // slice entirely left of the diagonal block
+    const int64_t cb = max(cs, r0);
     zd acc[KC];
 #pragma unroll
     for (int j = 0; j < KC; ++j) acc[j] = zzero();
     const bool valid = row < n;
-    for (int64_t col = r0 + w; col < n; col += 4) {
+    for (int64_t col = cb + w; col < ce; col += 4) {
         zd t = (valid && col >= row) ? T[row + ldt * col] : zzero();
 #pragma unroll
         for (int j = 0; j < KC; ++j)
-            if (j < kc) acc[j] = zfma(t, X[col + ldx * (c0 + j)], acc[j]);
+            if (j < k) acc[j] = zfma(t, X[col + ldx * j], acc[j]);
     }
     __shared__ zd red[3][64][KC];
     if (w > 0) {
@@ -191,10 +196,12 @@ k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restr
     }
     __syncthreads();
     if (w == 0 && valid) {
-        for (int j = 0; j < kc; ++j) {
-            zd s = acc[j];
-            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
-            Y[row + ldy * (c0 + j)] = s;
+        for (int j = 0; j < k; ++j) {
+            zd s2 = acc[j];
+            for (int v = 0; v < 3; ++v) s2 = zadd(s2, red[v][lane][j]);
+            double* yp = reinterpret_cast<double*>(Y + row + ldy * j);
+            atomicAdd(yp, s2.x);
+            atomicAdd(yp + 1, s2.y);
         }
     }
 }
@@ -373,6 +380,12 @@ k_trsm_h_step(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __rest
     }
 }
 
+__global__ void k_copy_cols_zero(int64_t n, zd* __restrict__ D, int64_t ldd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n) D[i + ldd * j] = zzero();
+}
+
 __global__ void k_copy_cols(int64_t n, const zd* __restrict__ S, int64_t lds, zd* __restrict__ D,
                             int64_t ldd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -473,12 +486,11 @@ void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X
 
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                        int64_t ldx, int k, zd* Y, int64_t ldy) {
-    if (k <= 8)
-        hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), 1), dim3(256), 0, st, n, T, ldt, X, ldx,
-                           k, Y, ldy);
-    else
-        hipLaunchKernelGGL(k_trmm_upper<16>, dim3(cdiv(n, 64), cdiv(k, 16)), dim3(256), 0, st, n, T,
-                           ldt, X, ldx, k, Y, ldy);
+    if (k > 8) throw RlsError{-1, "trmm: k > 8 unsupported"};
+    hipLaunchKernelGGL(k_copy_cols_zero, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, Y, ldy);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), cdiv(n, TRMM_SLICE)), dim3(256), 0, st, n, T,
+                       ldt, X, ldx, k, Y, ldy);
     LAUNCH_CHECK();
 }
 
@@ -735,5 +747,90 @@ void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt,
     const int nblk = cdiv(n, 64);
     hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(256), 0, st, n, T, ldt, Dinv, X, ldx, k,
                        upper ? 1 : 0, counter, flags, epoch, err);
+    LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Register/LDS-tiled complex FP64 GEMM for the batched recycle products:
+//   Y (n x N, ld ldy) = [A_0 .. A_{q-1}] (n x K) * B (K x N, ld ldb)
+// Workgroup tile 64 rows x 64 columns, K step 8; 256 threads, each a 4 x 4 block of outputs
+// (rows ty + 16 i, columns tx + 16 j), so every A element fetched from LDS feeds 4 columns
+// and every B element 4 rows.  The operand list is the same as k_tall_gemm's.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int ZG_BM = 64, ZG_BN = 64, ZG_BK = 8;
+__global__ void __launch_bounds__(256)
+k_zgemm_tiled(int64_t n, TallOperands ops, int Ktot, const zd* __restrict__ B, int64_t ldb, int N,
+              zd* __restrict__ Y, int64_t ldy) {
+    __shared__ zd As[ZG_BK][ZG_BM + 1];
+    __shared__ zd Bs[ZG_BK][ZG_BN + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * ZG_BM;
+    const int c0 = blockIdx.y * ZG_BN;
+    zd acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = zzero();
+    // operand boundaries
+    int qoff[TALL_MAX_OPS + 1];
+    qoff[0] = 0;
+    for (int q = 0; q < ops.nops; ++q) qoff[q + 1] = qoff[q] + ops.K[q];
+    for (int k0 = 0; k0 < Ktot; k0 += ZG_BK) {
+        // A tile: 64 rows x 8 k  (512 elements, 2 per thread), coalesced along rows
+        for (int e = tid; e < ZG_BM * ZG_BK; e += 256) {
+            const int i = e & 63, kk = e >> 6;
+            const int kg = k0 + kk;
+            zd v = zzero();
+            const int64_t row = r0 + i;
+            if (kg < Ktot && row < n) {
+                int q = 0;
+                while (kg >= qoff[q + 1]) ++q;
+                v = ops.A[q][row + ops.lda[q] * (kg - qoff[q])];
+            }
+            As[kk][i] = v;
+        }
+        // B tile: 8 k x 64 columns
+        for (int e = tid; e < ZG_BK * ZG_BN; e += 256) {
+            const int kk = e & 7, j = e >> 3;
+            const int kg = k0 + kk, cg = c0 + j;
+            Bs[kk][j] = (kg < Ktot && cg < N) ? B[kg + ldb * cg] : zzero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < ZG_BK; ++kk) {
+            zd a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t row = r0 + ty + 16 * i;
+        if (row >= n) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int cg = c0 + tx + 16 * j;
+            if (cg < N) Y[row + ldy * cg] = acc[i][j];
+        }
+    }
+}
+}  // namespace
+
+void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
+                        int N, zd* Y, int64_t ldy) {
+    if (N <= 0 || n <= 0) return;
+    int Ktot = 0;
+    for (int q = 0; q < ops.nops; ++q) Ktot += ops.K[q];
+    hipLaunchKernelGGL(k_zgemm_tiled, dim3(cdiv(n, ZG_BM), cdiv(N, ZG_BN)), dim3(256), 0, st, n, ops, Ktot,
+                       B, ldb, N, Y, ldy);
     LAUNCH_CHECK();
 }
