@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--plain-points", type=int, default=4000,
                     help="leading traversal points timed with the plain FP64 solver")
     ap.add_argument("--truth-samples", type=int, default=24)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -77,6 +77,7 @@ def main():
         log(f"warmup {w}: {time.perf_counter() - t0:.2f}s")
     times = []
     sig = st = None
+    solver.enable_kernel_timing(True)
     for s in range(args.steps):
         t0 = time.perf_counter()
         sig, st = solver.solve(rec.Z, ctl, rec)
@@ -85,6 +86,26 @@ def main():
         log(f"step {s}: {dt:.3f}s refined={int((st.lobpcg_it > 0).sum())} windows={st.n_windows}")
     total = sum(times)
     value = nZ * len(times) / total
+    kt = solver.kernel_timing()
+    solver.enable_kernel_timing(False)
+    log("kernel families: " + ", ".join(f"{k} {v['ms']:.0f} ms / {v['launches']} launches"
+                                         for k, v in kt.items()))
+    # roofline of the dominant kernel family (largest event-timed share of the step)
+    dom = max(kt, key=lambda k: kt[k]["ms"])
+    d = kt[dom]
+    avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
+    if dom == "trsm":
+        ach = d["bytes"] / max(d["launches"], 1) / avg_s / 1e9
+        roof = {"kernel": "k_trsm_persistent", "bound": "hbm", "achieved": ach, "peak": 8000.0,
+                "unit": "GB/s", "frac": ach / 8000.0, "traffic": None}
+    else:
+        ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
+        roof = {"kernel": "k_zgemm_tiled" if dom == "gemm" else "k_heev_rr/k_heev_wave",
+                "bound": "mfma", "achieved": ach, "peak": 78.6, "unit": "TFLOP/s",
+                "frac": ach / 78.6, "traffic": None}
+    roof["launches"] = d["launches"]
+    roof["avg_launch_us"] = 1e6 * avg_s
+    roof["share_of_step"] = d["ms"] / 1e3 / (total / len(times)) / len(times)
 
     # plain FP64 one-shift-at-a-time solver on the leading points of the same traversal
     npl = max(1, min(args.plain_points, nZ))
@@ -160,6 +181,8 @@ def main():
         "windows": int(st.n_windows),
         "preprocess_s": t_pre,
         "cpu_baseline": cpu,
+        "roofline": roof,
+        "kernel_families": kt,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -101,6 +101,18 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld);
 int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_control* control,
                   double* sigma_out, rls_tri_stats* stats);
 
+/* Kernel timing (no counterpart in the reference; the reference's timers are host-side
+ * time_ns() phase totals, psTriRecy.jl:264-268).  When enabled, every launch of a kernel
+ * family is bracketed with HIP events on the solver's stream and its algorithmic work
+ * (flops and bytes counted from the launch shapes) is accumulated. */
+#define RLS_KF_GEMM 0     /* batched recycled-Rayleigh-Ritz products (tiled complex GEMM) */
+#define RLS_KF_HEEV 1     /* per-shift Hermitian eigensolve of the projected matrix      */
+#define RLS_KF_TRSM 2     /* shifted triangular solve with the Schur factor               */
+#define RLS_KF_COUNT 3
+int rls_kernel_timing_enable(rls_ctx* ctx, int enable);
+int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,
+                      double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

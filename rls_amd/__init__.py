@@ -261,6 +261,22 @@ class TriSolver:
         self.n = n
         self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, n, n))
 
+    def enable_kernel_timing(self, on: bool = True):
+        """HIP-event timing of the kernel families (recycle GEMMs, eigensolves, triangular
+        solves) with their algorithmic flops/bytes; read with kernel_timing()."""
+        self.ctx.check(self.ctx.lib.rls_kernel_timing_enable(self.ctx.h, int(on)))
+
+    def kernel_timing(self) -> dict:
+        out = {}
+        for name, fam in (("gemm", _capi.RLS_KF_GEMM), ("heev", _capi.RLS_KF_HEEV),
+                          ("trsm", _capi.RLS_KF_TRSM)):
+            L = C.c_int64()
+            ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+            self.ctx.check(self.ctx.lib.rls_kernel_timing(self.ctx.h, fam, C.byref(L), C.byref(ms),
+                                                          C.byref(fl), C.byref(by)))
+            out[name] = {"launches": L.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+        return out
+
     def solve(self, Z: np.ndarray, control: PsTriRecyControl, rec: Recursion):
         Z = np.ascontiguousarray(Z, dtype=np.complex128)
         nZ = len(Z)

@@ -21,7 +21,10 @@ EXPORTED = [
     "rls_tri_control_default",
     "rls_set_tri_dense",
     "rls_tri_solve",
+    "rls_kernel_timing_enable",
+    "rls_kernel_timing",
 ]
+RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 
 
 class TriControl(C.Structure):
@@ -100,6 +103,11 @@ def load() -> C.CDLL:
     lib.rls_set_tri_dense.restype = C.c_int
     lib.rls_tri_solve.argtypes = [vp, vp, C.c_int64, C.POINTER(TriControl), vp, C.POINTER(TriStats)]
     lib.rls_tri_solve.restype = C.c_int
+    lib.rls_kernel_timing_enable.argtypes = [vp, C.c_int]
+    lib.rls_kernel_timing_enable.restype = C.c_int
+    lib.rls_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.rls_kernel_timing.restype = C.c_int
     _lib = lib
     return lib
 

@@ -85,6 +85,53 @@ struct rls_ctx {
     // last X (Ritz vectors) of the final state, for callers
     std::vector<cplx> lastX;
 
+    // Optional HIP-event timing of kernel families (recycle GEMMs, eigensolves, triangular
+    // solves) with the algorithmic flops / bytes of each launch, read by rls_kernel_timing.
+    struct KTimer {
+        std::vector<hipEvent_t> ev;   // pairs (start, stop)
+        size_t used = 0;
+        int64_t launches = 0;
+        double flops = 0.0, bytes = 0.0, ms = 0.0;
+    };
+    bool timing_on = false;
+    KTimer ktm[RLS_KF_COUNT];
+    int ktm_open = -1;
+    void kt_begin(int fam) {
+        if (!timing_on) return;
+        KTimer& k = ktm[fam];
+        if (k.used + 2 > k.ev.size()) {
+            for (int i = 0; i < 64; ++i) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                k.ev.push_back(e);
+            }
+        }
+        HIP_TRY(hipEventRecord(k.ev[k.used], st));
+        ktm_open = fam;
+    }
+    void kt_end(int fam, double flops, double bytes) {
+        if (!timing_on) return;
+        KTimer& k = ktm[fam];
+        HIP_TRY(hipEventRecord(k.ev[k.used + 1], st));
+        k.used += 2;
+        k.launches += 1;
+        k.flops += flops;
+        k.bytes += bytes;
+        ktm_open = -1;
+    }
+    void kt_collect() {
+        if (!timing_on) return;
+        sync();
+        for (auto& k : ktm) {
+            for (size_t i = 0; i + 1 < k.used; i += 2) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, k.ev[i], k.ev[i + 1]));
+                k.ms += ms;
+            }
+            k.used = 0;
+        }
+    }
+
     ~rls_ctx() {
         DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
@@ -94,6 +141,8 @@ struct rls_ctx {
         tri_ctr.release();
         tri_flags.release();
         tri_err.release();
+        for (auto& k : ktm)
+            for (auto e : k.ev) (void)hipEventDestroy(e);
         heevW2.release();
         rr2.release();
         for (auto* a : ds) a->release();
@@ -244,8 +293,12 @@ struct rls_ctx {
         for (int c0 = 0; c0 < k; c0 += 8) {
             int kc = std::min(8, k - c0);
             ++tri_epoch;
+            kt_begin(RLS_KF_TRSM);
             launch_trsm_persistent(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
                                    tri_flags.p, tri_epoch, tri_err.p);
+            // algorithmic: the triangle of T once + the RHS block read and written
+            kt_end(RLS_KF_TRSM, 8.0 * 0.5 * (double)n * n * kc,
+                   16.0 * (0.5 * (double)n * n + 2.0 * (double)n * kc));
         }
     }
     bool tri_ticket_base_ok = false;
@@ -447,8 +500,11 @@ struct rls_ctx {
         lamev.ensure((size_t)B * pr);
         Gev.ensure((size_t)B * pr * nvec);
         rr.ensure(B);
+        kt_begin(RLS_KF_HEEV);
         launch_heev_rr(st, Hz.p, pr, (int64_t)pr * pr, pr, B, heevW.p, lamev.p, pr, Gev.p, pr,
                        (int64_t)pr * nvec, rr.p, tausd.p, r, nvec, 0);
+        // algorithmic: Householder tridiagonalisation 16/3 pr^3 real flops per matrix
+        kt_end(RLS_KF_HEEV, (16.0 / 3.0) * (double)pr * pr * pr * B, 16.0 * (double)pr * pr * B);
         std::vector<int> rrh(B);
         d2h(rrh.data(), rr.p, B);
         std::vector<int> coffh(B);
@@ -469,7 +525,12 @@ struct rls_ctx {
             ops.A[0] = J5.p; ops.lda[0] = n; ops.K[0] = pr;
             ops.A[1] = V.p; ops.lda[1] = n; ops.K[1] = pr;
             ops.nops = 2;
-            if (tot >= 32) launch_zgemm_tiled(st, n, ops, C1.p, 2 * pr, tot, CzX.p, n);
+            if (tot >= 32) {
+                kt_begin(RLS_KF_GEMM);
+                launch_zgemm_tiled(st, n, ops, C1.p, 2 * pr, tot, CzX.p, n);
+                kt_end(RLS_KF_GEMM, 8.0 * n * (2.0 * pr) * tot,
+                       16.0 * ((double)n * 2 * pr + 2.0 * pr * tot + (double)n * tot));
+            }
             else launch_tall_gemm(st, n, ops, C1.p, 2 * pr, 0, tot, CzX.p, n, 0, 0.0, 1);
         }
         // per-shift Gram of CzX_ (rr_b x rr_b inside rmax x rmax)
@@ -517,10 +578,16 @@ struct rls_ctx {
             ops.A[3] = V.p; ops.lda[3] = n; ops.K[3] = pr;
             ops.nops = 4;
             if (B * r >= 32) {
+                kt_begin(RLS_KF_GEMM);
                 launch_zgemm_tiled(st, n, ops, CW.p, 4 * pr, B * r, Wr.p, n);
+                kt_end(RLS_KF_GEMM, 8.0 * n * (4.0 * pr) * B * r,
+                       16.0 * ((double)n * 4 * pr + 4.0 * pr * B * r + (double)n * B * r));
                 TallOperands ov{};
                 ov.A[0] = V.p; ov.lda[0] = n; ov.K[0] = pr; ov.nops = 1;
+                kt_begin(RLS_KF_GEMM);
                 launch_zgemm_tiled(st, n, ov, CG.p, pr, B * r, Xr.p, n);
+                kt_end(RLS_KF_GEMM, 8.0 * n * (double)pr * B * r,
+                       16.0 * ((double)n * pr + (double)pr * B * r + (double)n * B * r));
             } else {
                 launch_tall_gemm(st, n, ops, CW.p, 4 * pr, 0, B * r, Wr.p, n, 0, 0.0, 1, Xr.p, n, 0, 3, CG.p,
                                  pr);
@@ -599,8 +666,13 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     int64_t n_it = 0, n_pts = 0, n_windows = 0, n_wshifts = 0;
     uint64_t t_rec = 0, t_ref = 0, t_upd = 0;
     const bool batched = (c.mode == RLS_MODE_BATCHED);
-    int window = c.window > 0 ? c.window : 64;
-    int run_hint = 8;
+    // Window sizing: the eigensolves of a window run in parallel (one workgroup per shift),
+    // so a window costs about one eigensolve latency plus products linear in its width.
+    // Width = 1.5x the recent run length between refinements (EMA), clamped to [16, max];
+    // a window that is accepted completely doubles the next one.
+    const int window = c.window > 0 ? c.window : 128;
+    double avg_run = 32.0;
+    int run_hint = 32;
     int64_t next_report = std::max<int64_t>(nZ / 20, 1);
 
     int64_t i = 0;
@@ -611,7 +683,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
         if (ctx->p > 0) {
             uint64_t t0 = timing ? now_ns() : 0;
             int B = 1;
-            if (batched) B = (int)std::min<int64_t>({(int64_t)window, (int64_t)std::max(run_hint, 4), nZ - i});
+            if (batched) B = (int)std::min<int64_t>({(int64_t)window, (int64_t)std::max(run_hint, 1), nZ - i});
             std::vector<cplx> zs(Z + i, Z + i + B);
             std::vector<double> taus(B);
             for (int b = 0; b < B; ++b)
@@ -638,7 +710,14 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
                 scaled_remaining = scaled_remaining > 0 ? scaled_remaining - 1 : 0;
             }
             if (timing) { ctx->sync(); t_rec += now_ns() - t0; }
-            if (batched) run_hint = (f == B) ? std::min(window, 2 * B) : std::max(4, f + 1);
+            if (batched) {
+                if (f == B) {
+                    run_hint = std::min(window, 2 * B);
+                } else {
+                    avg_run = 0.7 * avg_run + 0.3 * (double)(f + 1);
+                    run_hint = std::max(16, std::min(window, (int)(1.5 * avg_run) + 8));
+                }
+            }
             if (f == B) { i += B; continue; }
             fail_at = i + f;
             // keep_tri.X <- X_recy of the failing shift (psTriRecy.jl:749-757)
@@ -686,6 +765,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
         }
     }
     ctx->sync();
+    ctx->kt_collect();
     ctx->lastX.resize((size_t)n * r);
     ctx->d2h(reinterpret_cast<zd*>(ctx->lastX.data()), ctx->S.p, (size_t)n * r);
     if (stats) {
@@ -772,6 +852,28 @@ void rls_tri_control_default(rls_tri_control* c, double normC) {
     c->mode = RLS_MODE_PLAIN_FP64;
     c->window = 0;
     c->seed = 0;
+}
+
+int rls_kernel_timing_enable(rls_ctx* ctx, int enable) {
+    return guarded(ctx, [&]() -> int {
+        ctx->timing_on = enable != 0;
+        for (auto& k : ctx->ktm) { k.launches = 0; k.flops = k.bytes = k.ms = 0.0; k.used = 0; }
+        return RLS_OK;
+    });
+}
+
+int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,
+                      double* bytes) {
+    return guarded(ctx, [&]() -> int {
+        if (family < 0 || family >= RLS_KF_COUNT) throw RlsError{RLS_E_ARG, "unknown kernel family"};
+        ctx->kt_collect();
+        const auto& k = ctx->ktm[family];
+        if (launches) *launches = k.launches;
+        if (ms) *ms = k.ms;
+        if (flops) *flops = k.flops;
+        if (bytes) *bytes = k.bytes;
+        return RLS_OK;
+    });
 }
 
 int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {

@@ -97,3 +97,9 @@ void read_heev_phases(long long* out);
 // Tiled FP64 complex GEMM Y = [A_q] * B for wide B (batched recycle products).
 void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
                         int N, zd* Y, int64_t ldy);
+void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z,
+                              const double* sig, zd* U, int nb);
+void launch_grad_z(hipStream_t st, int64_t n, const zd* U, const zd* V, double* dsdz, int nb);
+void launch_grad_pattern(hipStream_t st, int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                         const zd* U, const zd* V, double* out, int nb);
+void launch_grad_toeplitz(hipStream_t st, int64_t n, const zd* U, const zd* V, double* out, int nb);

@@ -1673,3 +1673,83 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
 size_t heev_wave_workspace_elems(int n, int nvec_max) {
     return (size_t)n * (n + 1) / 2 + ((size_t)n * 64 + 1) / 2 + 8;
 }
+
+// ---------------------------------------------------------------------------------------
+// Adjoint of sigma_min (no counterpart in the reference).  For a unit right singular vector v
+// of A = C - zI with A v = sigma u:  d sigma = Re(u^H dA v), so
+//   d sigma / d Re z = -Re(u^H v),  d sigma / d Im z = Im(u^H v),
+//   d sigma / d Re C_ij = Re(conj(u_i) v_j).
+// ---------------------------------------------------------------------------------------
+// U[:, b] = (Y[:, b] - z_b V[:, b]) / sigma_b     (Y = T V)
+__global__ void k_grad_left_vectors(int64_t n, const zd* __restrict__ Y, const zd* __restrict__ V,
+                                    const zd* __restrict__ z, const double* __restrict__ sig,
+                                    zd* __restrict__ U) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= n) return;
+    const zd y = Y[i + n * b], v = V[i + n * b];
+    U[i + n * b] = zscale(zsub(y, zmul(z[b], v)), 1.0 / sig[b]);
+}
+
+// dsdz[b] = (-Re(u^H v), Im(u^H v)) per column, one workgroup per column
+__global__ void __launch_bounds__(256)
+k_grad_z(int64_t n, const zd* __restrict__ U, const zd* __restrict__ V, double* __restrict__ dsdz) {
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    double pr = 0.0, pi = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const zd c = zcmul(U[i + n * b], V[i + n * b]);
+        pr += c.x; pi += c.y;
+    }
+    pr = block_sum(pr, red);
+    pi = block_sum(pi, red);
+    if (threadIdx.x == 0) { dsdz[2 * b] = -pr; dsdz[2 * b + 1] = pi; }
+}
+
+// out[e + nnz*b] = Re(conj(U[row_e, b]) V[col_e, b])   (pattern entries, 0-based indices)
+__global__ void k_grad_pattern(int64_t n, int64_t nnz, const int64_t* __restrict__ rows,
+                               const int64_t* __restrict__ cols, const zd* __restrict__ U,
+                               const zd* __restrict__ V, double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (e >= nnz) return;
+    const zd u = U[rows[e] + n * b], v = V[cols[e] + n * b];
+    out[e + nnz * b] = u.x * v.x + u.y * v.y;
+}
+
+// Toeplitz diagonals: out[(k + n - 1) + (2n - 1) b] = sum_{i - j = k} Re(conj(U[i,b]) V[j,b])
+__global__ void __launch_bounds__(256)
+k_grad_toeplitz(int64_t n, const zd* __restrict__ U, const zd* __restrict__ V, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const int64_t kidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // 0 .. 2n-2
+    if (kidx >= 2 * n - 1) return;
+    const int64_t k = kidx - (n - 1);
+    const zd* u = U + n * b;
+    const zd* v = V + n * b;
+    double s = 0.0;
+    const int64_t i0 = k > 0 ? k : 0, i1 = k > 0 ? n : n + k;
+    for (int64_t i = i0; i < i1; ++i) {
+        const zd a = u[i], c = v[i - k];
+        s += a.x * c.x + a.y * c.y;
+    }
+    out[kidx + (2 * n - 1) * b] = s;
+}
+
+void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z,
+                              const double* sig, zd* U, int nb) {
+    hipLaunchKernelGGL(k_grad_left_vectors, dim3(cdiv(n, 256), nb), dim3(256), 0, st, n, Y, V, z, sig, U);
+    LAUNCH_CHECK();
+}
+void launch_grad_z(hipStream_t st, int64_t n, const zd* U, const zd* V, double* dsdz, int nb) {
+    hipLaunchKernelGGL(k_grad_z, dim3(nb), dim3(256), 0, st, n, U, V, dsdz);
+    LAUNCH_CHECK();
+}
+void launch_grad_pattern(hipStream_t st, int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                         const zd* U, const zd* V, double* out, int nb) {
+    hipLaunchKernelGGL(k_grad_pattern, dim3(cdiv(nnz, 256), nb), dim3(256), 0, st, n, nnz, rows, cols, U, V, out);
+    LAUNCH_CHECK();
+}
+void launch_grad_toeplitz(hipStream_t st, int64_t n, const zd* U, const zd* V, double* out, int nb) {
+    hipLaunchKernelGGL(k_grad_toeplitz, dim3(cdiv(2 * n - 1, 256), nb), dim3(256), 0, st, n, U, V, out);
+    LAUNCH_CHECK();
+}
