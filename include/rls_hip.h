@@ -113,6 +113,39 @@ int rls_kernel_timing_enable(rls_ctx* ctx, int enable);
 int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,
                       double* bytes);
 
+/* ---- Adjoint of sigma_min (no counterpart in the reference: psTriRecy returns only sigma).
+ * For the certified right singular vector v of A = C - zI at a grid point and
+ * u = A v / ||A v||:  d sigma = Re(u^H dA v), hence
+ *     d sigma / d Re z = -Re(u^H v),   d sigma / d Im z = Im(u^H v),
+ *     d sigma / d Re C[i, j] = Re(conj(u_i) v_j).
+ * v is the vector that passed the reference's acceptance test at that point: column 1 of the
+ * recycled Ritz block (fastRaylRitz.jl:207-215) or of the LOBPCG block (lobpcgSvd.jl:466-476).
+ * Point ranges [first, first + count) index the nZ shifts of the last rls_tri_solve. */
+
+/* Keep the certified right singular vector of every point of subsequent solves in device
+ * memory (n x nZ complex).  Off by default. */
+int rls_tri_keep_vectors(rls_ctx* ctx, int on);
+
+/* Unitary Schur vectors Q (C = Q T Q^H, n x n, leading dimension ld) so that vectors and
+ * matrix gradients are returned in C's basis; without them they are in T's basis.  Cleared by
+ * rls_set_tri_dense. */
+int rls_set_schur_vectors(rls_ctx* ctx, const double* Q, int64_t ld);
+
+/* dsdz[2 i] = d sigma / d Re z, dsdz[2 i + 1] = d sigma / d Im z for points first..first+count-1. */
+int rls_sigma_grad(rls_ctx* ctx, int64_t first, int64_t count, double* dsdz);
+
+/* u and v (n x count complex each, column-major; either may be NULL). */
+int rls_tri_singular_vectors(rls_ctx* ctx, int64_t first, int64_t count, double* u_out, double* v_out);
+
+/* d sigma / d Re C restricted to a sparsity pattern (0-based row / column indices, nnz entries):
+ * out[e + nnz * b] = Re(conj(u_b[rows[e]]) v_b[cols[e]]) for b < count. */
+int rls_sigma_grad_pattern(rls_ctx* ctx, int64_t first, int64_t count, int64_t nnz, const int64_t* rows,
+                           const int64_t* cols, double* out);
+
+/* d sigma / d t_k for a Toeplitz parametrisation C[i, j] = t_{i-j}, k = -(n-1)..n-1:
+ * out[(k + n - 1) + (2 n - 1) * b] = sum over i - j = k of Re(conj(u_b[i]) v_b[j]). */
+int rls_sigma_grad_toeplitz(rls_ctx* ctx, int64_t first, int64_t count, double* out);
+
 #ifdef __cplusplus
 }
 #endif

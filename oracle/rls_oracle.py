@@ -712,6 +712,7 @@ class PsTriRecyStats:
     refined: np.ndarray = None        # bool per point (extra diagnostics)
     lobpcg_it: np.ndarray = None
     recy_res: np.ndarray = None       # ||W[:,1]|| of the recycle RR (NaN if none)
+    vectors: np.ndarray = None        # certified right singular vectors (keep_vectors=True)
 
     @classmethod
     def alloc(cls, nZ: int):
@@ -769,8 +770,12 @@ def sparse_tri_problem(C: sp.spmatrix, perm: Optional[np.ndarray] = None):
 
 def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed: int = 0,
                 X_init: Optional[np.ndarray] = None, verbose: int = 0,
-                on_point: Optional[Callable] = None):
+                on_point: Optional[Callable] = None, keep_vectors: bool = False):
     """Sequential psTriRecy over rec.Z (src/core/psTriRecy.jl:708-881).
+
+    With keep_vectors, stats.vectors[:, i] is column 1 of the block that passed the
+    acceptance test at point i (the recycled Ritz block or the LOBPCG block): the certified
+    right singular vector the adjoint is built from.
 
     ``problem`` = (M, S, refine_op) from dense_tri_problem / sparse_tri_problem.
     Returns (sigma_min, stats, recycle_space)."""
@@ -790,6 +795,8 @@ def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed:
     nZ = len(rec.Z)
     sigma_min = np.full(nZ, np.nan)
     stats = PsTriRecyStats.alloc(nZ)
+    if keep_vectors:
+        stats.vectors = np.zeros((n, nZ), dtype=np.complex128)
     last_failed = None
     ctrl_main = LobpcgSvdControl(m=n, n=n, r=r, conv_range=(1, 1),
                                  tol=float(control.lobpcg_tol_scale * control.tol),
@@ -810,12 +817,16 @@ def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed:
             stats.recy_res[i] = np.linalg.norm(Wr[:, 0])
             if residual_norm_ok(Wr, sr[0], control.tol, control.normC):
                 sigma_min[i] = sr[0]
+                if keep_vectors:
+                    stats.vectors[:, i] = Xr[:, 0]
                 if on_point:
                     on_point(i, z, "recycled", rs.p, r_svdrr, sigma_min[i])
                 continue
         Cop = refine_op(z)
         res = lobpcg_svd(Cop, X, None, ctrl_main)
         sigma_min[i] = res.sigma[0]
+        if keep_vectors:
+            stats.vectors[:, i] = res.X[:, 0]
         it = res.it
         if it > 0:
             r_apps += 2 * r + it * r + (it - 1) * r
@@ -839,3 +850,30 @@ def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed:
         if on_point:
             on_point(i, z, f"refine({it})", rs.p, r_svdrr, sigma_min[i])
     return sigma_min, stats, rs
+
+
+# --------------------------------------------------------------------------------------
+# Adjoint of sigma_min (no counterpart in the reference, which returns sigma only)
+# --------------------------------------------------------------------------------------
+def sigma_grad_from_vectors(T: np.ndarray, Z, Vr: np.ndarray):
+    """First-order perturbation of a simple singular value: for A = T - zI, unit right vector v
+    and u = A v / ||A v||, d sigma = Re(u^H dA v), so d sigma/dRe z = -Re(u^H v) and
+    d sigma/dIm z = Im(u^H v).  Returns (dsdz (nZ x 2), U (n x nZ)).  The basis of T and Vr
+    may be any unitary one: u^H v is invariant."""
+    Z = np.asarray(Z, dtype=np.complex128)
+    Y = T @ Vr - Vr * Z[None, :]
+    U = Y / np.linalg.norm(Y, axis=0)[None, :]
+    uv = np.sum(U.conj() * Vr, axis=0)
+    return np.stack([-uv.real, uv.imag], axis=1), U
+
+
+def toeplitz_grad(U: np.ndarray, V: np.ndarray):
+    """d sigma / d t_k for C[i, j] = t_{i-j}: sum over the k-th diagonal of Re(conj(u_i) v_j),
+    k = -(n-1)..n-1 in rows 0..2n-2 (column b = point b)."""
+    n, nb = U.shape
+    out = np.zeros((2 * n - 1, nb))
+    for k in range(-(n - 1), n):
+        i0, i1 = (k, n) if k > 0 else (0, n + k)
+        i = np.arange(i0, i1)
+        out[k + n - 1] = np.sum((U[i].conj() * V[i - k]).real, axis=0)
+    return out

@@ -26,7 +26,7 @@ from . import _capi
 __all__ = [
     "Recursion", "recursion_grid", "recursion_grid_zigzag", "recursion_natural_permutation",
     "reorder_to_natural", "reorder_to_natural_grid", "PsTriRecyControl", "PsTriRecyStats",
-    "psTriRecy", "schur_preprocess", "TriSolver",
+    "psTriRecy", "schur_preprocess", "schur_factor", "TriSolver",
 ]
 
 EPS = float(np.finfo(np.float64).eps)
@@ -212,15 +212,18 @@ def schur_preprocess(C) -> np.ndarray:
     return np.asfortranarray(T)
 
 
-_CTX = {}
-
-
-def _context(device: int = 0) -> _capi.Context:
-    ctx = _CTX.get(device)
-    if ctx is None:
-        ctx = _capi.Context(device)
-        _CTX[device] = ctx
-    return ctx
+def schur_factor(C):
+    """(T, Q) with C = Q T Q^H, T upper triangular: schur_preprocess plus the Schur vectors,
+    which the adjoint needs to return d sigma / dC in C's basis."""
+    import scipy.linalg as sla
+    import scipy.sparse as sp
+    if sp.issparse(C):
+        C = C.toarray()
+    C = np.asarray(C)
+    if C.ndim != 2 or C.shape[0] != C.shape[1]:
+        raise ValueError(f"pre_process only supports square matrices, got size {C.shape}")
+    T, Q = sla.schur(C.astype(np.complex128), output="complex")
+    return np.asfortranarray(T), np.asfortranarray(Q)
 
 
 def _c_control(control: PsTriRecyControl, rec: Recursion) -> _capi.TriControl:
@@ -252,14 +255,69 @@ def _c_control(control: PsTriRecyControl, rec: Recursion) -> _capi.TriControl:
 
 
 class TriSolver:
-    """A device-resident Schur factor; solve() may be called repeatedly (the bench times it)."""
+    """A device-resident Schur factor with its own library context (device buffers, stream);
+    solve() may be called repeatedly (the bench times it)."""
 
     def __init__(self, T: np.ndarray, device: int = 0):
-        self.ctx = _context(device)
+        self.ctx = _capi.Context(device)
         T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
         n = T.shape[0]
         self.n = n
+        self._last_nZ = 0
         self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, n, n))
+
+    # ---- adjoint (no counterpart in the reference) ----
+    def keep_vectors(self, on: bool = True):
+        """Keep every point's certified right singular vector on the device during solve()."""
+        self.ctx.check(self.ctx.lib.rls_tri_keep_vectors(self.ctx.h, int(on)))
+
+    def set_schur_vectors(self, Q: np.ndarray):
+        """Schur vectors (C = Q T Q^H) so that vectors / matrix gradients come back in C's basis."""
+        Q = np.asfortranarray(np.asarray(Q, dtype=np.complex128))
+        if Q.shape != (self.n, self.n):
+            raise ValueError(f"Q must be {self.n} x {self.n}")
+        self.ctx.check(self.ctx.lib.rls_set_schur_vectors(self.ctx.h, Q.ctypes.data, self.n))
+
+    def _range(self, first, count):
+        if count is None:
+            count = self._last_nZ - first
+        return int(first), int(count)
+
+    def sigma_grad(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        """d sigma / d(Re z, Im z) at the points of the last solve, shape (count, 2)."""
+        first, count = self._range(first, count)
+        out = np.zeros((count, 2))
+        self.ctx.check(self.ctx.lib.rls_sigma_grad(self.ctx.h, first, count, out.ctypes.data))
+        return out
+
+    def singular_vectors(self, first: int = 0, count: Optional[int] = None):
+        """(U, V), n x count each: certified right vectors v and u = (C - zI) v / ||(C - zI) v||."""
+        first, count = self._range(first, count)
+        U = np.zeros((self.n, count), dtype=np.complex128, order="F")
+        V = np.zeros((self.n, count), dtype=np.complex128, order="F")
+        self.ctx.check(self.ctx.lib.rls_tri_singular_vectors(self.ctx.h, first, count, U.ctypes.data,
+                                                             V.ctypes.data))
+        return U, V
+
+    def grad_pattern(self, rows, cols, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        """d sigma / d Re C[rows[e], cols[e]] (0-based), shape (count, nnz)."""
+        first, count = self._range(first, count)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        if rows.shape != cols.shape or rows.ndim != 1:
+            raise ValueError("rows and cols must be 1-D of equal length")
+        out = np.zeros((count, len(rows)))
+        self.ctx.check(self.ctx.lib.rls_sigma_grad_pattern(self.ctx.h, first, count, len(rows),
+                                                           rows.ctypes.data, cols.ctypes.data,
+                                                           out.ctypes.data))
+        return out
+
+    def grad_toeplitz(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        """d sigma / d t_k for C[i, j] = t_{i-j}, shape (count, 2n - 1), column k + n - 1."""
+        first, count = self._range(first, count)
+        out = np.zeros((count, 2 * self.n - 1))
+        self.ctx.check(self.ctx.lib.rls_sigma_grad_toeplitz(self.ctx.h, first, count, out.ctypes.data))
+        return out
 
     def enable_kernel_timing(self, on: bool = True):
         """HIP-event timing of the kernel families (recycle GEMMs, eigensolves, triangular
@@ -291,6 +349,7 @@ class TriSolver:
         cc = _c_control(control, rec)
         self.ctx.check(self.ctx.lib.rls_tri_solve(self.ctx.h, Z.ctypes.data, nZ, C.byref(cc),
                                                   sigma.ctypes.data, C.byref(st)))
+        self._last_nZ = nZ
         stats = PsTriRecyStats(arrs["p_count"], arrs["r_recy"], pts, arrs["M_vec_apps"],
                                arrs["M_vec_apps_no_recycle"], arrs["R_vec_apps"],
                                recycle_time_ns=st.recycle_time_ns, qr_time_ns=st.qr_time_ns,

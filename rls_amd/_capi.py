@@ -23,6 +23,12 @@ EXPORTED = [
     "rls_tri_solve",
     "rls_kernel_timing_enable",
     "rls_kernel_timing",
+    "rls_tri_keep_vectors",
+    "rls_set_schur_vectors",
+    "rls_sigma_grad",
+    "rls_tri_singular_vectors",
+    "rls_sigma_grad_pattern",
+    "rls_sigma_grad_toeplitz",
 ]
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 
@@ -108,6 +114,16 @@ def load() -> C.CDLL:
     lib.rls_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.rls_kernel_timing.restype = C.c_int
+    i64 = C.c_int64
+    for name, args in (("rls_tri_keep_vectors", [vp, C.c_int]),
+                       ("rls_set_schur_vectors", [vp, vp, i64]),
+                       ("rls_sigma_grad", [vp, i64, i64, vp]),
+                       ("rls_tri_singular_vectors", [vp, i64, i64, vp, vp]),
+                       ("rls_sigma_grad_pattern", [vp, i64, i64, i64, vp, vp, vp]),
+                       ("rls_sigma_grad_toeplitz", [vp, i64, i64, vp])):
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
     _lib = lib
     return lib
 

@@ -84,6 +84,14 @@ struct rls_ctx {
     DArr<zd> heevW2;
     // last X (Ritz vectors) of the final state, for callers
     std::vector<cplx> lastX;
+    // adjoint: the certified right singular vector of every point of the last solve
+    // (Schur basis, n x nkeep), the shifts, and optional Schur vectors Q (C = Q T Q^H)
+    bool keep_vec = false;
+    int64_t nkeep = 0;
+    DArr<zd> vkeep, zkeep, Qs, gY, gU, gUq, gVq;
+    DArr<double> gnrm, gout;
+    DArr<int64_t> grows, gcols;
+    bool have_q = false;
 
     // Optional HIP-event timing of kernel families (recycle GEMMs, eigensolves, triangular
     // solves) with the algorithmic flops / bytes of each launch, read by rls_kernel_timing.
@@ -135,9 +143,12 @@ struct rls_ctx {
     ~rls_ctx() {
         DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
-                          &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev};
+                          &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev,
+                          &vkeep, &zkeep, &Qs, &gY, &gU, &gUq, &gVq};
         for (auto* a : zs) a->release();
-        DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd, &lamev2};
+        DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd, &lamev2, &gnrm, &gout};
+        grows.release();
+        gcols.release();
         tri_ctr.release();
         tri_flags.release();
         tri_err.release();
@@ -649,6 +660,12 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     const double thresh = resolve_fail_dist(c);
     const bool timing = c.collect_timing != 0;
     for (int64_t i = 0; i < nZ; ++i) sigma[i] = std::numeric_limits<double>::quiet_NaN();
+    ctx->nkeep = 0;
+    if (ctx->keep_vec && nZ > 0) {
+        ctx->vkeep.ensure((size_t)n * nZ);
+        ctx->zkeep.ensure((size_t)nZ);
+        ctx->h2d(ctx->zkeep.p, reinterpret_cast<const zd*>(Z), (size_t)nZ);
+    }
     auto set_stat = [&](int64_t* arr, int64_t i, int64_t v) { if (arr) arr[i] = v; };
 
     // initial block: uniform [0,1) real and imaginary parts (rand!(X), psTriRecy.jl:607),
@@ -697,7 +714,9 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
             for (int b = 0; b < B; ++b) {
                 if (!residual_ok(res[b], sg[(size_t)b * r], c.tol, c.normC)) { f = b; break; }
             }
-            // accepted points
+            // accepted points (their right singular vector is column 0 of X_recy)
+            if (ctx->keep_vec && f > 0)
+                launch_copy2d(ctx->st, n, f, ctx->Xr.p, (int64_t)n * r, ctx->vkeep.p + n * i, n);
             for (int b = 0; b < f; ++b) {
                 const int64_t k = i + b;
                 set_stat(stats ? stats->p_count : nullptr, k, ctx->p);
@@ -735,6 +754,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
         uint64_t t1 = timing ? now_ns() : 0;
         auto [s_min, it] = ctx->lobpcg(zk, c);
         sigma[k] = s_min;
+        if (ctx->keep_vec) launch_copy2d(ctx->st, n, 1, ctx->S.p, n, ctx->vkeep.p + n * k, n);
         if (timing) t_ref += now_ns() - t1;
         int64_t r_apps = it > 0 ? (int64_t)2 * r + (int64_t)it * r + (int64_t)(it - 1) * r : 0;
         int64_t m_apps = 0;
@@ -766,6 +786,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     }
     ctx->sync();
     ctx->kt_collect();
+    if (ctx->keep_vec) ctx->nkeep = nZ;
     ctx->lastX.resize((size_t)n * r);
     ctx->d2h(reinterpret_cast<zd*>(ctx->lastX.data()), ctx->S.p, (size_t)n * r);
     if (stats) {
@@ -780,6 +801,39 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
         stats->n_window_shifts = n_wshifts;
     }
     return RLS_OK;
+}
+
+// Left vectors of points [b0, b0 + nb) of the last solve: gU = (T - zI) v / ||(T - zI) v||
+// (Schur basis); with Schur vectors set, gUq = Q u and gVq = Q v (original basis).
+constexpr int GRAD_CHUNK = 512;
+void grad_vectors(rls_ctx* ctx, int64_t b0, int nb, bool original_basis) {
+    const int64_t n = ctx->n;
+    ctx->gY.ensure((size_t)n * GRAD_CHUNK);
+    ctx->gU.ensure((size_t)n * GRAD_CHUNK);
+    ctx->gnrm.ensure(GRAD_CHUNK);
+    const zd* V = ctx->vkeep.p + n * b0;
+    TallOperands ops{};
+    ops.A[0] = ctx->T.p; ops.lda[0] = n; ops.K[0] = (int)n; ops.nops = 1;
+    launch_zgemm_tiled(ctx->st, n, ops, V, n, nb, ctx->gY.p, n);
+    launch_grad_left_vectors(ctx->st, n, ctx->gY.p, V, ctx->zkeep.p + b0, ctx->gU.p, nb);
+    ctx->part.ensure(gram_partial_elems(n, nb, 1));
+    launch_colnorm2(ctx->st, n, ctx->gU.p, n, nb, ctx->gnrm.p, ctx->part.p);
+    launch_grad_normalize(ctx->st, n, ctx->gU.p, ctx->gnrm.p, nb);
+    if (original_basis && ctx->have_q) {
+        ctx->gUq.ensure((size_t)n * GRAD_CHUNK);
+        ctx->gVq.ensure((size_t)n * GRAD_CHUNK);
+        TallOperands q{};
+        q.A[0] = ctx->Qs.p; q.lda[0] = n; q.K[0] = (int)n; q.nops = 1;
+        launch_zgemm_tiled(ctx->st, n, q, ctx->gU.p, n, nb, ctx->gUq.p, n);
+        launch_zgemm_tiled(ctx->st, n, q, V, n, nb, ctx->gVq.p, n);
+    }
+}
+
+void check_grad_range(rls_ctx* ctx, int64_t first, int64_t count) {
+    if (ctx->nkeep <= 0)
+        throw RlsError{RLS_E_STATE, "no singular vectors kept: call rls_tri_keep_vectors(ctx, 1) before rls_tri_solve"};
+    if (first < 0 || count < 0 || first + count > ctx->nkeep)
+        throw RlsError{RLS_E_ARG, "point range outside the last solve"};
 }
 
 template <typename F>
@@ -882,6 +936,8 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
         ctx->n = n;
+        ctx->have_q = false;   // Schur vectors and kept singular vectors belong to the old T
+        ctx->nkeep = 0;
         ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
         ctx->tri_ctr.release();
         ctx->T.ensure((size_t)n * n);
@@ -898,6 +954,113 @@ int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_contr
         if (!Z || nZ < 0 || !control || !sigma_out) throw RlsError{RLS_E_ARG, "rls_tri_solve: null argument"};
         HIP_TRY(hipSetDevice(ctx->device));
         return solve_tri(ctx, reinterpret_cast<const cplx*>(Z), nZ, *control, sigma_out, stats);
+    });
+}
+
+int rls_tri_keep_vectors(rls_ctx* ctx, int on) {
+    return guarded(ctx, [&]() -> int {
+        ctx->keep_vec = on != 0;
+        if (!ctx->keep_vec) {
+            ctx->vkeep.release();
+            ctx->nkeep = 0;
+        }
+        return RLS_OK;
+    });
+}
+
+int rls_set_schur_vectors(rls_ctx* ctx, const double* Q, int64_t ld) {
+    return guarded(ctx, [&]() -> int {
+        const int64_t n = ctx->n;
+        if (n <= 0) throw RlsError{RLS_E_STATE, "set the Schur factor first (rls_set_tri_dense)"};
+        if (!Q || ld < n) throw RlsError{RLS_E_ARG, "rls_set_schur_vectors: bad Q / ld"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->Qs.ensure((size_t)n * n);
+        HIP_TRY(hipMemcpy2DAsync(ctx->Qs.p, sizeof(zd) * n, Q, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->sync();
+        ctx->have_q = true;
+        return RLS_OK;
+    });
+}
+
+int rls_sigma_grad(rls_ctx* ctx, int64_t first, int64_t count, double* dsdz) {
+    return guarded(ctx, [&]() -> int {
+        check_grad_range(ctx, first, count);
+        if (!dsdz && count > 0) throw RlsError{RLS_E_ARG, "rls_sigma_grad: null output"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->gout.ensure((size_t)2 * GRAD_CHUNK);
+        for (int64_t b0 = first; b0 < first + count; b0 += GRAD_CHUNK) {
+            const int nb = (int)std::min<int64_t>(GRAD_CHUNK, first + count - b0);
+            grad_vectors(ctx, b0, nb, false);
+            launch_grad_z(ctx->st, ctx->n, ctx->gU.p, ctx->vkeep.p + ctx->n * b0, ctx->gout.p, nb);
+            ctx->d2h(dsdz + 2 * (b0 - first), ctx->gout.p, (size_t)2 * nb);
+        }
+        return RLS_OK;
+    });
+}
+
+int rls_tri_singular_vectors(rls_ctx* ctx, int64_t first, int64_t count, double* u_out, double* v_out) {
+    return guarded(ctx, [&]() -> int {
+        check_grad_range(ctx, first, count);
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = ctx->n;
+        for (int64_t b0 = first; b0 < first + count; b0 += GRAD_CHUNK) {
+            const int nb = (int)std::min<int64_t>(GRAD_CHUNK, first + count - b0);
+            grad_vectors(ctx, b0, nb, true);
+            const zd* U = ctx->have_q ? ctx->gUq.p : ctx->gU.p;
+            const zd* V = ctx->have_q ? ctx->gVq.p : ctx->vkeep.p + n * b0;
+            if (u_out) ctx->d2h(reinterpret_cast<zd*>(u_out) + n * (b0 - first), U, (size_t)n * nb);
+            if (v_out) ctx->d2h(reinterpret_cast<zd*>(v_out) + n * (b0 - first), V, (size_t)n * nb);
+        }
+        return RLS_OK;
+    });
+}
+
+int rls_sigma_grad_pattern(rls_ctx* ctx, int64_t first, int64_t count, int64_t nnz, const int64_t* rows,
+                           const int64_t* cols, double* out) {
+    return guarded(ctx, [&]() -> int {
+        check_grad_range(ctx, first, count);
+        const int64_t n = ctx->n;
+        if (nnz < 0 || (nnz > 0 && (!rows || !cols || !out))) throw RlsError{RLS_E_ARG, "rls_sigma_grad_pattern: null argument"};
+        for (int64_t e = 0; e < nnz; ++e)
+            if (rows[e] < 0 || rows[e] >= n || cols[e] < 0 || cols[e] >= n)
+                throw RlsError{RLS_E_ARG, "rls_sigma_grad_pattern: index outside 0..n-1"};
+        if (nnz == 0 || count == 0) return RLS_OK;
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->grows.ensure(nnz);
+        ctx->gcols.ensure(nnz);
+        ctx->h2d(ctx->grows.p, rows, nnz);
+        ctx->h2d(ctx->gcols.p, cols, nnz);
+        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(GRAD_CHUNK, (int64_t)(1 << 26) / nnz));
+        ctx->gout.ensure((size_t)nnz * chunk);
+        for (int64_t b0 = first; b0 < first + count; b0 += chunk) {
+            const int nb = (int)std::min<int64_t>(chunk, first + count - b0);
+            grad_vectors(ctx, b0, nb, true);
+            const zd* U = ctx->have_q ? ctx->gUq.p : ctx->gU.p;
+            const zd* V = ctx->have_q ? ctx->gVq.p : ctx->vkeep.p + n * b0;
+            launch_grad_pattern(ctx->st, n, nnz, ctx->grows.p, ctx->gcols.p, U, V, ctx->gout.p, nb);
+            ctx->d2h(out + nnz * (b0 - first), ctx->gout.p, (size_t)nnz * nb);
+        }
+        return RLS_OK;
+    });
+}
+
+int rls_sigma_grad_toeplitz(rls_ctx* ctx, int64_t first, int64_t count, double* out) {
+    return guarded(ctx, [&]() -> int {
+        check_grad_range(ctx, first, count);
+        if (!out && count > 0) throw RlsError{RLS_E_ARG, "rls_sigma_grad_toeplitz: null output"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = ctx->n, nd = 2 * n - 1;
+        ctx->gout.ensure((size_t)nd * GRAD_CHUNK);
+        for (int64_t b0 = first; b0 < first + count; b0 += GRAD_CHUNK) {
+            const int nb = (int)std::min<int64_t>(GRAD_CHUNK, first + count - b0);
+            grad_vectors(ctx, b0, nb, true);
+            const zd* U = ctx->have_q ? ctx->gUq.p : ctx->gU.p;
+            const zd* V = ctx->have_q ? ctx->gVq.p : ctx->vkeep.p + n * b0;
+            launch_grad_toeplitz(ctx->st, n, U, V, ctx->gout.p, nb);
+            ctx->d2h(out + nd * (b0 - first), ctx->gout.p, (size_t)nd * nb);
+        }
+        return RLS_OK;
     });
 }
 

@@ -97,8 +97,14 @@ void read_heev_phases(long long* out);
 // Tiled FP64 complex GEMM Y = [A_q] * B for wide B (batched recycle products).
 void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
                         int N, zd* Y, int64_t ldy);
-void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z,
-                              const double* sig, zd* U, int nb);
+// Adjoint of sigma_min at the certified right singular vectors V (n x nb, Schur basis):
+//   left_vectors: U = T V - V diag(z) (Y = T V given);  normalize: U[:, b] /= sqrt(nrm2[b])
+//   grad_z:       dsdz[2b], dsdz[2b+1] = -Re(u_b^H v_b), Im(u_b^H v_b)
+//   grad_pattern: out[e + nnz b] = Re(conj(U[rows[e], b]) V[cols[e], b])
+//   grad_toeplitz: out[(k + n - 1) + (2n - 1) b] = sum over i - j = k of Re(conj(U[i, b]) V[j, b])
+void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z, zd* U,
+                              int nb);
+void launch_grad_normalize(hipStream_t st, int64_t n, zd* U, const double* nrm2, int nb);
 void launch_grad_z(hipStream_t st, int64_t n, const zd* U, const zd* V, double* dsdz, int nb);
 void launch_grad_pattern(hipStream_t st, int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols,
                          const zd* U, const zd* V, double* out, int nb);

@@ -1680,15 +1680,23 @@ size_t heev_wave_workspace_elems(int n, int nvec_max) {
 //   d sigma / d Re z = -Re(u^H v),  d sigma / d Im z = Im(u^H v),
 //   d sigma / d Re C_ij = Re(conj(u_i) v_j).
 // ---------------------------------------------------------------------------------------
-// U[:, b] = (Y[:, b] - z_b V[:, b]) / sigma_b     (Y = T V)
+// U[:, b] = Y[:, b] - z_b V[:, b]     (Y = T V), unnormalised
 __global__ void k_grad_left_vectors(int64_t n, const zd* __restrict__ Y, const zd* __restrict__ V,
-                                    const zd* __restrict__ z, const double* __restrict__ sig,
-                                    zd* __restrict__ U) {
+                                    const zd* __restrict__ z, zd* __restrict__ U) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (i >= n) return;
     const zd y = Y[i + n * b], v = V[i + n * b];
-    U[i + n * b] = zscale(zsub(y, zmul(z[b], v)), 1.0 / sig[b]);
+    U[i + n * b] = zsub(y, zmul(z[b], v));
+}
+
+// U[:, b] /= sqrt(nrm2[b])
+__global__ void k_grad_normalize(int64_t n, zd* __restrict__ U, const double* __restrict__ nrm2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= n) return;
+    const double s = nrm2[b] > 0.0 ? 1.0 / sqrt(nrm2[b]) : 0.0;
+    U[i + n * b] = zscale(U[i + n * b], s);
 }
 
 // dsdz[b] = (-Re(u^H v), Im(u^H v)) per column, one workgroup per column
@@ -1735,9 +1743,13 @@ k_grad_toeplitz(int64_t n, const zd* __restrict__ U, const zd* __restrict__ V, d
     out[kidx + (2 * n - 1) * b] = s;
 }
 
-void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z,
-                              const double* sig, zd* U, int nb) {
-    hipLaunchKernelGGL(k_grad_left_vectors, dim3(cdiv(n, 256), nb), dim3(256), 0, st, n, Y, V, z, sig, U);
+void launch_grad_left_vectors(hipStream_t st, int64_t n, const zd* Y, const zd* V, const zd* z, zd* U,
+                              int nb) {
+    hipLaunchKernelGGL(k_grad_left_vectors, dim3(cdiv(n, 256), nb), dim3(256), 0, st, n, Y, V, z, U);
+    LAUNCH_CHECK();
+}
+void launch_grad_normalize(hipStream_t st, int64_t n, zd* U, const double* nrm2, int nb) {
+    hipLaunchKernelGGL(k_grad_normalize, dim3(cdiv(n, 256), nb), dim3(256), 0, st, n, U, nrm2);
     LAUNCH_CHECK();
 }
 void launch_grad_z(hipStream_t st, int64_t n, const zd* U, const zd* V, double* dsdz, int nb) {

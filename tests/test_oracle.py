@@ -271,3 +271,31 @@ def test_truth_methods_agree():
     c = truth.sparse_sigma_min(sp.csc_matrix(C), Z)
     assert np.allclose(a, b, rtol=1e-10)
     assert np.allclose(a, c, rtol=1e-8)
+
+
+def test_adjoint_formulas_pinned_by_finite_differences():
+    """The oracle's adjoint (dsigma/dz and the Toeplitz-diagonal gradient from the singular
+    vectors) against central differences of the FP64 SVD sigma_min, h = 1e-6."""
+    from oracle import truth
+    n = 30
+    rng = np.random.default_rng(5)
+    t = rng.standard_normal(2 * n - 1)
+    i = np.arange(n)
+    def mat(tt):
+        return tt[(i[:, None] - i[None, :]) + (n - 1)].astype(complex)
+    C = mat(t)
+    h = 1e-6
+    for z in (0.3 - 0.2j, -1.1 + 0.7j, 2.0 + 0.1j):
+        s, u, v = truth.svd_triplet(C, z)
+        T, Q = sla.schur(C, output="complex")
+        vs = Q.conj().T @ v                         # same vector in the Schur basis
+        dz, U = O.sigma_grad_from_vectors(T, [z], vs[:, None])
+        fx = (truth.svd_sigma_min(C, [z + h])[0] - truth.svd_sigma_min(C, [z - h])[0]) / (2 * h)
+        fy = (truth.svd_sigma_min(C, [z + 1j * h])[0] - truth.svd_sigma_min(C, [z - 1j * h])[0]) / (2 * h)
+        assert abs(dz[0, 0] - fx) < 1e-7 and abs(dz[0, 1] - fy) < 1e-7
+        g = O.toeplitz_grad(Q @ U, v[:, None])[:, 0]
+        for kk in (0, n - 1, n + 3, 2 * n - 2):
+            e = np.zeros_like(t)
+            e[kk] = h
+            fd = (truth.svd_sigma_min(mat(t + e), [z])[0] - truth.svd_sigma_min(mat(t - e), [z])[0]) / (2 * h)
+            assert abs(g[kk] - fd) < 1e-7, (kk, g[kk], fd)
