@@ -1114,11 +1114,16 @@ int rls_test_trsm(int64_t n, const double* T, double zr, double zi, int k, int u
         ctx->Dinv.ensure((size_t)cdiv(n, 64) * 4096);
         ctx->h2d(dX.p, reinterpret_cast<const zd*>(X), (size_t)n * k);
         launch_tri_diag_inv(ctx->st, n, ctx->T.p, n, zmk(zr, zi), ctx->Dinv.p);
-        // two solves in a row exercise the epoch/ticket bookkeeping
+        // a solve of a scratch copy first, so the checked solve runs with the next epoch
+        DArr<zd> dY;
+        dY.ensure((size_t)n * k);
+        launch_copy2d(ctx->st, n, k, dX.p, n, dY.p, n);
+        ctx->tri_solve(dY.p, k, upper != 0);
         ctx->tri_solve(dX.p, k, upper != 0);
         ctx->d2h(reinterpret_cast<zd*>(X), dX.p, (size_t)n * k);
         ctx->check_tri_err();
         dX.release();
+        dY.release();
         return RLS_OK;
     });
     rls_ctx_destroy(ctx);
@@ -1187,3 +1192,39 @@ extern "C" int rls_test_heev_batch_time(int n, const double* A, int batch, int r
 }
 
 extern "C" void rls_test_heev_phases(long long* out) { read_heev_phases(out); }
+
+// Average time of one shifted triangular solve (n x n upper T, k right-hand sides).
+extern "C" int rls_test_trsm_time(int64_t n, const double* T, int k, int upper, int reps, double* ms_out) {
+    rls_ctx* ctx = nullptr;
+    int rc = rls_ctx_create(0, &ctx);
+    if (rc) return rc;
+    rc = guarded(ctx, [&]() -> int {
+        if (rls_set_tri_dense(ctx, T, n, n) != RLS_OK) throw RlsError{RLS_E_ARG, ctx->err};
+        DArr<zd> dX;
+        dX.ensure((size_t)n * k);
+        std::vector<cplx> X0((size_t)n * k);
+        for (size_t e = 0; e < X0.size(); ++e) X0[e] = cplx(u01_splitmix(7, 2 * e), u01_splitmix(7, 2 * e + 1));
+        ctx->h2d(dX.p, reinterpret_cast<zd*>(X0.data()), X0.size());
+        ctx->Dinv.ensure((size_t)cdiv(n, 64) * 4096);
+        launch_tri_diag_inv(ctx->st, n, ctx->T.p, n, zmk(0.3, -0.2), ctx->Dinv.p);
+        ctx->tri_solve(dX.p, k, upper != 0);
+        ctx->sync();
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, ctx->st));
+        for (int i = 0; i < reps; ++i) ctx->tri_solve(dX.p, k, upper != 0);
+        HIP_TRY(hipEventRecord(e1, ctx->st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *ms_out = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->check_tri_err();
+        dX.release();
+        return RLS_OK;
+    });
+    rls_ctx_destroy(ctx);
+    return rc;
+}

@@ -656,38 +656,73 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     const int j = upper ? nblk - 1 - t : t;
     const int64_t o = (int64_t)j * 64;
     const int m = (int)min((int64_t)64, n - o);
+    // T block of dependency kb into registers: 16 elements per thread, all loads issued
+    // before any is consumed.  upper: Ts[a][c] = T[o + a, ko + c] (lanes over a);
+    // lower: Ts[a][c] = conj(T[ko + c, o + a]) (lanes over c, the contiguous index).
+    zd pre[16];
+    auto load_T = [&](int kb) {
+        const int64_t ko = (int64_t)kb * 64;
+        const int mk = (int)min((int64_t)64, n - ko);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u;
+            const int i0 = idx & 63, i1 = idx >> 6;
+            if (upper) pre[u] = (i0 < m && i1 < mk) ? T[(o + i0) + ldt * (ko + i1)] : zzero();
+            else pre[u] = (i1 < m && i0 < mk) ? zconj(T[(ko + i0) + ldt * (o + i1)]) : zzero();
+        }
+    };
+    auto store_T = [&]() {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u;
+            const int i0 = idx & 63, i1 = idx >> 6;
+            if (upper) Ts[i0][i1] = pre[u];
+            else Ts[i1][i0] = pre[u];
+        }
+    };
+    // dependencies: upper -> blocks j+1..nblk-1 (descending), lower -> blocks 0..j-1
+    const int ndep = upper ? (nblk - 1 - j) : j;
+    auto dep = [&](int q) { return upper ? nblk - 1 - q : q; };
+    if (ndep > 0) load_T(dep(0));
     // diagonal-block inverse (upper: Dinv_j, lower: Dinv_j^H) and right-hand side
-    const zd* Di = Dinv + (int64_t)j * 4096;
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int a = idx & 63, c = idx >> 6;
-        Ds[a][c] = upper ? Di[a + 64 * c] : zconj(Di[c + 64 * a]);
-    }
-    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
-        const int a = idx & 63, c = idx >> 6;
-        bj[a][c] = (a < m && c < k) ? X[(o + a) + ldx * c] : zzero();
+    {
+        const zd* Di = Dinv + (int64_t)j * 4096;
+        zd dv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u;
+            const int a = idx & 63, c = idx >> 6;
+            dv[u] = upper ? Di[a + 64 * c] : Di[c + 64 * a];
+        }
+        zd bv[PT_KC * 64 / 256];
+#pragma unroll
+        for (int u = 0; u < PT_KC * 64 / 256; ++u) {
+            const int idx = tid + 256 * u;
+            const int a = idx & 63, c = idx >> 6;
+            bv[u] = (a < m && c < k) ? X[(o + a) + ldx * c] : zzero();
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u;
+            const int a = idx & 63, c = idx >> 6;
+            Ds[a][c] = upper ? dv[u] : zconj(dv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < PT_KC * 64 / 256; ++u) {
+            const int idx = tid + 256 * u;
+            bj[idx & 63][idx >> 6] = bv[u];
+        }
     }
     zd acc[PT_KC];
 #pragma unroll
     for (int c = 0; c < PT_KC; ++c) acc[c] = zzero();
-    // dependencies: upper -> blocks j+1..nblk-1 (descending), lower -> blocks 0..j-1
-    const int ndep = upper ? (nblk - 1 - j) : j;
     for (int q = 0; q < ndep; ++q) {
-        const int kb = upper ? nblk - 1 - q : q;
+        const int kb = dep(q);
         const int64_t ko = (int64_t)kb * 64;
         const int mk = (int)min((int64_t)64, n - ko);
-        __syncthreads();
-        // T block (independent of other workgroups): upper reads T[j, kb]; lower reads
-        // (T[kb, j])^H, i.e. Ts[a][c] = conj(T[ko + c, o + a]).
-        for (int idx = tid; idx < 4096; idx += 256) {
-            const int a = idx & 63, c = idx >> 6;
-            if (upper) {
-                Ts[a][c] = (a < m && c < mk) ? T[(o + a) + ldt * (ko + c)] : zzero();
-            } else {
-                // coalesced over c: element (row ko+c', col o+a') with c' = idx & 63
-                const int cc = idx & 63, aa = idx >> 6;
-                Ts[aa][cc] = (aa < m && cc < mk) ? zconj(T[(ko + cc) + ldt * (o + aa)]) : zzero();
-            }
-        }
+        __syncthreads();                       // the previous block's products are done with Ts
+        store_T();
+        if (q + 1 < ndep) load_T(dep(q + 1));  // in flight while this dependency is awaited
         if (tid == 0) {
             unsigned int spins = 0;
             while (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -698,9 +733,19 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         }
         __syncthreads();
         if (s_bail) break;
-        for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
-            const int a = idx & 63, c = idx >> 6;
-            xs[a][c] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
+        {
+            zd xv[PT_KC * 64 / 256];
+#pragma unroll
+            for (int u = 0; u < PT_KC * 64 / 256; ++u) {
+                const int idx = tid + 256 * u;
+                const int a = idx & 63, c = idx >> 6;
+                xv[u] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
+            }
+#pragma unroll
+            for (int u = 0; u < PT_KC * 64 / 256; ++u) {
+                const int idx = tid + 256 * u;
+                xs[idx & 63][idx >> 6] = xv[u];
+            }
         }
         __syncthreads();
         // acc[lane][c] += sum over this wave's quarter of the block columns
@@ -719,13 +764,19 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         }
         __syncthreads();
     }
-    // x_j = Ds * bj
+    // x_j = Ds * bj: thread (row a, column pair), two independent accumulation chains
     for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
         const int a = idx & 63, c = idx >> 6;
-        zd s = zzero();
-        if (a < m && c < k)
-            for (int q = 0; q < m; ++q) s = zfma(Ds[a][q], bj[q][c], s);
-        xs[a][c] = s;
+        zd s0 = zzero(), s1 = zzero();
+        if (a < m && c < k) {
+            int q = 0;
+            for (; q + 1 < m; q += 2) {
+                s0 = zfma(Ds[a][q], bj[q][c], s0);
+                s1 = zfma(Ds[a][q + 1], bj[q + 1][c], s1);
+            }
+            if (q < m) s0 = zfma(Ds[a][q], bj[q][c], s0);
+        }
+        xs[a][c] = zadd(s0, s1);
     }
     __syncthreads();
     for (int idx = tid; idx < 64 * PT_KC; idx += 256) {

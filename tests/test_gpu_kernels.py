@@ -72,7 +72,8 @@ def test_heev_gap_rule_and_clusters():
     assert np.linalg.norm(P @ Gk - Gk) < 1e-6
 
 
-@pytest.mark.parametrize("n,upper,k", [(200, 1, 6), (200, 0, 6), (1000, 1, 3), (1000, 0, 8), (64, 1, 1)])
+@pytest.mark.parametrize("n,upper,k", [(200, 1, 6), (200, 0, 6), (1000, 1, 3), (1000, 0, 8), (64, 1, 1),
+                                        (513, 1, 8), (600, 0, 5), (2000, 1, 6), (2000, 0, 6)])
 def test_trsm_matches_scipy(n, upper, k):
     _say(f"trsm n={n} upper={upper} k={k}")
     rng = np.random.default_rng(n + upper)
