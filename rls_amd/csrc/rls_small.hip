@@ -23,16 +23,19 @@ namespace {
 
 constexpr int HEEV_THREADS = 1024;
 
+// Number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2.  The pivot
+// reciprocal uses the hardware estimate refined by one Newton step (only signs are used).
 __device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-    // number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2
     int cnt = 0;
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0) cnt++;
+    cnt += (q < 0);
     for (int i = 1; i < n; ++i) {
-        q = d[i] - x - e2[i - 1] / q;
+        double rq = __builtin_amdgcn_rcp(q);
+        rq = rq * fma(-q, rq, 2.0);
+        q = (d[i] - x) - e2[i - 1] * rq;
         if (fabs(q) < pivmin) q = -pivmin;
-        if (q < 0) cnt++;
+        cnt += (q < 0);
     }
     return cnt;
 }
@@ -229,38 +232,39 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
         const bool has_t = !(t.x == 0.0 && t.y == 0.0);
-        // 2-D split: thread (row gi, column group gg); groups reduce through LDS
+        // 2-D split: thread (row gi, column group gg); groups reduce through LDS.  Columns
+        // are taken HB at a time with all HB loads issued before any is used, so a thread
+        // keeps HB memory requests in flight instead of one.
         {
             const int RG = 256;
             const int NG = nthr / RG;
             const int gi = tid % RG, gg = tid / RG;
+            constexpr int HB = 8;
             for (int i0 = 0; i0 < m; i0 += RG) {
                 const int i = i0 + gi;
                 zd acc0 = zzero(), acc1 = zzero();
                 if (i < m) {
                     const zd pvi = pending ? pv[i + 1] : zzero();
                     const zd pwi = pending ? pw[i + 1] : zzero();
-                    int j = gg;
-                    for (; j + NG < m; j += 2 * NG) {
-                        const int j1 = j + NG;
-                        zd a0 = A22[i + (int64_t)n * j];
-                        zd a1 = A22[i + (int64_t)n * j1];
-                        if (pending) {
-                            a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
-                            a1 = zsub(a1, zadd(zmul(pvi, zconj(pw[j1 + 1])), zmul(pwi, zconj(pv[j1 + 1]))));
-                            A22[i + (int64_t)n * j] = a0;
-                            A22[i + (int64_t)n * j1] = a1;
+                    for (int jb = gg; jb < m; jb += HB * NG) {
+                        zd a[HB];
+#pragma unroll
+                        for (int u = 0; u < HB; ++u) {
+                            const int j = jb + u * NG;
+                            a[u] = (j < m) ? A22[i + (int64_t)n * j] : zzero();
                         }
-                        acc0 = zfma(a0, sv[j], acc0);
-                        acc1 = zfma(a1, sv[j1], acc1);
-                    }
-                    for (; j < m; j += NG) {
-                        zd a0 = A22[i + (int64_t)n * j];
-                        if (pending) {
-                            a0 = zsub(a0, zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
-                            A22[i + (int64_t)n * j] = a0;
+#pragma unroll
+                        for (int u = 0; u < HB; ++u) {
+                            const int j = jb + u * NG;
+                            if (j < m) {
+                                if (pending) {
+                                    a[u] = zsub(a[u], zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
+                                    A22[i + (int64_t)n * j] = a[u];
+                                }
+                                if (u & 1) acc1 = zfma(a[u], sv[j], acc1);
+                                else acc0 = zfma(a[u], sv[j], acc0);
+                            }
                         }
-                        acc0 = zfma(a0, sv[j], acc0);
                     }
                 }
                 s_acc[gg][gi] = zadd(acc0, acc1);
@@ -305,6 +309,10 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __syncthreads();
     for (int i = tid; i < n; i += nthr) e2[i] = e[i] * e[i];
     __syncthreads();
+    // the tridiagonal in LDS for the Sturm counts and the inverse iterations
+    __shared__ double s_d[HEEV_THREADS], s_e[HEEV_THREADS], s_e2[HEEV_THREADS];
+    for (int i = tid; i < n; i += nthr) { s_d[i] = d[i]; s_e[i] = e[i]; s_e2[i] = e2[i]; }
+    __syncthreads();
     if (b == 0 && tid == 0) g_heev_phase[1] = wall_clock64();
 
     // 3. Gershgorin interval
@@ -336,21 +344,26 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
                                             : ((2 * r < n) ? 2 * r : n);
         int target = n_init;
         while (true) {
+          if (have < target || !have_max) {
             if (tid == 0) {
                 int kq = 0, kc = 0;
                 if (!have_max && mode_fixed <= 0) s_idx[kq++] = n - 1;
-                for (int i = have; i < target && kq < 64; ++i) {
+                // the first call serves the first window; later ones fill a whole batch of
+                // 64 indices ahead, since later windows are then checked without new calls
+                const int upto = have == 0 ? target : (target > n ? n : (have + 64 > n ? n : have + 64));
+                for (int i = have; i < (upto > target ? upto : target) && kq < 64; ++i) {
                     if (i == n - 1 && !have_max && mode_fixed <= 0) { ++kc; continue; }
                     s_idx[kq++] = i; ++kc;
                 }
                 s_k = kq; s_kc = kc;
             }
             __syncthreads();
-            multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
+            multisect(s_d, s_e2, n, s_idx, s_k, lam, gl, gu, pivmin);
             __syncthreads();
             have_max = true;
             have += s_kc;
             if (have < target) continue;
+          }
             if (tid == 0) {
                 if (mode_fixed > 0) {
                     s_rr = n_init; s_done = 1;
@@ -384,7 +397,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
                 s_k = kq; s_kc = kq;
             }
             __syncthreads();
-            multisect(d, e2, n, s_idx, s_k, lam, gl, gu, pivmin);
+            multisect(s_d, s_e2, n, s_idx, s_k, lam, gl, gu, pivmin);
             __syncthreads();
             have += s_kc;
         }
@@ -410,7 +423,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
                     y[i] = (double)(s & 0xFFFFFF) / 16777216.0 - 0.5;
                 }
             }
-            tri_shifted_solve(d, e, n, lam[j], y, scratch + (int64_t)j * 4 * n, floorv);
+            tri_shifted_solve(s_d, s_e, n, lam[j], y, scratch + (int64_t)j * 4 * n, floorv);
             double nr = 0.0;
             for (int i = 0; i < n; ++i) nr += y[i] * y[i];
             nr = 1.0 / sqrt(nr);
@@ -1297,21 +1310,6 @@ constexpr int HW_NMAX = 128;
 
 __device__ __forceinline__ int poff(int n, int j) { return j * n - (j * (j - 1)) / 2; }
 
-__device__ int sturm_count_rcp(const double* d, const double* e2, int n, double x, double pivmin) {
-    int cnt = 0;
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += (q < 0);
-    for (int i = 1; i < n; ++i) {
-        double rq = __builtin_amdgcn_rcp(q);
-        rq = rq * fma(-q, rq, 2.0);            // one Newton step
-        q = (d[i] - x) - e2[i - 1] * rq;
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += (q < 0);
-    }
-    return cnt;
-}
-
 // wave multisection for wanted indices idx[0..k) (k <= 64)
 __device__ void multisect_wave(const double* d, const double* e2, int n, const int* idx, int k,
                                double* lam, double gl, double gu, double pivmin, double* s_lo,
@@ -1328,7 +1326,7 @@ __device__ void multisect_wave(const double* d, const double* e2, int n, const i
             double x = lo + (hi - lo) * (double)(t + 1) / (double)(G + 1);
             s_x[lane] = x;
             s_c[lane] = (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin)
-                            ? -1 : sturm_count_rcp(d, e2, n, x, pivmin);
+                            ? -1 : sturm_count(d, e2, n, x, pivmin);
         }
         __syncthreads();
         int active = 0;
@@ -1519,10 +1517,12 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
                                             : ((2 * r < n) ? 2 * r : n);
         int target = n_init;
         while (true) {
+          if (have < target || !have_max) {
             int kq = 0, kc = 0;
             if (tid0 == 0) {
                 if (!have_max && mode_fixed <= 0) s_idx[kq++] = n - 1;
-                for (int i = have; i < target && kq < 64; ++i) {
+                const int upto = have == 0 ? target : (target > n ? n : (have + 32 > n ? n : have + 32));
+                for (int i = have; i < (upto > target ? upto : target) && kq < 64; ++i) {
                     if (i == n - 1 && !have_max && mode_fixed <= 0) { ++kc; continue; }
                     s_idx[kq++] = i; ++kc;
                 }
@@ -1535,6 +1535,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             have_max = true;
             have += kc;
             if (have < target) continue;
+          }
             bool done;
             if (mode_fixed > 0) {
                 rr = n_init; done = true;
