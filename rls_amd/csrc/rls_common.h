@@ -59,6 +59,24 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return s;
 }
 
+// Two block-wide sums with one pair of barriers; red holds >= 2 * (blockDim.x / 64) doubles.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < nw; ++i) { sa += red[2 * i]; sb += red[2 * i + 1]; }
+    __syncthreads();
+    a = sa;
+    b = sb;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;

@@ -167,6 +167,8 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     double* Y = reinterpret_cast<double*>(tau + n);   // n x nvec_max real eigenvectors of T
     double* scratch = Y + (int64_t)n * nvec_max;     // per-vector tri-solve workspace 4n x nvec
     __shared__ double red[HEEV_THREADS / 64];
+    __shared__ double red2s[2 * (HEEV_THREADS / 64)];
+    __shared__ zd s_alpha;
     __shared__ zd sv[1024];    // current Householder vector v_k
     __shared__ zd sw[1024];    // w_k
     __shared__ zd pv[1024];    // pending update vectors (v_{k-1}, w_{k-1})
@@ -193,42 +195,39 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     for (int k = 0; k < n - 1; ++k) {
         const int m = n - k - 1;                 // x = A[k+1:n, k]
         zd* x = M + (k + 1) + (int64_t)n * k;
-        // x' = x - (pending update of step k-1 restricted to column k, rows k+1..)
-        // in the k-1 frame, row k+1+i <-> index i+1, column k <-> index 0
-        if (pending) {
-            zd w0c = zconj(pw[0]), v0c = zconj(pv[0]);
-            for (int i = tid; i < m; i += nthr) {
-                zd a = x[i];
-                a = zsub(a, zmul(pv[i + 1], w0c));
-                a = zsub(a, zmul(pw[i + 1], v0c));
-                x[i] = a;
-            }
-            if (tid == 0) {
-                zd dk = M[k + (int64_t)n * k];
-                dk.x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
-                M[k + (int64_t)n * k] = dk;
-            }
+        // x' = x - (pending update of step k-1 restricted to column k, rows k+1..), kept in a
+        // register of thread i (m < blockDim); in the k-1 frame row k+1+i <-> index i+1 and
+        // column k <-> index 0
+        zd xi = zzero();
+        if (tid < m) {
+            xi = x[tid];
+            if (pending)
+                xi = zsub(xi, zadd(zmul(pv[tid + 1], zconj(pw[0])), zmul(pw[tid + 1], zconj(pv[0]))));
         }
-        __syncthreads();
-        double part = 0.0;
-        for (int i = 1 + tid; i < m; i += nthr) part += zabs2(x[i]);
+        if (tid == 0) {
+            double dk = M[k + (int64_t)n * k].x;
+            if (pending) dk -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+            d[k] = dk;
+            s_alpha = xi;
+        }
+        double part = (tid >= 1 && tid < m) ? zabs2(xi) : 0.0;
         double xn2 = block_sum(part, red);
-        zd alpha = x[0];
+        const zd alpha = s_alpha;
         zd t = zzero();
         double beta;
         if (xn2 == 0.0 && alpha.y == 0.0) {
             beta = alpha.x;
-            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zzero();
+            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zzero();
         } else {
             double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
             beta = (alpha.x >= 0.0) ? -nrm : nrm;
             t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
             zd scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
-            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : zmul(x[i], scal);
+            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zmul(xi, scal);
         }
+        if (tid >= 1 && tid < m) x[tid] = sv[tid];   // keep v for the back-transform
+        if (tid == 0) { e[k] = beta; tau[k] = t; }
         __syncthreads();
-        if (tid == 0) { e[k] = beta; tau[k] = t; d[k] = M[k + (int64_t)n * k].x; }
-        for (int i = 1 + tid; i < m; i += nthr) x[i] = sv[i];   // keep v for the back-transform
         // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
         const bool has_t = !(t.x == 0.0 && t.y == 0.0);
@@ -284,13 +283,12 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
             continue;
         }
         // alpha2 = -1/2 tau (w^H v); w += alpha2 v
-        double pr = 0.0, pi = 0.0;
-        for (int i = tid; i < m; i += nthr) {
-            zd c = zcmul(sw[i], sv[i]);
-            pr += c.x; pi += c.y;
+        double whv_r = 0.0, whv_i = 0.0;
+        if (tid < m) {
+            const zd c = zcmul(sw[tid], sv[tid]);
+            whv_r = c.x; whv_i = c.y;
         }
-        double whv_r = block_sum(pr, red);
-        double whv_i = block_sum(pi, red);
+        block_sum2(whv_r, whv_i, red2s);
         zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
         for (int i = tid; i < m; i += nthr) {
             pw[i] = zfma(a2, sv[i], sw[i]);
