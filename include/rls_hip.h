@@ -95,6 +95,26 @@ void rls_tri_control_default(rls_tri_control* c, double normC);
  * column-major) is copied to device memory. */
 int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld);
 
+/* Sparse operator for the sparse psTriRecy overload (psTriRecy.jl:47-61, M = C, S = I here):
+ * C in 0-based CSR (complex values).  Julia's 1-based SparseMatrixCSC of C is, read as CSR,
+ * C^T; the shim transposes it once.  The refine factor of each refine shift z comes from the
+ * host through the callback below (the reference factors M - zS with SPQR on the host,
+ * psTriRecy.jl:222-234, 513-530): the callback computes a sparse LU
+ *     Pr (C - zI) Pc = L U,   (Pr x)[perm_r[j]] = x[j],   (Pc y)[j] = y[perm_c[j]],
+ * and hands it over with rls_set_sparse_factor before returning 0. */
+typedef struct rls_csr {
+    int64_t n;
+    const int64_t* rowptr;   /* n + 1 */
+    const int64_t* colind;   /* rowptr[n] */
+    const double* values;    /* rowptr[n] complex */
+} rls_csr;
+typedef int (*rls_factor_fn)(void* user, double z_re, double z_im);
+int rls_set_csr(rls_ctx* ctx, const rls_csr* C);
+int rls_set_factor_callback(rls_ctx* ctx, rls_factor_fn fn, void* user);
+/* L unit lower, U upper, LH = L^H (unit upper), UH = U^H (lower); all n x n CSR. */
+int rls_set_sparse_factor(rls_ctx* ctx, const rls_csr* L, const rls_csr* U, const rls_csr* LH,
+                          const rls_csr* UH, const int64_t* perm_r, const int64_t* perm_c);
+
 /* Run psTriRecy over the nZ shifts `Z` (complex, traversal order, i.e. rec.Z) and write
  * sigma_min to `sigma_out` (nZ doubles, NaN where unset).  Replaces _psTriRecy_impl
  * (src/core/psTriRecy.jl:708-881).  `stats` may be NULL. */

@@ -26,7 +26,7 @@ from . import _capi
 __all__ = [
     "Recursion", "recursion_grid", "recursion_grid_zigzag", "recursion_natural_permutation",
     "reorder_to_natural", "reorder_to_natural_grid", "PsTriRecyControl", "PsTriRecyStats",
-    "psTriRecy", "schur_preprocess", "schur_factor", "TriSolver",
+    "psTriRecy", "schur_preprocess", "schur_factor", "TriSolver", "SparseTriSolver",
 ]
 
 EPS = float(np.finfo(np.float64).eps)
@@ -359,6 +359,84 @@ class TriSolver:
         return sigma, stats
 
 
+class SparseTriSolver(TriSolver):
+    """Sparse psTriRecy operator (psTriRecy.jl:47-61 with M = C, S = I): C stays sparse on
+    the device (CSR products for the recycle space), and each refine shift z gets a sparse LU
+    Pr (C - zI) Pc = L U from the host (scipy's SuperLU with a COLAMD column ordering, in the
+    place of the reference's SPQR factor, psTriRecy.jl:222-234, 513-530), whose triangular
+    solves run on the device."""
+
+    def __init__(self, Cmat, device: int = 0, permc_spec: str = "COLAMD"):
+        import scipy.sparse as sp
+        self.ctx = _capi.Context(device)
+        A = sp.csr_matrix(Cmat, dtype=np.complex128)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError(f"psTriRecy needs a square matrix, got size {A.shape}")
+        A.sort_indices()
+        self.n = A.shape[0]
+        self._last_nZ = 0
+        self._C = A.tocsc()
+        self._permc = permc_spec
+        self._keep = []
+        self.factor_count = 0
+        self.factor_time_s = 0.0
+        csr, keep = self._csr(A)
+        self.ctx.check(self.ctx.lib.rls_set_csr(self.ctx.h, C.byref(csr)))
+
+        def cb(user, zr, zi):
+            return self._factor(complex(zr, zi))
+        self._cb = _capi.FACTOR_FN(cb)
+        self.ctx.check(self.ctx.lib.rls_set_factor_callback(self.ctx.h, self._cb, None))
+
+    @staticmethod
+    def _csr(A):
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        ci = np.ascontiguousarray(A.indices, dtype=np.int64)
+        v = np.ascontiguousarray(A.data, dtype=np.complex128)
+        s = _capi.Csr(A.shape[0], rp.ctypes.data, ci.ctypes.data, v.ctypes.data)
+        return s, (rp, ci, v)
+
+    def _op(self, X, which: int) -> np.ndarray:
+        X = np.asfortranarray(X, dtype=np.complex128)
+        Y = np.zeros_like(X, order="F")
+        f = self.ctx.lib.rls_test_sparse_op
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        self.ctx.check(f(self.ctx.h, X.ctypes.data, X.shape[1], which, Y.ctypes.data))
+        return Y
+
+    def apply_operator(self, X, adjoint: bool = False) -> np.ndarray:
+        """C X or C^H X on the device (test helper)."""
+        return self._op(X, 1 if adjoint else 0)
+
+    def apply_factor_solve(self, X, adjoint: bool = False) -> np.ndarray:
+        """(C - zI)^{-1} X or (C - zI)^{-H} X with the last factor (test helper)."""
+        return self._op(X, 3 if adjoint else 2)
+
+    def _factor(self, z: complex) -> int:
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        t0 = time.perf_counter()
+        n = self.n
+        lu = spla.splu((self._C - z * sp.identity(n, dtype=np.complex128, format="csc")).tocsc(),
+                       permc_spec=self._permc)
+        L = lu.L.tocsr()
+        U = lu.U.tocsr()
+        mats = [L, U, L.conj().T.tocsr(), U.conj().T.tocsr()]
+        structs, keep = [], []
+        for M in mats:
+            M.sort_indices()
+            s_, k_ = self._csr(M)
+            structs.append(s_)
+            keep.append(k_)
+        pr = np.ascontiguousarray(lu.perm_r, dtype=np.int64)
+        pc = np.ascontiguousarray(lu.perm_c, dtype=np.int64)
+        rc = self.ctx.lib.rls_set_sparse_factor(self.ctx.h, *[C.byref(s_) for s_ in structs],
+                                                pr.ctypes.data, pc.ctypes.data)
+        self.factor_count += 1
+        self.factor_time_s += time.perf_counter() - t0
+        return rc
+
+
 def initial_block(n: int, r: int, seed: int = 0) -> np.ndarray:
     """The library's start block (the reference's rand!(X), psTriRecy.jl:607): uniform [0,1)
     real/imaginary parts from a counter-based splitmix64 stream, element e = i + n j uses
@@ -405,12 +483,18 @@ def psTriRecy(C, rec: Recursion, control: PsTriRecyControl = None, verbose: int 
     io = io or sys.stdout
     if verbose >= 1:
         _print_startup(io, rec, control)
+    import scipy.sparse as sp
     t0 = time.perf_counter_ns()
-    T = schur_preprocess(C)
+    if sp.issparse(C):
+        # sparse overload (psTriRecy.jl:47-61): C stays sparse, refine factors per shift
+        solver = SparseTriSolver(C, device)
+    else:
+        solver = TriSolver(schur_preprocess(C), device)
     t_pre = time.perf_counter_ns() - t0
-    solver = TriSolver(T, device)
     sigma, stats = solver.solve(rec.Z, control, rec)
     stats.preprocess_time_ns = int(t_pre)
+    if isinstance(solver, SparseTriSolver):
+        stats.qr_time_ns = int(solver.factor_time_s * 1e9)   # shifted-factor time (SPQR's place)
     if verbose >= 2:
         _print_steps(io, rec, sigma, stats)
     if verbose >= 1:

@@ -29,6 +29,9 @@ EXPORTED = [
     "rls_tri_singular_vectors",
     "rls_sigma_grad_pattern",
     "rls_sigma_grad_toeplitz",
+    "rls_set_csr",
+    "rls_set_factor_callback",
+    "rls_set_sparse_factor",
 ]
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 
@@ -84,6 +87,13 @@ class TriStats(C.Structure):
     ]
 
 
+class Csr(C.Structure):
+    """rls_csr: 0-based CSR with interleaved complex values."""
+    _fields_ = [("n", C.c_int64), ("rowptr", C.c_void_p), ("colind", C.c_void_p), ("values", C.c_void_p)]
+
+
+FACTOR_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double)
+
 _lib = None
 
 
@@ -120,7 +130,11 @@ def load() -> C.CDLL:
                        ("rls_sigma_grad", [vp, i64, i64, vp]),
                        ("rls_tri_singular_vectors", [vp, i64, i64, vp, vp]),
                        ("rls_sigma_grad_pattern", [vp, i64, i64, i64, vp, vp, vp]),
-                       ("rls_sigma_grad_toeplitz", [vp, i64, i64, vp])):
+                       ("rls_sigma_grad_toeplitz", [vp, i64, i64, vp]),
+                       ("rls_set_csr", [vp, C.POINTER(Csr)]),
+                       ("rls_set_factor_callback", [vp, FACTOR_FN, vp]),
+                       ("rls_set_sparse_factor", [vp, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
+                                                  C.POINTER(Csr), vp, vp])):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_int

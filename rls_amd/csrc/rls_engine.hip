@@ -63,9 +63,27 @@ struct rls_ctx {
     int device = 0;
     hipStream_t st = nullptr;
     std::string err;
-    // operator
+    // operator: dense upper-triangular Schur factor T, or (sparse path) C in CSR with the CSR
+    // of C^H, and the host-supplied factor Pr (C - zI) Pc = L U of the current refine shift
     int64_t n = 0;
     DArr<zd> T;
+    bool sparse = false;
+    DArr<int64_t> crp, cci, hrp, hci;
+    DArr<zd> cv, hv;
+    struct SpTri {
+        DArr<int64_t> rp, ci;
+        DArr<zd> v;
+        DArr<int> dep;
+    };
+    SpTri fL, fU, fLH, fUH;
+    DArr<int64_t> perm_r, perm_c;
+    DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block)]
+    DArr<int> sp_err;
+    unsigned int sp_epoch = 0;
+    rls_factor_fn factor_cb = nullptr;
+    void* factor_user = nullptr;
+    bool factor_ready = false;
+    DArr<zd> sp_tmp;
     // recycle space
     int r = 0, pmax = 0, p = 0, prm = 0;
     int64_t ld = 0;
@@ -149,6 +167,10 @@ struct rls_ctx {
         DArr<double>* ds[] = {&sig, &lam, &offI, &nrm2, &lamev, &sigr, &tausd, &lamev2, &gnrm, &gout};
         grows.release();
         gcols.release();
+        DArr<int64_t>* is[] = {&crp, &cci, &hrp, &hci, &perm_r, &perm_c};
+        for (auto* a : is) a->release();
+        cv.release(); hv.release(); sp_tmp.release(); sp_cnt.release(); sp_err.release();
+        for (SpTri* f : {&fL, &fU, &fLH, &fUH}) { f->rp.release(); f->ci.release(); f->v.release(); f->dep.release(); }
         tri_ctr.release();
         tri_flags.release();
         tri_err.release();
@@ -314,13 +336,74 @@ struct rls_ctx {
     }
     bool tri_ticket_base_ok = false;
     void check_tri_err() {
-        if (!tri_err.p) return;
         int e = 0;
-        d2h(&e, tri_err.p, 1);
+        if (tri_err.p) d2h(&e, tri_err.p, 1);
         if (e) throw RlsError{RLS_E_HIP, "triangular solve: dependency wait exceeded its bound"};
+        if (sp_err.p) d2h(&e, sp_err.p, 1);
+        if (e) throw RlsError{RLS_E_HIP, "sparse triangular solve: dependency wait exceeded its bound"};
     }
-    void apply_C(zd* X, int k) { tri_solve(X, k, false); }   // X <- (T - zI)^{-H} X
-    void apply_CH(zd* X, int k) { tri_solve(X, k, true); }   // X <- (T - zI)^{-1} X
+    // Y = M X and Y = M^H X (n x k): dense triangular multiply or CSR products
+    void apply_M(const zd* X, int k, zd* Y) {
+        if (sparse) launch_csr_spmm(st, n, crp.p, cci.p, cv.p, zzero(), X, n, k, Y, n);
+        else launch_trmm_upper(st, n, T.p, n, X, n, k, Y, n);
+    }
+    void apply_MH(const zd* X, int k, zd* Y) {
+        if (sparse) launch_csr_spmm(st, n, hrp.p, hci.p, hv.p, zzero(), X, n, k, Y, n);
+        else launch_trmmh_upper(st, n, T.p, n, X, n, k, Y, n);
+    }
+    // the refine operator's factor for shift z: dense -> diagonal-block inverses of T - zI;
+    // sparse -> the host's factor callback, which uploads L, U (rls_set_sparse_factor)
+    void refine_factor(cplx z) {
+        if (!sparse) {
+            launch_tri_diag_inv(st, n, T.p, n, zmk(z.real(), z.imag()), Dinv.p);
+            return;
+        }
+        if (!factor_cb) throw RlsError{RLS_E_STATE, "sparse operator: no factor callback set (rls_set_factor_callback)"};
+        factor_ready = false;
+        const int rc = factor_cb(factor_user, z.real(), z.imag());
+        if (rc != 0 || !factor_ready)
+            throw RlsError{RLS_E_NUMERIC, "factor callback failed for a refine shift"};
+    }
+    void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
+        const int nb = (int)cdiv(n, sptrsv_block_rows());
+        if (!sp_cnt.p || sp_cnt.n < (size_t)nb + 1) {
+            sp_cnt.ensure((size_t)nb + 1);
+            sp_err.ensure(1);
+            HIP_TRY(hipMemsetAsync(sp_cnt.p, 0, sizeof(unsigned int) * sp_cnt.n, st));
+            HIP_TRY(hipMemsetAsync(sp_err.p, 0, sizeof(int), st));
+            sp_epoch = 0;
+        }
+        for (int c0 = 0; c0 < k; c0 += 8) {
+            const int kc = std::min(8, k - c0);
+            launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X + (int64_t)n * c0, n, kc,
+                          sp_cnt.p, sp_cnt.p + 1, ++sp_epoch, sp_err.p);
+        }
+    }
+    // X <- A^{-1} X or A^{-H} X for A = C - zI = Pr^T L U Pc^T (sparse path)
+    void sp_apply(zd* X, int k, bool inverse_h) {
+        sp_tmp.ensure((size_t)n * k);
+        kt_begin(RLS_KF_TRSM);
+        if (!inverse_h) {
+            launch_permute_rows(st, n, perm_r.p, X, n, sp_tmp.p, n, k, true);
+            sp_solve(fL, true, true, sp_tmp.p, k);
+            sp_solve(fU, false, false, sp_tmp.p, k);
+            launch_permute_rows(st, n, perm_c.p, sp_tmp.p, n, X, n, k, false);
+        } else {
+            launch_permute_rows(st, n, perm_c.p, X, n, sp_tmp.p, n, k, true);
+            sp_solve(fUH, true, false, sp_tmp.p, k);
+            sp_solve(fLH, false, true, sp_tmp.p, k);
+            launch_permute_rows(st, n, perm_r.p, sp_tmp.p, n, X, n, k, false);
+        }
+        kt_end(RLS_KF_TRSM, 0.0, 0.0);
+    }
+    void apply_C(zd* X, int k) {    // X <- (M - zS)^{-H} X
+        if (sparse) sp_apply(X, k, true);
+        else tri_solve(X, k, false);
+    }
+    void apply_CH(zd* X, int k) {   // X <- (M - zS)^{-1} X
+        if (sparse) sp_apply(X, k, false);
+        else tri_solve(X, k, true);
+    }
 
     // svdRaylRitz! + basisUpdate! on the first nS columns of (S, CS): new X|P (2r or r cols)
     void svd_rr_update(int nS, bool side_L, int keep) {
@@ -361,7 +444,7 @@ struct rls_ctx {
         const bool side_L = (c.lobpcg_side != 'S');
         if (!side_L) throw RlsError{RLS_E_ARG, "lobpcg_side 'S' is not supported on the Schur path"};
         const double tol = c.lobpcg_tol_scale * c.tol;
-        launch_tri_diag_inv(st, n, T.p, n, zmk(z.real(), z.imag()), Dinv.p);
+        refine_factor(z);
         // S = [X P W], CS = [CX CP CW]; svd_rr_update swaps the double buffers, so block
         // pointers are re-derived after every call.
         auto Xb = [&]() { return S.p; };
@@ -435,9 +518,9 @@ struct rls_ctx {
         zd* J5n = J5.p + (int64_t)n * old;
         zd* J1n = J1.p + (int64_t)n * old;
         zd* J2n = J2.p + (int64_t)n * old;
-        launch_trmm_upper(st, n, T.p, n, Vn, n, r, J5n, n);     // J5 = M V
-        launch_trmmh_upper(st, n, T.p, n, J5n, n, r, J1n, n);   // J1 = M^H J5
-        launch_trmmh_upper(st, n, T.p, n, Vn, n, r, J2n, n);    // J2 = M^H J6, J6 = V
+        apply_M(Vn, r, J5n);      // J5 = M V
+        apply_MH(J5n, r, J1n);    // J1 = M^H J5
+        apply_MH(Vn, r, J2n);     // J2 = M^H J6, J6 = V
         zd* h1nn = H1.p + old + (int64_t)prm * old;
         zd* h2nn = H2.p + old + (int64_t)prm * old;
         gram_op(Vn, n, r, J1n, n, r, h1nn, prm);                 // H1[new,new] = V^H J1
@@ -647,7 +730,8 @@ bool residual_ok(double res, double sigma, double tol, double normC) {
 
 int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c, double* sigma,
               rls_tri_stats* stats) {
-    if (ctx->n <= 0 || !ctx->T.p) throw RlsError{RLS_E_STATE, "no operator set (call rls_set_tri_dense)"};
+    if (ctx->n <= 0 || (!ctx->T.p && !ctx->sparse))
+        throw RlsError{RLS_E_STATE, "no operator set (call rls_set_tri_dense or rls_set_csr)"};
     if (c.r <= 0 || c.r > 16) throw RlsError{RLS_E_ARG, "r must be in 1..16"};
     if (c.p <= 0) throw RlsError{RLS_E_ARG, "p_max must be positive"};
     if (c.p_min < 0 || c.p_min > c.p) throw RlsError{RLS_E_ARG, "p_min must satisfy 0 <= p_min <= p_max"};
@@ -812,9 +896,13 @@ void grad_vectors(rls_ctx* ctx, int64_t b0, int nb, bool original_basis) {
     ctx->gU.ensure((size_t)n * GRAD_CHUNK);
     ctx->gnrm.ensure(GRAD_CHUNK);
     const zd* V = ctx->vkeep.p + n * b0;
-    TallOperands ops{};
-    ops.A[0] = ctx->T.p; ops.lda[0] = n; ops.K[0] = (int)n; ops.nops = 1;
-    launch_zgemm_tiled(ctx->st, n, ops, V, n, nb, ctx->gY.p, n);
+    if (ctx->sparse) {
+        launch_csr_spmm(ctx->st, n, ctx->crp.p, ctx->cci.p, ctx->cv.p, zzero(), V, n, nb, ctx->gY.p, n);
+    } else {
+        TallOperands ops{};
+        ops.A[0] = ctx->T.p; ops.lda[0] = n; ops.K[0] = (int)n; ops.nops = 1;
+        launch_zgemm_tiled(ctx->st, n, ops, V, n, nb, ctx->gY.p, n);
+    }
     launch_grad_left_vectors(ctx->st, n, ctx->gY.p, V, ctx->zkeep.p + b0, ctx->gU.p, nb);
     ctx->part.ensure(gram_partial_elems(n, nb, 1));
     launch_colnorm2(ctx->st, n, ctx->gU.p, n, nb, ctx->gnrm.p, ctx->part.p);
@@ -834,6 +922,53 @@ void check_grad_range(rls_ctx* ctx, int64_t first, int64_t count) {
         throw RlsError{RLS_E_STATE, "no singular vectors kept: call rls_tri_keep_vectors(ctx, 1) before rls_tri_solve"};
     if (first < 0 || count < 0 || first + count > ctx->nkeep)
         throw RlsError{RLS_E_ARG, "point range outside the last solve"};
+}
+
+// Validate a 0-based n x n CSR (monotone row pointers, columns in range).
+void check_csr(const rls_csr* A, int64_t n, const char* what) {
+    if (!A || !A->rowptr || (A->rowptr[n] > 0 && (!A->colind || !A->values)))
+        throw RlsError{RLS_E_ARG, std::string(what) + ": null CSR array"};
+    if (A->n != n) throw RlsError{RLS_E_ARG, std::string(what) + ": dimension mismatch"};
+    if (A->rowptr[0] != 0) throw RlsError{RLS_E_ARG, std::string(what) + ": rowptr[0] must be 0"};
+    for (int64_t i = 0; i < n; ++i)
+        if (A->rowptr[i + 1] < A->rowptr[i]) throw RlsError{RLS_E_ARG, std::string(what) + ": rowptr not monotone"};
+    for (int64_t e = 0; e < A->rowptr[n]; ++e)
+        if (A->colind[e] < 0 || A->colind[e] >= n)
+            throw RlsError{RLS_E_ARG, std::string(what) + ": column index outside 0..n-1"};
+}
+
+template <typename T_>
+void upload(rls_ctx* ctx, DArr<T_>& d, const T_* h, size_t cnt) {
+    d.ensure(std::max<size_t>(cnt, 1));
+    if (cnt) ctx->h2d(d.p, h, cnt);
+}
+
+// Triangular factor upload with the dependency analysis of the sync-free solve: for each
+// block of sptrsv_block_rows() rows, the first (lower) or last (upper) other block it reads.
+void upload_tri(rls_ctx* ctx, rls_ctx::SpTri& F, const rls_csr* A, bool lower, const char* what) {
+    const int64_t n = ctx->n;
+    check_csr(A, n, what);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e)
+            if (lower ? A->colind[e] > i : A->colind[e] < i)
+                throw RlsError{RLS_E_ARG, std::string(what) + ": entry outside the triangle"};
+    const int RB = sptrsv_block_rows();
+    const int nb = (int)cdiv(n, RB);
+    std::vector<int> dep(nb);
+    for (int b = 0; b < nb; ++b) {
+        int d = b;
+        for (int64_t i = (int64_t)b * RB; i < std::min<int64_t>(n, (int64_t)(b + 1) * RB); ++i)
+            for (int64_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e) {
+                const int cb = (int)(A->colind[e] / RB);
+                d = lower ? std::min(d, cb) : std::max(d, cb);
+            }
+        dep[b] = d;
+    }
+    const int64_t nnz = A->rowptr[n];
+    upload(ctx, F.rp, A->rowptr, (size_t)n + 1);
+    upload(ctx, F.ci, A->colind, (size_t)nnz);
+    upload(ctx, F.v, reinterpret_cast<const zd*>(A->values), (size_t)nnz);
+    upload(ctx, F.dep, dep.data(), (size_t)nb);
 }
 
 template <typename F>
@@ -936,6 +1071,7 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
         ctx->n = n;
+        ctx->sparse = false;
         ctx->have_q = false;   // Schur vectors and kept singular vectors belong to the old T
         ctx->nkeep = 0;
         ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
@@ -954,6 +1090,79 @@ int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_contr
         if (!Z || nZ < 0 || !control || !sigma_out) throw RlsError{RLS_E_ARG, "rls_tri_solve: null argument"};
         HIP_TRY(hipSetDevice(ctx->device));
         return solve_tri(ctx, reinterpret_cast<const cplx*>(Z), nZ, *control, sigma_out, stats);
+    });
+}
+
+int rls_set_csr(rls_ctx* ctx, const rls_csr* C) {
+    return guarded(ctx, [&]() -> int {
+        if (!C || C->n <= 0) throw RlsError{RLS_E_ARG, "rls_set_csr: bad matrix"};
+        const int64_t n = C->n;
+        check_csr(C, n, "rls_set_csr");
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
+        // C^H in CSR: the CSR transpose with conjugated values
+        const int64_t nnz = C->rowptr[n];
+        std::vector<int64_t> hp(n + 1, 0), hc(nnz);
+        std::vector<cplx> hvv(nnz);
+        const cplx* cvals = reinterpret_cast<const cplx*>(C->values);
+        for (int64_t e = 0; e < nnz; ++e) hp[C->colind[e] + 1]++;
+        for (int64_t i = 0; i < n; ++i) hp[i + 1] += hp[i];
+        std::vector<int64_t> fill(hp.begin(), hp.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t e = C->rowptr[i]; e < C->rowptr[i + 1]; ++e) {
+                const int64_t q = fill[C->colind[e]]++;
+                hc[q] = i;
+                hvv[q] = std::conj(cvals[e]);
+            }
+        ctx->n = n;
+        ctx->T.release();
+        ctx->sparse = true;
+        ctx->have_q = false;
+        ctx->nkeep = 0;
+        ctx->tri_flags.release();
+        ctx->tri_ctr.release();
+        ctx->sp_cnt.release();
+        upload(ctx, ctx->crp, C->rowptr, (size_t)n + 1);
+        upload(ctx, ctx->cci, C->colind, (size_t)nnz);
+        upload(ctx, ctx->cv, reinterpret_cast<const zd*>(C->values), (size_t)nnz);
+        upload(ctx, ctx->hrp, hp.data(), (size_t)n + 1);
+        upload(ctx, ctx->hci, hc.data(), (size_t)nnz);
+        upload(ctx, ctx->hv, reinterpret_cast<const zd*>(hvv.data()), (size_t)nnz);
+        ctx->sync();
+        return RLS_OK;
+    });
+}
+
+int rls_set_factor_callback(rls_ctx* ctx, rls_factor_fn fn, void* user) {
+    return guarded(ctx, [&]() -> int {
+        ctx->factor_cb = fn;
+        ctx->factor_user = user;
+        return RLS_OK;
+    });
+}
+
+int rls_set_sparse_factor(rls_ctx* ctx, const rls_csr* L, const rls_csr* U, const rls_csr* LH, const rls_csr* UH,
+                          const int64_t* perm_r, const int64_t* perm_c) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "rls_set_sparse_factor: no sparse operator set"};
+        const int64_t n = ctx->n;
+        if (!perm_r || !perm_c) throw RlsError{RLS_E_ARG, "rls_set_sparse_factor: null permutation"};
+        std::vector<char> seen_r(n, 0), seen_c(n, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            if (perm_r[i] < 0 || perm_r[i] >= n || seen_r[perm_r[i]]++ || perm_c[i] < 0 || perm_c[i] >= n ||
+                seen_c[perm_c[i]]++)
+                throw RlsError{RLS_E_ARG, "rls_set_sparse_factor: perm_r / perm_c is not a permutation"};
+        }
+        HIP_TRY(hipSetDevice(ctx->device));
+        upload_tri(ctx, ctx->fL, L, true, "L");
+        upload_tri(ctx, ctx->fU, U, false, "U");
+        upload_tri(ctx, ctx->fLH, LH, false, "L^H");
+        upload_tri(ctx, ctx->fUH, UH, true, "U^H");
+        upload(ctx, ctx->perm_r, perm_r, (size_t)n);
+        upload(ctx, ctx->perm_c, perm_c, (size_t)n);
+        ctx->sync();
+        ctx->factor_ready = true;
+        return RLS_OK;
     });
 }
 
@@ -1227,4 +1436,29 @@ extern "C" int rls_test_trsm_time(int64_t n, const double* T, int k, int upper, 
     });
     rls_ctx_destroy(ctx);
     return rc;
+}
+
+// Sparse-path test hooks on a context with a CSR operator: Y = C X or C^H X (spmm), and the
+// factor solves X <- (C - zI)^{-1} X or (C - zI)^{-H} X with the factor last uploaded.
+extern "C" int rls_test_sparse_op(rls_ctx* ctx, const double* X, int k, int which, double* Y) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "no sparse operator"};
+        const int64_t n = ctx->n;
+        DArr<zd> dX, dY;
+        dX.ensure((size_t)n * k);
+        dY.ensure((size_t)n * k);
+        ctx->h2d(dX.p, reinterpret_cast<const zd*>(X), (size_t)n * k);
+        if (which == 0) ctx->apply_M(dX.p, k, dY.p);
+        else if (which == 1) ctx->apply_MH(dX.p, k, dY.p);
+        else {
+            if (!ctx->factor_ready) throw RlsError{RLS_E_STATE, "no factor uploaded"};
+            ctx->sp_apply(dX.p, k, which == 3);
+            launch_copy2d(ctx->st, n, k, dX.p, n, dY.p, n);
+        }
+        ctx->d2h(reinterpret_cast<zd*>(Y), dY.p, (size_t)n * k);
+        ctx->check_tri_err();
+        dX.release();
+        dY.release();
+        return RLS_OK;
+    });
 }

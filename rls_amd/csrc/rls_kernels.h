@@ -109,3 +109,20 @@ void launch_grad_z(hipStream_t st, int64_t n, const zd* U, const zd* V, double* 
 void launch_grad_pattern(hipStream_t st, int64_t n, int64_t nnz, const int64_t* rows, const int64_t* cols,
                          const zd* U, const zd* V, double* out, int nb);
 void launch_grad_toeplitz(hipStream_t st, int64_t n, const zd* U, const zd* V, double* out, int nb);
+
+// ---- sparse (rls_sparse.hip) --------------------------------------------------------------
+// Y = (A - s I) X for an n x n CSR A (row pointers rp[n+1], column indices ci, values v;
+// 0-based), X and Y n x k column-major.
+void launch_csr_spmm(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, zd s,
+                     const zd* X, int64_t ldx, int k, zd* Y, int64_t ldy);
+// Y[i] = X[p[i]] (gather) or Y[p[i]] = X[i] (scatter), k columns
+void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t ldx, zd* Y,
+                         int64_t ldy, int k, bool scatter);
+// X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), k <= 8.
+// dep[b] (b = block of sptrsv_block_rows() rows): lower -> the first block any row of b
+// references, upper -> the last.  counter / flags (one per block) start at zero; epoch grows
+// by one per solve; *err is raised if a wait overran its bound.
+int sptrsv_block_rows();
+void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
+                   bool unit_diag, const int* dep, zd* X, int64_t ldx, int k, unsigned int* counter,
+                   unsigned int* flags, unsigned int epoch, int* err);

@@ -1,0 +1,192 @@
+// Sparse kernels of the sparse psTriRecy path (M = C, S = I):
+//   * shifted CSR times a dense block, Y = (A - sI) X, used for C and C^H (the host passes the
+//     CSR of C^H), as in RLSMatrices' ShiftedMat products (RLSMatrices.jl:24-76) feeding the
+//     recycle-space caches (recySpace.jl:368-389);
+//   * a sync-free triangular solve with a sparse triangular CSR factor (the R^{-1} / R^{-H}
+//     applications of the shifted factor, psTriRecy.jl:222-234, 513-530, here with the
+//     factors of a sparse LU computed by the host).
+#include "rls_common.h"
+#include "rls_kernels.h"
+
+namespace {
+
+// Y[:, c] = A X[:, c] - s X[:, c]; one wave per row, lanes split the row's entries,
+// columns c < k <= 16 accumulated per lane and reduced across the wave.
+template <int KC>
+__global__ void __launch_bounds__(256)
+k_csr_spmm(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
+           const zd* __restrict__ v, zd s, const zd* __restrict__ X, int64_t ldx, int k,
+           zd* __restrict__ Y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    zd acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[c] = zzero();
+    const int64_t e0 = rp[row], e1 = rp[row + 1];
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
+        const zd a = v[e];
+        const int64_t j = ci[e];
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            if (c < k) acc[c] = zfma(a, X[j + ldx * c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        double re = acc[c].x, im = acc[c].y;
+        for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o);
+            im += __shfl_xor(im, o);
+        }
+        acc[c] = zmk(re, im);
+    }
+    if (lane < k) {
+        zd out = zzero();
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            if (c == lane) out = acc[c];
+        Y[row + ldy * lane] = zsub(out, zmul(s, X[row + ldx * lane]));
+    }
+}
+
+// Row permutation: Y[i, c] = X[p[i], c]  (gather) or Y[p[i], c] = X[i, c] (scatter)
+__global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const zd* __restrict__ X,
+                               int64_t ldx, zd* __restrict__ Y, int64_t ldy, int scatter) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n) return;
+    if (scatter) Y[p[i] + ldy * c] = X[i + ldx * c];
+    else Y[i + ldy * c] = X[p[i] + ldx * c];
+}
+
+// Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
+// ascending order, upper: descending).  One workgroup per block of SP_RB rows, started in
+// dependency order by a ticket counter.  A block first waits for every earlier block its
+// rows reference (host analysis: dep_lo[b] for lower / dep_hi[b] for upper), then solves
+// its rows in order, one wave per right-hand-side pair, lanes splitting each row's entries.
+// The wait is bounded: on overrun *err is raised and the workgroup exits.
+constexpr int SP_RB = 32;
+
+template <bool LOWER>
+__global__ void __launch_bounds__(256)
+k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
+         const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t ldx,
+         int k, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
+    __shared__ unsigned int s_ticket;
+    __shared__ int s_ok;
+    __shared__ zd xs[SP_RB][8];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int nblk = (int)((n + SP_RB - 1) / SP_RB);
+    if (tid == 0) { s_ticket = atomicAdd(counter, 1u); s_ok = 1; }
+    __syncthreads();
+    const int t = (int)(s_ticket % (unsigned)nblk);
+    const int b = LOWER ? t : nblk - 1 - t;
+    const int64_t r0 = (int64_t)b * SP_RB;
+    const int rows = (int)min((int64_t)SP_RB, n - r0);
+    // wait for the earlier blocks this block reads
+    if (tid == 0) {
+        const int d = dep[b];
+        const int lo = LOWER ? d : b + 1, hi = LOWER ? b : d + 1;
+        for (int q = lo; q < hi && s_ok; ++q) {
+            unsigned int spins = 0;
+            while (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 24)) { s_ok = 0; atomicExch(err, 1); break; }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    // wave w handles right-hand sides c = 2w, 2w+1 (k <= 8)
+    const int c0 = 2 * w;
+    for (int rr = 0; rr < rows; ++rr) {
+        const int li = LOWER ? rr : rows - 1 - rr;
+        const int64_t i = r0 + li;
+        const int64_t e0 = rp[i], e1 = rp[i + 1];
+        zd s0 = zzero(), s1 = zzero(), dg = zmk(1.0, 0.0);
+        for (int64_t e = e0 + lane; e < e1; e += 64) {
+            const int64_t j = ci[e];
+            const zd a = v[e];
+            if (j == i) { dg = a; continue; }
+            zd x0, x1;
+            if (j >= r0 && j < r0 + rows) {
+                x0 = (c0 < k) ? xs[j - r0][c0] : zzero();
+                x1 = (c0 + 1 < k) ? xs[j - r0][c0 + 1] : zzero();
+            } else {
+                x0 = (c0 < k) ? X[j + ldx * c0] : zzero();
+                x1 = (c0 + 1 < k) ? X[j + ldx * (c0 + 1)] : zzero();
+            }
+            s0 = zfma(a, x0, s0);
+            s1 = zfma(a, x1, s1);
+        }
+        double a0 = s0.x, a1 = s0.y, a2 = s1.x, a3 = s1.y, dr = dg.x, di = dg.y;
+        for (int o = 32; o > 0; o >>= 1) {
+            a0 += __shfl_xor(a0, o); a1 += __shfl_xor(a1, o);
+            a2 += __shfl_xor(a2, o); a3 += __shfl_xor(a3, o);
+        }
+        // the diagonal entry sits in exactly one lane (or none for a unit diagonal)
+        int hasd = !(dr == 1.0 && di == 0.0);
+        for (int o = 32; o > 0; o >>= 1) {
+            const double pr = __shfl_xor(dr, o), pi = __shfl_xor(di, o);
+            const int ph = __shfl_xor(hasd, o);
+            if (!hasd && ph) { dr = pr; di = pi; hasd = 1; }
+        }
+        const zd diag = unit_diag ? zmk(1.0, 0.0) : zmk(dr, di);
+        if (lane == 0) {
+            if (c0 < k) xs[li][c0] = zdiv(zsub(X[i + ldx * c0], zmk(a0, a1)), diag);
+            if (c0 + 1 < k) xs[li][c0 + 1] = zdiv(zsub(X[i + ldx * (c0 + 1)], zmk(a2, a3)), diag);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < rows * 8; idx += 256) {
+        const int li = idx % rows, c = idx / rows;
+        if (c < k) X[(r0 + li) + ldx * c] = xs[li][c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace
+
+void launch_csr_spmm(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, zd s,
+                     const zd* X, int64_t ldx, int k, zd* Y, int64_t ldy) {
+    for (int c0 = 0; c0 < k; c0 += 16) {
+        const int kc = std::min(16, k - c0);
+        if (kc <= 8)
+            hipLaunchKernelGGL(k_csr_spmm<8>, dim3(cdiv(n, 4)), dim3(256), 0, st, n, rp, ci, v, s, X + ldx * c0,
+                               ldx, kc, Y + ldy * c0, ldy);
+        else
+            hipLaunchKernelGGL(k_csr_spmm<16>, dim3(cdiv(n, 4)), dim3(256), 0, st, n, rp, ci, v, s, X + ldx * c0,
+                               ldx, kc, Y + ldy * c0, ldy);
+        LAUNCH_CHECK();
+    }
+}
+
+void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t ldx, zd* Y,
+                         int64_t ldy, int k, bool scatter) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_permute_rows, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, p, X, ldx, Y, ldy,
+                       scatter ? 1 : 0);
+    LAUNCH_CHECK();
+}
+
+int sptrsv_block_rows() { return SP_RB; }
+
+void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
+                   bool unit_diag, const int* dep, zd* X, int64_t ldx, int k, unsigned int* counter,
+                   unsigned int* flags, unsigned int epoch, int* err) {
+    if (k > 8) throw RlsError{-1, "sptrsv: k > 8 unsupported"};
+    const int nblk = (int)cdiv(n, SP_RB);
+    if (lower)
+        hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(256), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
+                           ldx, k, counter, flags, epoch, err);
+    else
+        hipLaunchKernelGGL(k_sptrsv<false>, dim3(nblk), dim3(256), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
+                           ldx, k, counter, flags, epoch, err);
+    LAUNCH_CHECK();
+}

@@ -137,9 +137,11 @@ def test_errors_are_reported():
         solver.solve(rec.Z, R.PsTriRecyControl(recycle_fail_dist_thresh=-1.0), rec)
 
 
-def test_sparse_input_through_the_schur_path():
-    """Sparse C (the psTriRecy sparse overload, psTriRecy.jl:47-61) on a sprand-shaped matrix:
-    sigma_min within 1e-4 of sparse-LU ground truth at every point."""
+def test_sparse_path_matches_truth_and_oracle():
+    """Sparse C (the psTriRecy sparse overload, psTriRecy.jl:47-61): CSR products on the
+    device and a host LU per refine shift whose triangular solves run on the device.  The
+    oracle's sparse problem (M = C, S = I, SuperLU refine) is the same operator, so with the
+    same start block both follow one trajectory."""
     C = WL.sprand(n=500, density=4e-3, norm2=3.2)
     _say(f"sparse sprand n=500 nnz={C.nnz}")
     rec = R.recursion_grid(0.45, 0.55, 0.0, 0.1, 6, 6, 3)
@@ -148,4 +150,38 @@ def test_sparse_input_through_the_schur_path():
     tr = truth.sparse_sigma_min(C, rec.Z)
     rel = np.abs(sig - tr) / tr
     assert rel.max() <= 1e-4, rel.max()
-    assert st.preprocess_time_ns > 0
+    assert st.qr_time_ns > 0 and st.n_pts > 0
+    octl = O.PsTriRecyControl(tol=1e-4, normC=3.2, p=20, recy_tau=1.6)
+    osig, ost, _ = O.ps_tri_recy(O.sparse_tri_problem(C), O.recursion_grid(0.45, 0.55, 0.0, 0.1, 6, 6, 3),
+                                 octl, X_init=R.initial_block(500, 6, 0))
+    assert np.max(np.abs(sig - osig) / osig) <= 1e-6
+    assert int(((st.lobpcg_it > 0) != ost.refined).sum()) <= 1
+
+
+def test_sparse_triangular_solves_match_scipy():
+    """The device LU solves (A^{-1} and A^{-H}) against scipy on the same factor."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    C = WL.sprand(n=700, density=5e-3, norm2=3.2)
+    _say("sparse LU solves n=700")
+    s = R.SparseTriSolver(C)
+    z = 0.5 + 0.05j
+    assert s._factor(z) == 0
+    A = (C - z * sp.identity(700, format="csc")).tocsc()
+    rng = np.random.default_rng(4)
+    X = np.asfortranarray(rng.standard_normal((700, 6)) + 1j * rng.standard_normal((700, 6)))
+    got = s.apply_factor_solve(X, adjoint=False)
+    np.testing.assert_allclose(got, spla.spsolve(A, X), rtol=0, atol=1e-9 * np.abs(got).max())
+    got_h = s.apply_factor_solve(X, adjoint=True)
+    np.testing.assert_allclose(got_h, spla.spsolve(A.conj().T.tocsc(), X), rtol=0, atol=1e-9 * np.abs(got_h).max())
+
+
+def test_csr_products_match_scipy():
+    C = WL.sprand(n=900, density=6e-3, norm2=3.2)
+    _say("csr products n=900")
+    s = R.SparseTriSolver(C)
+    rng = np.random.default_rng(5)
+    for k in (1, 6, 12, 18):
+        X = rng.standard_normal((900, k)) + 1j * rng.standard_normal((900, k))
+        np.testing.assert_allclose(s.apply_operator(X), C @ X, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(s.apply_operator(X, adjoint=True), C.conj().T @ X, rtol=0, atol=1e-12)
