@@ -39,6 +39,10 @@ def build_workload(name: str):
         n = 4000
         C = WL.sprand(n=n, density=1e-3, norm2=3.2)
         normC = 3.2
+    elif name == "cfd":
+        n = 23560
+        C = WL.banded_cfd(n=n)
+        normC = 645.74
     else:
         raise SystemExit(f"unknown workload {name}")
     g = WL.GRIDS[name]
@@ -50,7 +54,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="laser", choices=["laser", "toeplitz", "sprand"])
+    ap.add_argument("--workload", default="laser", choices=["laser", "toeplitz", "sprand", "cfd"])
+    ap.add_argument("--window", type=int, default=0, help="max shifts per speculative window (0 = auto)")
+    ap.add_argument("--max-points", type=int, default=0,
+                    help="solve only the first N points of the traversal (0 = the whole grid)")
     ap.add_argument("--plain-points", type=int, default=4000,
                     help="leading traversal points timed with the plain FP64 solver")
     ap.add_argument("--truth-samples", type=int, default=24)
@@ -83,15 +90,28 @@ def main():
     C, normC, g, n = build_workload(args.workload)
     x_st, x_ed, y_st, y_ed, nx, ny, k = g
     rec = R.recursion_grid(x_st, x_ed, y_st, y_ed, nx, ny, k)
+    if args.max_points > 0:
+        rec.Z = rec.Z[: args.max_points]
+    import scipy.sparse as sp
+    # sparse configurations: the factor path for large n (the reference's structure), the
+    # Schur factor of the densified matrix for n <= 8000 (cheaper than per-shift sparse LUs)
+    sparse = sp.issparse(C) and n > 8000
+    if sp.issparse(C) and not sparse:
+        C = C.toarray()
     t0 = time.perf_counter()
-    T, Q = R.schur_factor(C)
+    if sparse:
+        T = Q = None
+        solver = R.SparseTriSolver(C)
+    else:
+        T, Q = R.schur_factor(C)
+        solver = R.TriSolver(T)
     t_pre = time.perf_counter() - t0
-    log(f"schur n={n} {t_pre:.1f}s, normC={normC:.6g}")
-    solver = R.TriSolver(T)
-    if args.adjoint:
+    log(f"{'csr setup' if sparse else 'schur'} n={n} {t_pre:.1f}s, normC={normC:.6g}")
+    if args.adjoint and not sparse:
         solver.set_schur_vectors(Q)
         solver.keep_vectors(True)
-    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched", progress=1)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched", progress=1,
+                             window=args.window)
     nZ = len(rec.Z)
 
     for w in range(args.warmup):
@@ -171,7 +191,9 @@ def main():
     idx = rng.choice(nZ, size=min(args.truth_samples, nZ), replace=False)
     t0 = time.perf_counter()
     rel_any = None
-    if n <= 2000:
+    if sparse:
+        tr = truth.sparse_sigma_min(C, rec.Z[idx])
+    elif n <= 2000:
         import scipy.linalg as sla
         Cd = np.asarray(C.toarray() if hasattr(C, "toarray") else C)
         spectra = [sla.svdvals(Cd - z * np.eye(n)) for z in rec.Z[idx]]
@@ -202,7 +224,8 @@ def main():
             if time.perf_counter() - t_start > args.cpu_seconds:
                 raise Stop()
         try:
-            O.ps_tri_recy(O.dense_tri_problem(T), orec, octl, on_point=cb)
+            O.ps_tri_recy(O.sparse_tri_problem(C) if sparse else O.dense_tri_problem(T), orec, octl,
+                          on_point=cb)
         except Stop:
             pass
         t_cpu = time.perf_counter() - t_start
@@ -224,8 +247,10 @@ def main():
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} dense n={n}, grid {nx}x{ny} (k={k}), psTriRecy "
-                               f"tol=1e-4 p=60 r=6 recy_tau=1.6, normC={normC:.6g}",
+        "config": {"workload": f"{args.workload} {'sparse nnz=%d (LU per refine)' % C.nnz if sparse else 'dense (Schur)'} n={n}, "
+                               f"grid {nx}x{ny} (k={k}), psTriRecy tol=1e-4 p=60 r=6 recy_tau=1.6, "
+                               f"normC={normC:.6g}"
+                               + (f", first {nZ} traversal points" if args.max_points > 0 else ""),
                    "grid_points": nZ},
         "max_rel_err": float(rel.max()),
         "max_rel_err_nearest_singular_value": None if rel_any is None else float(rel_any.max()),
@@ -235,6 +260,7 @@ def main():
         "refined_points": int((st.lobpcg_it > 0).sum()),
         "windows": int(st.n_windows),
         "preprocess_s": t_pre,
+        "refine_factor_s": getattr(solver, "factor_time_s", None),
         "cpu_baseline": cpu,
         "roofline": roof,
         "kernel_families": kt,

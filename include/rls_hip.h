@@ -52,7 +52,7 @@ typedef struct rls_tri_control {
     double rec_dx, rec_dy;
     /* --- MI355X execution knobs (no counterpart in the reference) --- */
     int32_t mode;                     /* RLS_MODE_PLAIN_FP64 or RLS_MODE_BATCHED */
-    int32_t window;                   /* max shifts per speculative window (batched mode), 0 = auto */
+    int32_t window;                   /* max shifts per speculative window (batched mode), 0 = 256 */
     uint64_t seed;                    /* seed of the initial block (reference: rand!(X)) */
     int32_t verbose;                  /* >= 1: progress lines on stderr every ~5% of the grid */
 } rls_tri_control;

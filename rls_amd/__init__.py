@@ -127,7 +127,10 @@ class PsTriRecyControl:
 
     Execution knobs of the GPU build (no reference counterpart): ``mode`` ("plain" = one
     shift at a time, "batched" = speculative windows over one recycle snapshot),
-    ``window`` (max shifts per window, 0 = auto), ``seed`` (initial block), ``maxit``."""
+    ``window`` (max shifts per window, 0 = auto), ``seed`` (initial block), ``maxit``,
+    ``sparse_path`` for sparse C: "factor" keeps C sparse and factors C - zI at each refine
+    shift (the reference's structure), "schur" densifies and uses the Schur factor (cheaper
+    when the shifted LU fills in heavily and n is moderate), "auto" = "schur" for n <= 8000."""
     tol: float = 1e-3
     normC: float = 1.0
     shift: Optional[float] = None
@@ -149,6 +152,7 @@ class PsTriRecyControl:
     seed: int = 0
     maxit: int = 2000
     progress: int = 0
+    sparse_path: str = "auto"
 
     def __post_init__(self):
         if self.shift is None:
@@ -157,6 +161,8 @@ class PsTriRecyControl:
             raise ValueError("lobpcg_side must be 'L' or 'S'")
         if self.mode not in ("plain", "batched"):
             raise ValueError("mode must be 'plain' or 'batched'")
+        if self.sparse_path not in ("auto", "schur", "factor"):
+            raise ValueError("sparse_path must be 'auto', 'schur' or 'factor'")
 
 
 @dataclass
@@ -485,7 +491,9 @@ def psTriRecy(C, rec: Recursion, control: PsTriRecyControl = None, verbose: int 
         _print_startup(io, rec, control)
     import scipy.sparse as sp
     t0 = time.perf_counter_ns()
-    if sp.issparse(C):
+    use_factor = sp.issparse(C) and (control.sparse_path == "factor" or
+                                     (control.sparse_path == "auto" and C.shape[0] > 8000))
+    if use_factor:
         # sparse overload (psTriRecy.jl:47-61): C stays sparse, refine factors per shift
         solver = SparseTriSolver(C, device)
     else:

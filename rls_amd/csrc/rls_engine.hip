@@ -767,11 +767,11 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     int64_t n_it = 0, n_pts = 0, n_windows = 0, n_wshifts = 0;
     uint64_t t_rec = 0, t_ref = 0, t_upd = 0;
     const bool batched = (c.mode == RLS_MODE_BATCHED);
-    // Window sizing: the eigensolves of a window run in parallel (one workgroup per shift),
-    // so a window costs about one eigensolve latency plus products linear in its width.
-    // Width = 1.5x the recent run length between refinements (EMA), clamped to [16, max];
-    // a window that is accepted completely doubles the next one.
-    const int window = c.window > 0 ? c.window : 128;
+    // Window sizing: the eigensolves of a window run in parallel (one workgroup per shift, up
+    // to one per CU), so a window costs about one eigensolve latency plus products linear in
+    // its width.  Width = 2x the recent run length between refinements (EMA) + 16, clamped to
+    // [16, max]; a window that is accepted completely doubles the next one.
+    const int window = c.window > 0 ? c.window : 256;
     double avg_run = 32.0;
     int run_hint = 32;
     int64_t next_report = std::max<int64_t>(nZ / 20, 1);
@@ -818,7 +818,7 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
                     run_hint = std::min(window, 2 * B);
                 } else {
                     avg_run = 0.7 * avg_run + 0.3 * (double)(f + 1);
-                    run_hint = std::max(16, std::min(window, (int)(1.5 * avg_run) + 8));
+                    run_hint = std::max(16, std::min(window, (int)(2.0 * avg_run) + 16));
                 }
             }
             if (f == B) { i += B; continue; }

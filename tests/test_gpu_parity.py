@@ -145,12 +145,17 @@ def test_sparse_path_matches_truth_and_oracle():
     C = WL.sprand(n=500, density=4e-3, norm2=3.2)
     _say(f"sparse sprand n=500 nnz={C.nnz}")
     rec = R.recursion_grid(0.45, 0.55, 0.0, 0.1, 6, 6, 3)
-    ctl = R.PsTriRecyControl(tol=1e-4, normC=3.2, p=20, recy_tau=1.6, mode="batched")
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=3.2, p=20, recy_tau=1.6, mode="batched",
+                             sparse_path="factor")
     sig, st = R.psTriRecy(C, rec, control=ctl, verbose=0)
     tr = truth.sparse_sigma_min(C, rec.Z)
     rel = np.abs(sig - tr) / tr
     assert rel.max() <= 1e-4, rel.max()
     assert st.qr_time_ns > 0 and st.n_pts > 0
+    # the Schur path on the same sparse input agrees with ground truth as well
+    sig2, _ = R.psTriRecy(C, rec, control=R.PsTriRecyControl(tol=1e-4, normC=3.2, p=20, recy_tau=1.6,
+                                                               sparse_path="schur"), verbose=0)
+    assert np.max(np.abs(sig2 - tr) / tr) <= 1e-4
     octl = O.PsTriRecyControl(tol=1e-4, normC=3.2, p=20, recy_tau=1.6)
     osig, ost, _ = O.ps_tri_recy(O.sparse_tri_problem(C), O.recursion_grid(0.45, 0.55, 0.0, 0.1, 6, 6, 3),
                                  octl, X_init=R.initial_block(500, 6, 0))
