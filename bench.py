@@ -146,6 +146,14 @@ def main():
         roof = {"kernel": "k_zgemm_tiled" if dom == "gemm" else "k_heev_rr/k_heev_wave",
                 "bound": "mfma", "achieved": ach, "peak": 78.6, "unit": "TFLOP/s",
                 "frac": ach / 78.6, "traffic": None}
+    # HBM bytes per launch of the dominant family, from the FETCH_SIZE / WRITE_SIZE counter
+    # runs committed for this workload (profiles/traffic_<workload>.json), when available
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}{n}.json")
+    if dom == "heev" and os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr_ = json.load(fh)
+        roof["traffic"] = tr_["eigensolver_family_bytes_per_launch"]
+        roof["traffic_GBps"] = roof["traffic"] / avg_s / 1e9
     roof["launches"] = d["launches"]
     roof["avg_launch_us"] = 1e6 * avg_s
     roof["share_of_step"] = d["ms"] / 1e3 / (total / len(times)) / len(times)
