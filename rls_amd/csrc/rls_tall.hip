@@ -634,11 +634,14 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 // ---------------------------------------------------------------------------------------
 namespace {
 constexpr int PT_KC = 8;
-__global__ void __launch_bounds__(256)
+constexpr int PT_THREADS = 512;   // 8 waves: two per SIMD, so LDS and FMA latency overlap
+__global__ void __launch_bounds__(PT_THREADS)
 k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
                   zd* X, int64_t ldx, int k, int upper, unsigned int* counter, unsigned int* flags,
                   unsigned int epoch, int* err) {
-    __shared__ zd Ts[64][65];
+    constexpr int NW = PT_THREADS / 64;
+    constexpr int PER = 4096 / PT_THREADS;      // T / Dinv block elements per thread
+    __shared__ zd Ts[64][65];                   // also the waves' partial sums at the end
     __shared__ zd Ds[64][65];
     __shared__ zd xs[64][PT_KC];
     __shared__ zd bj[64][PT_KC];
@@ -656,16 +659,16 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     const int j = upper ? nblk - 1 - t : t;
     const int64_t o = (int64_t)j * 64;
     const int m = (int)min((int64_t)64, n - o);
-    // T block of dependency kb into registers: 16 elements per thread, all loads issued
-    // before any is consumed.  upper: Ts[a][c] = T[o + a, ko + c] (lanes over a);
-    // lower: Ts[a][c] = conj(T[ko + c, o + a]) (lanes over c, the contiguous index).
-    zd pre[16];
+    // T block of dependency kb into registers, all loads issued before any is consumed.
+    // upper: Ts[a][c] = T[o + a, ko + c] (lanes over a); lower: Ts[a][c] = conj(T[ko + c, o + a])
+    // (lanes over c, the contiguous index).
+    zd pre[PER];
     auto load_T = [&](int kb) {
         const int64_t ko = (int64_t)kb * 64;
         const int mk = (int)min((int64_t)64, n - ko);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = tid + 256 * u;
+        for (int u = 0; u < PER; ++u) {
+            const int idx = tid + PT_THREADS * u;
             const int i0 = idx & 63, i1 = idx >> 6;
             if (upper) pre[u] = (i0 < m && i1 < mk) ? T[(o + i0) + ldt * (ko + i1)] : zzero();
             else pre[u] = (i1 < m && i0 < mk) ? zconj(T[(ko + i0) + ldt * (o + i1)]) : zzero();
@@ -673,8 +676,8 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     };
     auto store_T = [&]() {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = tid + 256 * u;
+        for (int u = 0; u < PER; ++u) {
+            const int idx = tid + PT_THREADS * u;
             const int i0 = idx & 63, i1 = idx >> 6;
             if (upper) Ts[i0][i1] = pre[u];
             else Ts[i1][i0] = pre[u];
@@ -685,33 +688,24 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     auto dep = [&](int q) { return upper ? nblk - 1 - q : q; };
     if (ndep > 0) load_T(dep(0));
     // diagonal-block inverse (upper: Dinv_j, lower: Dinv_j^H) and right-hand side
+    const int a_out = tid & 63, c_out = tid >> 6;      // this thread's output element
     {
         const zd* Di = Dinv + (int64_t)j * 4096;
-        zd dv[16];
+        zd dv[PER];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = tid + 256 * u;
+        for (int u = 0; u < PER; ++u) {
+            const int idx = tid + PT_THREADS * u;
             const int a = idx & 63, c = idx >> 6;
             dv[u] = upper ? Di[a + 64 * c] : Di[c + 64 * a];
         }
-        zd bv[PT_KC * 64 / 256];
+        const zd bv = (a_out < m && c_out < k) ? X[(o + a_out) + ldx * c_out] : zzero();
 #pragma unroll
-        for (int u = 0; u < PT_KC * 64 / 256; ++u) {
-            const int idx = tid + 256 * u;
-            const int a = idx & 63, c = idx >> 6;
-            bv[u] = (a < m && c < k) ? X[(o + a) + ldx * c] : zzero();
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = tid + 256 * u;
+        for (int u = 0; u < PER; ++u) {
+            const int idx = tid + PT_THREADS * u;
             const int a = idx & 63, c = idx >> 6;
             Ds[a][c] = upper ? dv[u] : zconj(dv[u]);
         }
-#pragma unroll
-        for (int u = 0; u < PT_KC * 64 / 256; ++u) {
-            const int idx = tid + 256 * u;
-            bj[idx & 63][idx >> 6] = bv[u];
-        }
+        bj[a_out][c_out] = bv;
     }
     zd acc[PT_KC];
 #pragma unroll
@@ -733,56 +727,40 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         }
         __syncthreads();
         if (s_bail) break;
-        {
-            zd xv[PT_KC * 64 / 256];
-#pragma unroll
-            for (int u = 0; u < PT_KC * 64 / 256; ++u) {
-                const int idx = tid + 256 * u;
-                const int a = idx & 63, c = idx >> 6;
-                xv[u] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
-            }
-#pragma unroll
-            for (int u = 0; u < PT_KC * 64 / 256; ++u) {
-                const int idx = tid + 256 * u;
-                xs[idx & 63][idx >> 6] = xv[u];
-            }
-        }
+        xs[a_out][c_out] = (a_out < mk && c_out < k) ? X[(ko + a_out) + ldx * c_out] : zzero();
         __syncthreads();
-        // acc[lane][c] += sum over this wave's quarter of the block columns
-        for (int cc = w; cc < 64; cc += 4) {
+        // acc[lane][c] += sum over this wave's eighth of the block columns
+#pragma unroll
+        for (int tt = 0; tt < 64 / NW; ++tt) {
+            const int cc = w + NW * tt;
             const zd tv = Ts[lane][cc];
 #pragma unroll
             for (int c = 0; c < PT_KC; ++c) acc[c] = zfma(tv, xs[cc][c], acc[c]);
         }
     }
-    // reduce the 4 waves' partial sums into bj (bj -= acc)
+    // the waves' partial sums through LDS (Ts is free now), one output element per thread
     __syncthreads();
-    for (int v = 0; v < 4; ++v) {
-        if (w == v) {
+    zd* part = &Ts[0][0];                       // NW x 64 x PT_KC <= 64 x 65
 #pragma unroll
-            for (int c = 0; c < PT_KC; ++c) bj[lane][c] = zsub(bj[lane][c], acc[c]);
-        }
-        __syncthreads();
-    }
-    // x_j = Ds * bj: thread (row a, column pair), two independent accumulation chains
-    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
-        const int a = idx & 63, c = idx >> 6;
-        zd s0 = zzero(), s1 = zzero();
-        if (a < m && c < k) {
-            int q = 0;
-            for (; q + 1 < m; q += 2) {
-                s0 = zfma(Ds[a][q], bj[q][c], s0);
-                s1 = zfma(Ds[a][q + 1], bj[q + 1][c], s1);
-            }
-            if (q < m) s0 = zfma(Ds[a][q], bj[q][c], s0);
-        }
-        xs[a][c] = zadd(s0, s1);
+    for (int c = 0; c < PT_KC; ++c) part[(w * 64 + lane) * PT_KC + c] = acc[c];
+    __syncthreads();
+    {
+        zd sacc = zzero();
+#pragma unroll
+        for (int v = 0; v < NW; ++v) sacc = zadd(sacc, part[(v * 64 + a_out) * PT_KC + c_out]);
+        bj[a_out][c_out] = zsub(bj[a_out][c_out], sacc);
     }
     __syncthreads();
-    for (int idx = tid; idx < 64 * PT_KC; idx += 256) {
-        const int a = idx & 63, c = idx >> 6;
-        if (a < m && c < k) X[(o + a) + ldx * c] = xs[a][c];
+    // x_j = Ds * bj, one output per thread, two accumulation chains
+    zd s0 = zzero(), s1 = zzero();
+    if (c_out < k) {
+#pragma unroll 8
+        for (int qq = 0; qq < 64; qq += 2) {
+            s0 = zfma(Ds[a_out][qq], bj[qq][c_out], s0);
+            s1 = zfma(Ds[a_out][qq + 1], bj[qq + 1][c_out], s1);
+        }
     }
+    if (a_out < m && c_out < k) X[(o + a_out) + ldx * c_out] = zadd(s0, s1);
     __syncthreads();
     if (tid == 0) {
         __threadfence();
@@ -796,7 +774,7 @@ void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt,
                             unsigned int epoch, int* err) {
     if (k > PT_KC) throw RlsError{-1, "trsm: k > 8 unsupported"};
     const int nblk = cdiv(n, 64);
-    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(256), 0, st, n, T, ldt, Dinv, X, ldx, k,
+    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, T, ldt, Dinv, X, ldx, k,
                        upper ? 1 : 0, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
