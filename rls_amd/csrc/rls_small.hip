@@ -456,58 +456,71 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     }
 
     if (b == 0 && tid == 0) g_heev_phase[3] = wall_clock64();
-    // 6. back-transformation G = H(0) H(1) ... H(n-2) y, y <- H(k) y for k descending.
-    //    Dots of all vectors with v_k in one pass: thread (j, slice) partial sums.
-    zd* Gb = G + (int64_t)b * sG;
-    for (int64_t idx = tid; idx < (int64_t)n * nv; idx += nthr) {
-        const int i = (int)(idx % n), j = (int)(idx / n);
-        Gb[i + ldg * j] = zmk(Y[(int64_t)j * n + i], 0.0);
-    }
-    __syncthreads();
-    for (int j0 = 0; j0 < nv; j0 += nthr) {
-        const int nvc = min(nv - j0, nthr);
-        const int S = nthr / nvc;                 // row slices per vector
-        const int jj = tid % nvc, sl = tid / nvc;
-        for (int k = n - 2; k >= 0; --k) {
-            const zd t = tau[k];
-            if (t.x == 0.0 && t.y == 0.0) continue;
-            const int m = n - k - 1;
-            const zd* x = M + (k + 1) + (int64_t)n * k;   // v[0] = 1 implicit, v[1:] stored
-            for (int i = tid; i < m; i += nthr) sv[i] = (i == 0) ? zmk(1.0, 0.0) : x[i];
-            __syncthreads();
-            double pr = 0.0, pi = 0.0;
-            if (sl < S) {
-                const zd* g = Gb + (k + 1) + ldg * (j0 + jj);
-                for (int i = sl; i < m; i += S) {
-                    zd c2 = zcmul(sv[i], g[i]);
-                    pr += c2.x; pi += c2.y;
-                }
-            }
-            s_part[tid] = pr;
-            __syncthreads();
-            if (tid < nvc) {
-                double sr = 0.0;
-                for (int q = 0; q < S; ++q) sr += s_part[q * nvc + tid];
-                sw[tid].x = sr;
-            }
-            __syncthreads();
-            s_part[tid] = pi;
-            __syncthreads();
-            if (tid < nvc) {
-                double si = 0.0;
-                for (int q = 0; q < S; ++q) si += s_part[q * nvc + tid];
-                sw[tid] = zmul(t, zmk(sw[tid].x, si));   // H = I - tau v v^H
-            }
-            __syncthreads();
-            for (int idx = tid; idx < m * nvc; idx += nthr) {
-                const int i = idx % m, j = idx / m;
-                zd* g = Gb + (k + 1) + ldg * (j0 + j);
-                g[i] = zsub(g[i], zmul(sv[i], sw[j]));
-            }
-            __syncthreads();
-        }
-    }
+    // 6. the back-transformation runs in k_heev_backtr (many workgroups per problem)
     if (b == 0 && tid == 0) g_heev_phase[4] = wall_clock64();
+}
+
+// Back-transformation of k_heev_rr's eigenvectors, G = H(0) H(1) ... H(n-2) Y (y <- H(k) y for
+// k descending), for vectors [blockIdx.y * HB_CH, +HB_CH) of problem blockIdx.x: many
+// workgroups per problem instead of one.  Reflectors, tau and Y come from the per-problem
+// workspace of k_heev_rr; nv = min(rr, nvec_max).
+constexpr int HB_CH = 32;
+__global__ void __launch_bounds__(256)
+k_heev_backtr(int n, const zd* __restrict__ Wk, int64_t sW, zd* __restrict__ G, int64_t ldg, int64_t sG,
+              const int* __restrict__ rr_out, int nvec_max) {
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int rr = rr_out[b];
+    const int nv = rr < nvec_max ? rr : nvec_max;
+    const int j0 = blockIdx.y * HB_CH;
+    if (j0 >= nv) return;
+    const int nvc = min(HB_CH, nv - j0);
+    const zd* M = Wk + (int64_t)b * sW;
+    const double* dws = reinterpret_cast<const double*>(M + (int64_t)n * n);
+    const zd* tau = reinterpret_cast<const zd*>(dws + 4 * (int64_t)n);
+    const double* Y = reinterpret_cast<const double*>(tau + n);
+    zd* Gb = G + (int64_t)b * sG;
+    __shared__ zd sv[1024];
+    __shared__ double s_pr[256], s_pi[256];
+    __shared__ zd sw[HB_CH];
+    for (int64_t idx = tid; idx < (int64_t)n * nvc; idx += 256) {
+        const int i = (int)(idx % n), jj = (int)(idx / n);
+        Gb[i + ldg * (j0 + jj)] = zmk(Y[(int64_t)(j0 + jj) * n + i], 0.0);
+    }
+    const int S = 256 / nvc;                    // row slices per vector
+    const int jj = tid % nvc, sl = tid / nvc;
+    __syncthreads();
+    for (int k = n - 2; k >= 0; --k) {
+        const zd t = tau[k];
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        const int m = n - k - 1;
+        const zd* x = M + (k + 1) + (int64_t)n * k;   // v[0] = 1 implicit, v[1:] stored
+        for (int i = tid; i < m; i += 256) sv[i] = (i == 0) ? zmk(1.0, 0.0) : x[i];
+        __syncthreads();
+        double pr = 0.0, pi = 0.0;
+        if (sl < S) {
+            const zd* g = Gb + (k + 1) + ldg * (j0 + jj);
+            for (int i = sl; i < m; i += S) {
+                const zd c2 = zcmul(sv[i], g[i]);
+                pr += c2.x; pi += c2.y;
+            }
+        }
+        s_pr[tid] = pr;
+        s_pi[tid] = pi;
+        __syncthreads();
+        if (tid < nvc) {
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < S; ++q) { sr += s_pr[q * nvc + tid]; si += s_pi[q * nvc + tid]; }
+            sw[tid] = zmul(t, zmk(sr, si));              // H = I - tau v v^H
+        }
+        __syncthreads();
+        for (int idx = tid; idx < m * nvc; idx += 256) {
+            const int i = idx % m, j = idx / m;
+            zd* g = Gb + (k + 1) + ldg * (j0 + j);
+            g[i] = zsub(g[i], zmul(sv[i], sw[j]));
+        }
+        __syncthreads();
+    }
 }
 
 void read_heev_phases(long long* out) {
@@ -562,6 +575,9 @@ void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n,
     int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
     hipLaunchKernelGGL(k_heev_rr, dim3(batch), dim3(HEEV_THREADS), 0, st, A, lda, sA, n, Wk, sW,
                        lam, sL, G, ldg, sG, rr, taus, r, nvec_max, mode_fixed);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_heev_backtr, dim3(batch, cdiv(nvec_max, HB_CH)), dim3(256), 0, st, n, Wk, sW, G, ldg, sG,
+                       rr, nvec_max);
     LAUNCH_CHECK();
 }
 
