@@ -42,47 +42,69 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
 
 // Solve (T - lam I) y = b in place for a real symmetric tridiagonal T by Gaussian
 // elimination with partial pivoting (LAPACK dlagtf/dlagts idea), sequential, one thread.
+// The elimination carries the active rows in registers; wk (>= 3n doubles) receives the
+// factor (diagonal a, first and second superdiagonals b, u2) for the back substitution,
+// whose loads do not depend on the recurrence and can be issued ahead.
 __device__ void tri_shifted_solve(const double* d, const double* e, int n, double lam,
                                   double* y, double* wk, double eps_floor) {
-    // wk layout: a[n] (diag), b[n] (super), c[n] (sub), u2[n] (second super after pivot), piv[n]
     double* a = wk;
     double* b = wk + n;
     double* u2 = wk + 2 * n;
-    double* pv = wk + 3 * n;
-    for (int i = 0; i < n; ++i) { a[i] = d[i] - lam; b[i] = (i < n - 1) ? e[i] : 0.0; u2[i] = 0.0; pv[i] = 0.0; }
-    // forward elimination
+    if (n == 1) {
+        double a0 = d[0] - lam;
+        if (fabs(a0) < eps_floor) a0 = eps_floor;
+        y[0] /= a0;
+        return;
+    }
+    double ai = d[0] - lam, bi = e[0];                 // row i
+    double an = d[1] - lam, bn = (n > 2) ? e[1] : 0.0; // row i + 1
+    double yi = y[0];
     for (int i = 0; i < n - 1; ++i) {
-        double sub = e[i];  // T[i+1, i]
-        if (fabs(a[i]) >= fabs(sub)) {
-            if (fabs(a[i]) < eps_floor) a[i] = eps_floor;
-            double m = sub / a[i];
-            pv[i] = 0.0;
-            a[i + 1] -= m * b[i];
-            // rhs
-            y[i + 1] -= m * y[i];
-            u2[i] = 0.0;
-            // store multiplier not needed afterwards
+        const double sub = e[i];                       // T[i+1, i]
+        double yn = y[i + 1];
+        double u2i = 0.0;
+        if (fabs(ai) >= fabs(sub)) {
+            if (fabs(ai) < eps_floor) ai = eps_floor;
+            const double m = sub / ai;
+            an -= m * bi;
+            yn -= m * yi;
         } else {
             // swap rows i and i+1
-            double m = a[i] / sub;
-            pv[i] = 1.0;
-            double ai = a[i], bi = b[i];
-            a[i] = sub;
-            b[i] = a[i + 1];
-            u2[i] = (i < n - 2) ? b[i + 1] : 0.0;
-            a[i + 1] = bi - m * b[i];
-            if (i < n - 2) b[i + 1] = -m * u2[i];
-            double yi = y[i];
-            y[i] = y[i + 1];
-            y[i + 1] = yi - m * y[i];
-            (void)ai;
+            const double m = ai / sub;
+            const double bold = bi;
+            ai = sub;
+            bi = an;
+            u2i = (i < n - 2) ? bn : 0.0;
+            an = bold - m * bi;
+            if (i < n - 2) bn = -m * u2i;
+            const double ysw = yi;
+            yi = yn;
+            yn = ysw - m * yi;
+        }
+        a[i] = ai;
+        b[i] = bi;
+        u2[i] = u2i;
+        y[i] = yi;
+        ai = an;
+        bi = bn;
+        yi = yn;
+        if (i + 2 < n) {
+            an = d[i + 2] - lam;
+            bn = (i + 2 < n - 1) ? e[i + 2] : 0.0;
         }
     }
-    if (fabs(a[n - 1]) < eps_floor) a[n - 1] = eps_floor;
-    // back substitution
-    y[n - 1] /= a[n - 1];
-    if (n >= 2) y[n - 2] = (y[n - 2] - b[n - 2] * y[n - 1]) / a[n - 2];
-    for (int i = n - 3; i >= 0; --i) y[i] = (y[i] - b[i] * y[i + 1] - u2[i] * y[i + 2]) / a[i];
+    if (fabs(ai) < eps_floor) ai = eps_floor;
+    // back substitution with y[i+1], y[i+2] carried in registers
+    double y2 = yi / ai;                                // y[n-1]
+    y[n - 1] = y2;
+    double y1 = (y[n - 2] - b[n - 2] * y2) / a[n - 2];  // y[n-2]
+    y[n - 2] = y1;
+    for (int i = n - 3; i >= 0; --i) {
+        const double yv = (y[i] - b[i] * y1 - u2[i] * y2) / a[i];
+        y[i] = yv;
+        y2 = y1;
+        y1 = yv;
+    }
 }
 
 
