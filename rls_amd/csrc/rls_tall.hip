@@ -117,14 +117,21 @@ k_gram_partial(int64_t n, int64_t rows_per_chunk, const zd* __restrict__ A, int6
     }
 }
 
-__global__ void k_gram_reduce(int nchunk, int ka, int kb, const zd* __restrict__ part,
-                              zd* __restrict__ C, int64_t ldc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per output element: lanes split the chunk partials, then a wave reduction.
+__global__ void __launch_bounds__(256)
+k_gram_reduce(int nchunk, int ka, int kb, const zd* __restrict__ part, zd* __restrict__ C, int64_t ldc) {
+    const int lane = threadIdx.x & 63;
+    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (idx >= ka * kb) return;
-    const int a = idx % ka, b = idx / ka;
-    zd s = zzero();
-    for (int c = 0; c < nchunk; ++c) s = zadd(s, part[(int64_t)c * ka * kb + idx]);
-    C[a + ldc * b] = s;
+    double re = 0.0, im = 0.0;
+    for (int c = lane; c < nchunk; c += 64) {
+        const zd v = part[(int64_t)c * ka * kb + idx];
+        re += v.x;
+        im += v.y;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) C[(idx % ka) + ldc * (idx / ka)] = zmk(re, im);
 }
 
 __global__ void __launch_bounds__(256)
@@ -140,13 +147,15 @@ k_colnorm2_partial(int64_t n, int64_t rows_per_chunk, const zd* __restrict__ A, 
     if (threadIdx.x == 0) part[(int64_t)blockIdx.x * k + j] = s;
 }
 
-__global__ void k_colnorm2_reduce(int nchunk, int k, const double* __restrict__ part,
-                                  double* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+k_colnorm2_reduce(int nchunk, int k, const double* __restrict__ part, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= k) return;
     double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * k + j];
-    out[j] = s;
+    for (int c = lane; c < nchunk; c += 64) s += part[(int64_t)c * k + j];
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
 }
 
 __global__ void k_scale_sub(int64_t n, zd* __restrict__ Y, int64_t ldy, const zd* __restrict__ X,
@@ -460,7 +469,7 @@ void launch_gram(hipStream_t st, int64_t n, const zd* A, int64_t lda, int ka, co
     hipLaunchKernelGGL(k_gram_partial, dim3(nch_eff, cdiv(ka, GR_T), cdiv(kb, GR_T)), dim3(256), 0,
                        st, n, rpc, A, lda, ka, B, ldb, kb, part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gram_reduce, dim3(cdiv(ka * kb, 256)), dim3(256), 0, st, nch_eff, ka, kb,
+    hipLaunchKernelGGL(k_gram_reduce, dim3(cdiv(ka * kb, 4)), dim3(256), 0, st, nch_eff, ka, kb,
                        part, C, ldc);
     LAUNCH_CHECK();
 }
@@ -473,7 +482,7 @@ void launch_colnorm2(hipStream_t st, int64_t n, const zd* A, int64_t lda, int k,
     double* dp = reinterpret_cast<double*>(part);
     hipLaunchKernelGGL(k_colnorm2_partial, dim3(nch_eff, k), dim3(256), 0, st, n, rpc, A, lda, k, dp);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colnorm2_reduce, dim3(cdiv(k, 256)), dim3(256), 0, st, nch_eff, k, dp, out);
+    hipLaunchKernelGGL(k_colnorm2_reduce, dim3(cdiv(k, 4)), dim3(256), 0, st, nch_eff, k, dp, out);
     LAUNCH_CHECK();
 }
 
