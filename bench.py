@@ -60,6 +60,8 @@ def main():
                     help="solve only the first N points of the traversal (0 = the whole grid)")
     ap.add_argument("--plain-points", type=int, default=4000,
                     help="leading traversal points timed with the plain FP64 solver")
+    ap.add_argument("--mid-points", type=int, default=1000,
+                    help="points of a mid-grid segment timed with both solvers (0 = skip)")
     ap.add_argument("--truth-samples", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -176,18 +178,44 @@ def main():
                "dsdz_max_abs_err_vs_svd": max(errs), "svd_samples": len(errs)}
         log(f"adjoint: dsdz {t_z:.2f}s, dsdt {t_t:.2f}s, |err| vs SVD vectors {max(errs):.2e}")
 
-    # plain FP64 one-shift-at-a-time solver on the leading points of the same traversal
+    # plain FP64 one-shift-at-a-time solver against the batched one on the same points:
+    # (1) the leading points of the traversal (the recycle space still growing), and
+    # (2) a mid-grid segment at steady state: both solvers resume from the state the batched
+    #     solver reaches at the segment start (the trajectories of the two modes are identical)
     npl = max(1, min(args.plain_points, nZ))
     ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain", progress=1)
     t0 = time.perf_counter()
     sig_p, st_p = solver.solve(rec.Z[:npl], ctl_plain, rec)
     t_plain = time.perf_counter() - t0
     plain_pts = npl / t_plain
-    # batched on the same leading points for a like-for-like ratio
     t0 = time.perf_counter()
     sig_b, _ = solver.solve(rec.Z[:npl], ctl, rec)
     t_b = time.perf_counter() - t0
     log(f"plain FP64 {npl} pts {t_plain:.2f}s ({plain_pts:.0f} pts/s); batched same pts {t_b:.2f}s")
+    plain_agree = float(np.nanmax(np.abs(sig_p - sig[:npl]) / sig[:npl]))
+    mid = None
+    nmid = min(args.mid_points, nZ // 2)
+    if nmid > 0:
+        P = nZ // 2
+        seg = rec.Z[P:P + nmid]
+        ctl_res_b = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched",
+                                       progress=1, window=args.window, resume=True)
+        ctl_res_p = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain",
+                                       progress=1, resume=True)
+        solver.solve(rec.Z[:P], ctl, rec)
+        t0 = time.perf_counter()
+        sig_mp, _ = solver.solve(seg, ctl_res_p, rec)
+        t_mp = time.perf_counter() - t0
+        solver.solve(rec.Z[:P], ctl, rec)
+        t0 = time.perf_counter()
+        sig_mb, _ = solver.solve(seg, ctl_res_b, rec)
+        t_mb = time.perf_counter() - t0
+        mid = {"first_point": int(P), "points": int(nmid), "plain_pts_per_s": nmid / t_mp,
+               "batched_pts_per_s": nmid / t_mb, "speedup": t_mp / t_mb,
+               "max_rel_diff": float(np.max(np.abs(sig_mp - sig_mb) / sig_mb)),
+               "max_rel_diff_vs_full_grid": float(np.max(np.abs(sig_mb - sig[P:P + nmid]) / sig[P:P + nmid]))}
+        log(f"mid-grid points {P}..{P + nmid}: plain {nmid / t_mp:.0f} pts/s, batched {nmid / t_mb:.0f} pts/s "
+            f"({t_mp / t_mb:.1f}x)")
 
     # accuracy against FP64 ground truth on sampled grid points: the full singular spectrum
     # (dense SVD) for n <= 2000, else block inverse iteration on the Schur factor.  Besides the
@@ -212,7 +240,6 @@ def main():
     rel = np.abs(sig[idx] - tr) / tr
     log(f"truth on {len(idx)} points {time.perf_counter() - t0:.1f}s: max rel {rel.max():.2e}"
         + (f", to nearest singular value {rel_any.max():.2e}" if rel_any is not None else ""))
-    plain_agree = float(np.nanmax(np.abs(sig_p - sig[:npl]) / sig[:npl]))
 
     cpu = None
     if not args.no_cpu:
@@ -264,7 +291,8 @@ def main():
         "max_rel_err_nearest_singular_value": None if rel_any is None else float(rel_any.max()),
         "rel_err_samples": int(len(idx)),
         "plain_fp64": {"value": plain_pts, "points": npl, "batched_same_points": npl / t_b,
-                       "speedup": (npl / t_b) / plain_pts, "max_rel_diff_vs_batched": plain_agree},
+                       "speedup": (npl / t_b) / plain_pts, "max_rel_diff_vs_batched": plain_agree,
+                       "mid_grid": mid},
         "refined_points": int((st.lobpcg_it > 0).sum()),
         "windows": int(st.n_windows),
         "preprocess_s": t_pre,

@@ -55,6 +55,9 @@ typedef struct rls_tri_control {
     int32_t window;                   /* max shifts per speculative window (batched mode), 0 = 256 */
     uint64_t seed;                    /* seed of the initial block (reference: rand!(X)) */
     int32_t verbose;                  /* >= 1: progress lines on stderr every ~5% of the grid */
+    int32_t resume;                   /* 1: continue from the previous solve's recycle space, block
+                                         and acceptance state on this context (a long traversal
+                                         split into several calls gives the same trajectory) */
 } rls_tri_control;
 
 #define RLS_MODE_PLAIN_FP64 0   /* one shift at a time, FP64 everywhere                        */

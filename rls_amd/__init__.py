@@ -128,7 +128,9 @@ class PsTriRecyControl:
     Execution knobs of the GPU build (no reference counterpart): ``mode`` ("plain" = one
     shift at a time, "batched" = speculative windows over one recycle snapshot),
     ``window`` (max shifts per window, 0 = auto), ``seed`` (initial block), ``maxit``,
-    ``sparse_path`` for sparse C: "factor" keeps C sparse and factors C - zI at each refine
+    ``resume`` continues from the previous solve of the same solver (recycle space, start
+    block and acceptance state), so a traversal split into several calls follows the same
+    trajectory as one call.  ``sparse_path`` for sparse C: "factor" keeps C sparse and factors C - zI at each refine
     shift (the reference's structure), "schur" densifies and uses the Schur factor (cheaper
     when the shifted LU fills in heavily and n is moderate), "auto" = "schur" for n <= 8000."""
     tol: float = 1e-3
@@ -153,6 +155,7 @@ class PsTriRecyControl:
     maxit: int = 2000
     progress: int = 0
     sparse_path: str = "auto"
+    resume: bool = False
 
     def __post_init__(self):
         if self.shift is None:
@@ -257,6 +260,7 @@ def _c_control(control: PsTriRecyControl, rec: Recursion) -> _capi.TriControl:
     c.window = control.window
     c.seed = control.seed
     c.verbose = int(control.progress)
+    c.resume = int(control.resume)
     return c
 
 

@@ -62,6 +62,7 @@ class TriControl(C.Structure):
         ("window", C.c_int32),
         ("seed", C.c_uint64),
         ("verbose", C.c_int32),
+        ("resume", C.c_int32),
     ]
 
 

@@ -102,6 +102,11 @@ struct rls_ctx {
     DArr<zd> heevW2;
     // last X (Ritz vectors) of the final state, for callers
     std::vector<cplx> lastX;
+    // state carried from one solve to the next when the control asks to resume
+    bool resumable = false;
+    int res_scaled = 0;
+    bool res_have_failed = false;
+    cplx res_last_failed{0.0, 0.0};
     // adjoint: the certified right singular vector of every point of the last solve
     // (Schur basis, n x nkeep), the shifts, and optional Schur vectors Q (C = Q T Q^H)
     bool keep_vec = false;
@@ -739,7 +744,14 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     if (!c.implict_recycle) throw RlsError{RLS_E_ARG, "implict_recycle=false is not supported"};
     const int64_t n = ctx->n;
     const int r = c.r;
-    ctx->setup_space(r, c.p);
+    const bool resume = c.resume != 0;
+    if (resume) {
+        if (!ctx->resumable || ctx->r != r || ctx->pmax != c.p)
+            throw RlsError{RLS_E_STATE, "resume: no state of a previous solve with the same r and p"};
+    } else {
+        ctx->setup_space(r, c.p);
+    }
+    ctx->resumable = false;
     ctx->pmax_min = c.p_min;
     const double thresh = resolve_fail_dist(c);
     const bool timing = c.collect_timing != 0;
@@ -754,16 +766,17 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
 
     // initial block: uniform [0,1) real and imaginary parts (rand!(X), psTriRecy.jl:607),
     // drawn from a counter-based splitmix64 stream so hosts can reproduce it exactly:
-    // element e = i + n j (column-major) uses counters 2e (re) and 2e+1 (im).
-    {
+    // element e = i + n j (column-major) uses counters 2e (re) and 2e+1 (im).  A resumed
+    // solve keeps the previous solve's block, recycle space and acceptance state instead.
+    if (!resume) {
         std::vector<cplx> X0((size_t)n * r);
         for (size_t e = 0; e < X0.size(); ++e)
             X0[e] = cplx(u01_splitmix(c.seed, 2 * e), u01_splitmix(c.seed, 2 * e + 1));
         ctx->h2d(ctx->S.p, reinterpret_cast<zd*>(X0.data()), X0.size());
     }
-    int scaled_remaining = 0;
-    bool have_failed = false;
-    cplx last_failed(0, 0);
+    int scaled_remaining = resume ? ctx->res_scaled : 0;
+    bool have_failed = resume ? ctx->res_have_failed : false;
+    cplx last_failed = resume ? ctx->res_last_failed : cplx(0, 0);
     int64_t n_it = 0, n_pts = 0, n_windows = 0, n_wshifts = 0;
     uint64_t t_rec = 0, t_ref = 0, t_upd = 0;
     const bool batched = (c.mode == RLS_MODE_BATCHED);
@@ -870,6 +883,10 @@ int solve_tri(rls_ctx* ctx, const cplx* Z, int64_t nZ, const rls_tri_control& c,
     }
     ctx->sync();
     ctx->kt_collect();
+    ctx->res_scaled = scaled_remaining;
+    ctx->res_have_failed = have_failed;
+    ctx->res_last_failed = last_failed;
+    ctx->resumable = true;
     if (ctx->keep_vec) ctx->nkeep = nZ;
     ctx->lastX.resize((size_t)n * r);
     ctx->d2h(reinterpret_cast<zd*>(ctx->lastX.data()), ctx->S.p, (size_t)n * r);
@@ -1072,6 +1089,7 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         ctx->sync();
         ctx->n = n;
         ctx->sparse = false;
+        ctx->resumable = false;
         ctx->have_q = false;   // Schur vectors and kept singular vectors belong to the old T
         ctx->nkeep = 0;
         ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
@@ -1117,6 +1135,7 @@ int rls_set_csr(rls_ctx* ctx, const rls_csr* C) {
         ctx->n = n;
         ctx->T.release();
         ctx->sparse = true;
+        ctx->resumable = false;
         ctx->have_q = false;
         ctx->nkeep = 0;
         ctx->tri_flags.release();

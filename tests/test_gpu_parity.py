@@ -190,3 +190,24 @@ def test_csr_products_match_scipy():
         X = rng.standard_normal((900, k)) + 1j * rng.standard_normal((900, k))
         np.testing.assert_allclose(s.apply_operator(X), C @ X, rtol=0, atol=1e-12)
         np.testing.assert_allclose(s.apply_operator(X, adjoint=True), C.conj().T @ X, rtol=0, atol=1e-12)
+
+
+def test_resumed_solves_follow_the_same_trajectory():
+    """A traversal split into calls (resume=True) gives the one-call trajectory, also when
+    the second part runs in the other execution mode."""
+    C, T, normC, win, p = _setup("laser300")
+    _say("resume laser300")
+    rec = R.recursion_grid(*win)
+    solver = R.TriSolver(T)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="batched")
+    s_all, st_all = solver.solve(rec.Z, ctl, rec)
+    k1 = len(rec.Z) // 3
+    s1, st1 = solver.solve(rec.Z[:k1], ctl, rec)
+    ctl2 = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="plain", resume=True)
+    s2, st2 = solver.solve(rec.Z[k1:], ctl2, rec)
+    s_split = np.concatenate([s1, s2])
+    assert np.max(np.abs(s_split - s_all) / s_all) < 1e-10
+    ref_split = np.concatenate([st1.lobpcg_it > 0, st2.lobpcg_it > 0])
+    assert np.array_equal(ref_split, st_all.lobpcg_it > 0)
+    with pytest.raises(R._capi.RlsError, match="resume"):
+        R.TriSolver(T).solve(rec.Z, ctl2, rec)
