@@ -60,7 +60,7 @@ def main():
                     help="solve only the first N points of the traversal (0 = the whole grid)")
     ap.add_argument("--plain-points", type=int, default=4000,
                     help="leading traversal points timed with the plain FP64 solver")
-    ap.add_argument("--mid-points", type=int, default=1000,
+    ap.add_argument("--mid-points", type=int, default=2000,
                     help="points of a mid-grid segment timed with both solvers (0 = skip)")
     ap.add_argument("--truth-samples", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
@@ -208,10 +208,11 @@ def main():
         t_mp = time.perf_counter() - t0
         solver.solve(rec.Z[:P], ctl, rec)
         t0 = time.perf_counter()
-        sig_mb, _ = solver.solve(seg, ctl_res_b, rec)
+        sig_mb, st_mb = solver.solve(seg, ctl_res_b, rec)
         t_mb = time.perf_counter() - t0
         mid = {"first_point": int(P), "points": int(nmid), "plain_pts_per_s": nmid / t_mp,
                "batched_pts_per_s": nmid / t_mb, "speedup": t_mp / t_mb,
+               "refined_points": int((st_mb.lobpcg_it > 0).sum()),
                "max_rel_diff": float(np.max(np.abs(sig_mp - sig_mb) / sig_mb)),
                "max_rel_diff_vs_full_grid": float(np.max(np.abs(sig_mb - sig[P:P + nmid]) / sig[P:P + nmid]))}
         log(f"mid-grid points {P}..{P + nmid}: plain {nmid / t_mp:.0f} pts/s, batched {nmid / t_mb:.0f} pts/s "
