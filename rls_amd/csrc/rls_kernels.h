@@ -37,18 +37,14 @@ void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X
 //   trmm:   Y = T X              (upper, no shift)
 //   trmmh:  Y = T^H X
 //   diag_inv: Dinv_j = (T_jj - z I)^{-1} for 64 x 64 diagonal blocks
-//   trsm_u: X <- (T - zI)^{-1} X
-//   trsm_h: X <- (T - zI)^{-H} X
+//   (the solves with T - zI are launch_trsm_persistent below)
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                        int64_t ldx, int k, zd* Y, int64_t ldy);
 void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                         int64_t ldx, int k, zd* Y, int64_t ldy);
 constexpr int TRI_NB = 64;
 void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* Dinv);
-void launch_trsm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
-                       zd* X, int64_t ldx, int k, zd* tmp);
-void launch_trsm_upper_h(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
-                         zd* X, int64_t ldx, int k, zd* tmp);
+
 
 // Block rotations of the recycle space: for i = 0..p-2 (block index i, newest block pb),
 // [A_pb A_i] <- [A_pb A_i] * U_i^H with U_i (2r x 2r) stored at U + i*4r^2 (column-major).

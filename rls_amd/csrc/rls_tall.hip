@@ -273,122 +273,6 @@ k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __res
     for (int a = 0; a < 64; ++a) out[a + 64 * c] = x[a];
 }
 
-// One step of the blocked back substitution for (T - zI) X = B (upper):
-// x_jb = Dinv_jb * X_jb (into tmp), and X_i -= T[i, jb] x_jb for the row block of this WG.
-template <int KC>
-__global__ void __launch_bounds__(256)
-k_trsm_u_step(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
-              zd* __restrict__ X, int64_t ldx, int k, zd* __restrict__ tmp, int jb) {
-    __shared__ zd xs[64][KC];
-    const int tid = threadIdx.x;
-    const int64_t o = (int64_t)jb * 64;
-    const int m = (int)min((int64_t)64, n - o);
-    const zd* Di = Dinv + (int64_t)jb * 4096;
-    // x_jb
-    for (int idx = tid; idx < 64 * KC; idx += 256) {
-        int a = idx & 63, j = idx >> 6;
-        zd s = zzero();
-        if (a < m && j < k) {
-            for (int c = a; c < m; ++c) s = zfma(Di[a + 64 * c], X[(o + c) + ldx * j], s);
-        }
-        xs[a][j] = s;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0) {
-        for (int idx = tid; idx < 64 * KC; idx += 256) {
-            int a = idx & 63, j = idx >> 6;
-            if (a < m && j < k) tmp[(o + a) + ldx * j] = xs[a][j];
-        }
-    }
-    if (jb == 0) return;
-    // update row block ib = blockIdx.x (< jb)
-    const int ib = blockIdx.x;
-    const int64_t ro = (int64_t)ib * 64;
-    const int lane = tid & 63, w = tid >> 6;
-    zd acc[KC];
-#pragma unroll
-    for (int j = 0; j < KC; ++j) acc[j] = zzero();
-    for (int c = w; c < m; c += 4) {
-        zd t = T[(ro + lane) + ldt * (o + c)];
-#pragma unroll
-        for (int j = 0; j < KC; ++j) acc[j] = zfma(t, xs[c][j], acc[j]);
-    }
-    __shared__ zd red[3][64][KC];
-    if (w > 0) {
-#pragma unroll
-        for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = acc[j];
-    }
-    __syncthreads();
-    if (w == 0) {
-        for (int j = 0; j < k; ++j) {
-            zd s = acc[j];
-            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
-            zd* xp = X + (ro + lane) + ldx * j;
-            *xp = zsub(*xp, s);
-        }
-    }
-}
-
-// One step of the blocked forward substitution for (T - zI)^H X = B:
-// x_jb = Dinv_jb^H * X_jb (into tmp), and X_i -= (T[jb, i])^H x_jb for row block i > jb.
-template <int KC>
-__global__ void __launch_bounds__(256)
-k_trsm_h_step(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
-              zd* __restrict__ X, int64_t ldx, int k, zd* __restrict__ tmp, int jb, int nblk) {
-    __shared__ zd xs[64][KC];
-    const int tid = threadIdx.x;
-    const int64_t o = (int64_t)jb * 64;
-    const int m = (int)min((int64_t)64, n - o);
-    const zd* Di = Dinv + (int64_t)jb * 4096;
-    for (int idx = tid; idx < 64 * KC; idx += 256) {
-        int a = idx & 63, j = idx >> 6;
-        zd s = zzero();
-        if (a < m && j < k) {
-            // (Dinv^H)[a, c] = conj(Dinv[c, a]), nonzero for c <= a
-            for (int c = 0; c <= a; ++c) s = zcfma(Di[c + 64 * a], X[(o + c) + ldx * j], s);
-        }
-        xs[a][j] = s;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0) {
-        for (int idx = tid; idx < 64 * KC; idx += 256) {
-            int a = idx & 63, j = idx >> 6;
-            if (a < m && j < k) tmp[(o + a) + ldx * j] = xs[a][j];
-        }
-    }
-    const int ib = jb + 1 + blockIdx.x;
-    if (ib >= nblk) return;
-    const int64_t ro = (int64_t)ib * 64;
-    const int mi = (int)min((int64_t)64, n - ro);
-    const int lane = tid & 63, w = tid >> 6;
-    zd acc[KC];
-#pragma unroll
-    for (int j = 0; j < KC; ++j) acc[j] = zzero();
-    // X_i[a] -= sum_c conj(T[o + c, ro + a]) x[c]
-    if (lane < mi) {
-        const zd* Tc = T + ldt * (ro + lane) + o;
-        for (int c = w; c < m; c += 4) {
-            zd t = Tc[c];
-#pragma unroll
-            for (int j = 0; j < KC; ++j) acc[j] = zcfma(t, xs[c][j], acc[j]);
-        }
-    }
-    __shared__ zd red[3][64][KC];
-    if (w > 0) {
-#pragma unroll
-        for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = acc[j];
-    }
-    __syncthreads();
-    if (w == 0 && lane < mi) {
-        for (int j = 0; j < k; ++j) {
-            zd s = acc[j];
-            for (int v = 0; v < 3; ++v) s = zadd(s, red[v][lane][j]);
-            zd* xp = X + (ro + lane) + ldx * j;
-            *xp = zsub(*xp, s);
-        }
-    }
-}
-
 __global__ void k_copy_cols_zero(int64_t n, zd* __restrict__ D, int64_t ldd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -519,32 +403,6 @@ void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd
     LAUNCH_CHECK();
 }
 
-void launch_trsm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
-                       zd* X, int64_t ldx, int k, zd* tmp) {
-    if (k > 8) throw RlsError{-1, "trsm: k > 8 unsupported"};
-    const int nblk = cdiv(n, 64);
-    for (int jb = nblk - 1; jb >= 0; --jb) {
-        hipLaunchKernelGGL(k_trsm_u_step<8>, dim3(jb > 0 ? jb : 1), dim3(256), 0, st, n, T, ldt, Dinv,
-                           X, ldx, k, tmp, jb);
-    }
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_copy_cols, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, tmp, ldx, X, ldx);
-    LAUNCH_CHECK();
-}
-
-void launch_trsm_upper_h(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv,
-                         zd* X, int64_t ldx, int k, zd* tmp) {
-    if (k > 8) throw RlsError{-1, "trsm: k > 8 unsupported"};
-    const int nblk = cdiv(n, 64);
-    for (int jb = 0; jb < nblk; ++jb) {
-        int below = nblk - 1 - jb;
-        hipLaunchKernelGGL(k_trsm_h_step<8>, dim3(below > 0 ? below : 1), dim3(256), 0, st, n, T, ldt,
-                           Dinv, X, ldx, k, tmp, jb, nblk);
-    }
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_copy_cols, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, tmp, ldx, X, ldx);
-    LAUNCH_CHECK();
-}
 
 void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int narr, int64_t ld,
                            int r, int p, const zd* U) {
