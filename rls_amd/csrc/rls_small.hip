@@ -674,6 +674,56 @@ __device__ void lds_jacobi_svd(zd* A, zd* V, double* sig, int* perm, int k, int*
     const double tolj = eps * (k > 4 ? k : 4);
     const int kk = (k + 1) & ~1;                 // even player count (player k is a dummy)
     const int npair = kk / 2;
+    if (npair * 4 <= 64) {
+        // Small k: wave 0 alone, four lanes per pair, no workgroup barriers between steps
+        // (wave-level fences order its LDS traffic); the other waves wait at the end.
+        if (tid < 64) {
+            const int lane = tid & 3, pb = tid >> 2;
+            for (int sweep = 0; sweep < 40; ++sweep) {
+                int rotated = 0;
+                for (int step = 0; step < kk - 1; ++step) {
+                    int a = (pb == 0) ? kk - 1 : (step + pb) % (kk - 1);
+                    int b = (step + kk - 1 - pb) % (kk - 1);
+                    if (pb == 0) b = step % (kk - 1);
+                    const int p = a < b ? a : b, q = a < b ? b : a;
+                    const bool live = pb < npair && q < k;
+                    double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                    if (live) {
+                        for (int i = lane; i < k; i += 4) {
+                            const zd ap = A[i + k * p], aq = A[i + k * q];
+                            al += zabs2(ap); be += zabs2(aq);
+                            const zd g = zcmul(ap, aq);
+                            gr += g.x; gi += g.y;
+                        }
+                    }
+                    for (int o = 2; o > 0; o >>= 1) {
+                        al += __shfl_xor(al, o); be += __shfl_xor(be, o);
+                        gr += __shfl_xor(gr, o); gi += __shfl_xor(gi, o);
+                    }
+                    const double ag = sqrt(gr * gr + gi * gi);
+                    if (live && ag > tolj * sqrt(al * be) && ag > 1e-300) {
+                        const double zeta = (be - al) / (2.0 * ag);
+                        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                        const zd ph = zmk(gr / ag, -gi / ag);   // e^{-i phi}
+                        for (int i = lane; i < k; i += 4) {
+                            const zd ap = A[i + k * p], aq = zmul(ph, A[i + k * q]);
+                            A[i + k * p] = zsub(zscale(ap, c), zscale(aq, sn));
+                            A[i + k * q] = zadd(zscale(ap, sn), zscale(aq, c));
+                            const zd vp = V[i + k * p], vq = zmul(ph, V[i + k * q]);
+                            V[i + k * p] = zsub(zscale(vp, c), zscale(vq, sn));
+                            V[i + k * q] = zadd(zscale(vp, sn), zscale(vq, c));
+                        }
+                        rotated = 1;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (!__any(rotated)) break;
+            }
+        }
+        __syncthreads();
+    } else {
     const int gsz = 8;                           // threads per pair
     for (int sweep = 0; sweep < 40; ++sweep) {
         if (tid == 0) *s_rot = 0;
@@ -721,6 +771,7 @@ __device__ void lds_jacobi_svd(zd* A, zd* V, double* sig, int* perm, int k, int*
         __syncthreads();
         if (*s_rot == 0) break;
         __syncthreads();
+    }
     }
     for (int j = tid; j < k; j += blockDim.x) {
         double s2 = 0.0;
