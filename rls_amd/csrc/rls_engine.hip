@@ -90,7 +90,7 @@ struct rls_ctx {
     DArr<zd> V, J1, J2, J5, L, H1, H2;
     std::vector<cplx> pts;
     // LOBPCG blocks
-    DArr<zd> S, CS, S2, CS2, Wt, Dinv, trsm_tmp;
+    DArr<zd> S, CS, S2, CS2, Wt, Dinv, trsm_tmp, trmm_part;
     // small / reduction workspaces
     DArr<zd> part, gram, Gs, Rm, Rinv, Racc, Rtmp, Wtot, Wk, Wneg, U;
     DArr<double> sig, lam, offI, nrm2;
@@ -164,7 +164,7 @@ struct rls_ctx {
     }
 
     ~rls_ctx() {
-        DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp,
+        DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp, &trmm_part,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
                           &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev,
                           &vkeep, &zkeep, &Qs, &gY, &gU, &gUq, &gVq};
@@ -350,7 +350,10 @@ struct rls_ctx {
     // Y = M X and Y = M^H X (n x k): dense triangular multiply or CSR products
     void apply_M(const zd* X, int k, zd* Y) {
         if (sparse) launch_csr_spmm(st, n, crp.p, cci.p, cv.p, zzero(), X, n, k, Y, n);
-        else launch_trmm_upper(st, n, T.p, n, X, n, k, Y, n);
+        else {
+            trmm_part.ensure(trmm_part_elems(n));
+            launch_trmm_upper(st, n, T.p, n, X, n, k, Y, n, trmm_part.p);
+        }
     }
     void apply_MH(const zd* X, int k, zd* Y) {
         if (sparse) launch_csr_spmm(st, n, hrp.p, hci.p, hv.p, zzero(), X, n, k, Y, n);

@@ -38,8 +38,10 @@ void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X
 //   trmmh:  Y = T^H X
 //   diag_inv: Dinv_j = (T_jj - z I)^{-1} for 64 x 64 diagonal blocks
 //   (the solves with T - zI are launch_trsm_persistent below)
+// part: workspace of trmm_part_elems(n) complex values
+size_t trmm_part_elems(int64_t n);
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
-                       int64_t ldx, int k, zd* Y, int64_t ldy);
+                       int64_t ldx, int k, zd* Y, int64_t ldy, zd* part);
 void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                         int64_t ldx, int k, zd* Y, int64_t ldy);
 constexpr int TRI_NB = 64;

@@ -173,20 +173,28 @@ __global__ void k_scale_sub(int64_t n, zd* __restrict__ Y, int64_t ldy, const zd
 }
 
 // Y = T X for upper-triangular T: workgroup = 64 rows x one 512-column slice of T (lane =
-// row, waves split the slice); slices accumulate into the pre-zeroed Y with FP64 atomics so
-// that ~n^2/(64*512) workgroups stream T in parallel.
+// row, waves split the slice), so that ~n^2/(64*512) workgroups stream T in parallel; each
+// slice writes its partial product to part (slice-major), and k_trmm_reduce sums the slices
+// in a fixed order (the result does not depend on scheduling).
 constexpr int TRMM_SLICE = 512;
 template <int KC>
 __global__ void __launch_bounds__(256)
 k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ X,
-             int64_t ldx, int k, zd* __restrict__ Y, int64_t ldy) {
+             int64_t ldx, int k, zd* __restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t r0 = (int64_t)blockIdx.x * 64;
     const int64_t row = r0 + lane;
     const int64_t cs = (int64_t)blockIdx.y * TRMM_SLICE;
     const int64_t ce = min(n, cs + TRMM_SLICE);
-    if (ce <= r0) return;                       // slice entirely left of the diagonal block
+    zd* Ps = part + (int64_t)blockIdx.y * KC * n;   // this slice's partial, n x KC
+    if (ce <= r0) {                             // slice entirely left of the diagonal block
+        for (int idx = threadIdx.x; idx < 64 * KC; idx += 256) {
+            const int64_t rr = r0 + (idx & 63);
+            if (rr < n) Ps[rr + n * (idx >> 6)] = zzero();
+        }
+        return;
+    }
     const int64_t cb = max(cs, r0);
     zd acc[KC];
 #pragma unroll
@@ -205,14 +213,23 @@ k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restr
     }
     __syncthreads();
     if (w == 0 && valid) {
-        for (int j = 0; j < k; ++j) {
+        for (int j = 0; j < KC; ++j) {
             zd s2 = acc[j];
             for (int v = 0; v < 3; ++v) s2 = zadd(s2, red[v][lane][j]);
-            double* yp = reinterpret_cast<double*>(Y + row + ldy * j);
-            atomicAdd(yp, s2.x);
-            atomicAdd(yp + 1, s2.y);
+            Ps[row + n * j] = s2;
         }
     }
+}
+
+// Y[:, j] = sum over slices of part (fixed slice order)
+__global__ void k_trmm_reduce(int64_t n, int nsl, int KC, int k, const zd* __restrict__ part, zd* __restrict__ Y,
+                              int64_t ldy) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (row >= n || j >= k) return;
+    zd s = zzero();
+    for (int sl = 0; sl < nsl; ++sl) s = zadd(s, part[(int64_t)sl * KC * n + row + n * j]);
+    Y[row + ldy * j] = s;
 }
 
 // Y = T^H X: one wave per output row i (= column i of T), lanes over j <= i.
@@ -273,11 +290,6 @@ k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __res
     for (int a = 0; a < 64; ++a) out[a + 64 * c] = x[a];
 }
 
-__global__ void k_copy_cols_zero(int64_t n, zd* __restrict__ D, int64_t ldd) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i < n) D[i + ldd * j] = zzero();
-}
 
 __global__ void k_copy_cols(int64_t n, const zd* __restrict__ S, int64_t lds, zd* __restrict__ D,
                             int64_t ldd) {
@@ -377,13 +389,15 @@ void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X
     LAUNCH_CHECK();
 }
 
+size_t trmm_part_elems(int64_t n) { return (size_t)cdiv(n, TRMM_SLICE) * 8 * n; }
+
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
-                       int64_t ldx, int k, zd* Y, int64_t ldy) {
+                       int64_t ldx, int k, zd* Y, int64_t ldy, zd* part) {
     if (k > 8) throw RlsError{-1, "trmm: k > 8 unsupported"};
-    hipLaunchKernelGGL(k_copy_cols_zero, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, Y, ldy);
+    const int nsl = (int)cdiv(n, TRMM_SLICE);
+    hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), nsl), dim3(256), 0, st, n, T, ldt, X, ldx, k, part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), cdiv(n, TRMM_SLICE)), dim3(256), 0, st, n, T,
-                       ldt, X, ldx, k, Y, ldy);
+    hipLaunchKernelGGL(k_trmm_reduce, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, nsl, 8, k, part, Y, ldy);
     LAUNCH_CHECK();
 }
 

@@ -211,3 +211,16 @@ def test_resumed_solves_follow_the_same_trajectory():
     assert np.array_equal(ref_split, st_all.lobpcg_it > 0)
     with pytest.raises(R._capi.RlsError, match="resume"):
         R.TriSolver(T).solve(rec.Z, ctl2, rec)
+
+
+def test_solves_are_repeatable():
+    """No floating-point atomics on the path: the same solve twice is bitwise identical."""
+    C, T, normC, win, p = _setup("toeplitz300")
+    _say("repeatability toeplitz300")
+    rec = R.recursion_grid(*win)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="batched")
+    solver = R.TriSolver(T)
+    s1, st1 = solver.solve(rec.Z, ctl, rec)
+    s2, st2 = solver.solve(rec.Z, ctl, rec)
+    assert np.array_equal(s1, s2)
+    assert np.array_equal(st1.lobpcg_it, st2.lobpcg_it)
