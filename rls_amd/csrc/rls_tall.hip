@@ -515,14 +515,34 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 // ---------------------------------------------------------------------------------------
 namespace {
 constexpr int PT_KC = 8;
-constexpr int PT_THREADS = 512;   // 8 waves: two per SIMD, so LDS and FMA latency overlap
+constexpr int PT_THREADS = 256;   // 4 waves, one 16-row MFMA tile of the 64-row block each
+typedef double pt_dv4 __attribute__((ext_vector_type(4)));
+
+// acc (16 x 16 real tile of wave w) += A[16w .. 16w+15][0..63] * B[0..63][0..7] in complex
+// arithmetic, laid out as [Re | Im] over the tile's 16 columns: D = Ar [Br | Bi] + Ai [-Bi | Br].
+// FP64 MFMA 16x16x4: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of each 4-deep step,
+// and the result D[(l / 16) + 4 v][l % 16] in acc[v].
+__device__ __forceinline__ void pt_mfma_tile(pt_dv4& acc, const zd (*A)[65], const zd (*B)[PT_KC], int w,
+                                             int l) {
+    const int i = 16 * w + (l & 15), kq = l >> 4, jc = l & 15;
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const int kc = 4 * ks + kq;
+        const zd a = A[i][kc];
+        const zd xb = B[kc][jc & 7];
+        const double b1 = (jc < 8) ? xb.x : xb.y;
+        const double b2 = (jc < 8) ? -xb.y : xb.x;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b2, acc, 0, 0, 0);
+    }
+}
+
 __global__ void __launch_bounds__(PT_THREADS)
 k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
                   zd* X, int64_t ldx, int k, int upper, unsigned int* counter, unsigned int* flags,
                   unsigned int epoch, int* err) {
-    constexpr int NW = PT_THREADS / 64;
     constexpr int PER = 4096 / PT_THREADS;      // T / Dinv block elements per thread
-    __shared__ zd Ts[64][65];                   // also the waves' partial sums at the end
+    __shared__ zd Ts[64][65];
     __shared__ zd Ds[64][65];
     __shared__ zd xs[64][PT_KC];
     __shared__ zd bj[64][PT_KC];
@@ -569,7 +589,6 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     auto dep = [&](int q) { return upper ? nblk - 1 - q : q; };
     if (ndep > 0) load_T(dep(0));
     // diagonal-block inverse (upper: Dinv_j, lower: Dinv_j^H) and right-hand side
-    const int a_out = tid & 63, c_out = tid >> 6;      // this thread's output element
     {
         const zd* Di = Dinv + (int64_t)j * 4096;
         zd dv[PER];
@@ -579,18 +598,26 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
             const int a = idx & 63, c = idx >> 6;
             dv[u] = upper ? Di[a + 64 * c] : Di[c + 64 * a];
         }
-        const zd bv = (a_out < m && c_out < k) ? X[(o + a_out) + ldx * c_out] : zzero();
+        zd bv[PT_KC * 64 / PT_THREADS];
+#pragma unroll
+        for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+            const int idx = tid + PT_THREADS * u;
+            const int a = idx & 63, c = idx >> 6;
+            bv[u] = (a < m && c < k) ? X[(o + a) + ldx * c] : zzero();
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int idx = tid + PT_THREADS * u;
             const int a = idx & 63, c = idx >> 6;
             Ds[a][c] = upper ? dv[u] : zconj(dv[u]);
         }
-        bj[a_out][c_out] = bv;
-    }
-    zd acc[PT_KC];
 #pragma unroll
-    for (int c = 0; c < PT_KC; ++c) acc[c] = zzero();
+        for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+            const int idx = tid + PT_THREADS * u;
+            bj[idx & 63][idx >> 6] = bv[u];
+        }
+    }
+    pt_dv4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int q = 0; q < ndep; ++q) {
         const int kb = dep(q);
         const int64_t ko = (int64_t)kb * 64;
@@ -608,40 +635,48 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         }
         __syncthreads();
         if (s_bail) break;
-        xs[a_out][c_out] = (a_out < mk && c_out < k) ? X[(ko + a_out) + ldx * c_out] : zzero();
-        __syncthreads();
-        // acc[lane][c] += sum over this wave's eighth of the block columns
+        {
+            zd xv[PT_KC * 64 / PT_THREADS];
 #pragma unroll
-        for (int tt = 0; tt < 64 / NW; ++tt) {
-            const int cc = w + NW * tt;
-            const zd tv = Ts[lane][cc];
+            for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+                const int idx = tid + PT_THREADS * u;
+                const int a = idx & 63, c = idx >> 6;
+                xv[u] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
+            }
 #pragma unroll
-            for (int c = 0; c < PT_KC; ++c) acc[c] = zfma(tv, xs[cc][c], acc[c]);
+            for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+                const int idx = tid + PT_THREADS * u;
+                xs[idx & 63][idx >> 6] = xv[u];
+            }
         }
+        __syncthreads();
+        pt_mfma_tile(acc, Ts, xs, w, lane);   // acc += T[j, kb] x_kb on the matrix cores
     }
-    // the waves' partial sums through LDS (Ts is free now), one output element per thread
-    __syncthreads();
-    zd* part = &Ts[0][0];                       // NW x 64 x PT_KC <= 64 x 65
-#pragma unroll
-    for (int c = 0; c < PT_KC; ++c) part[(w * 64 + lane) * PT_KC + c] = acc[c];
+    // bj -= acc: each real or imaginary part has exactly one owner lane
     __syncthreads();
     {
-        zd sacc = zzero();
+        double* bd = reinterpret_cast<double*>(&bj[0][0]);
+        const int jc = lane & 15;
 #pragma unroll
-        for (int v = 0; v < NW; ++v) sacc = zadd(sacc, part[(v * 64 + a_out) * PT_KC + c_out]);
-        bj[a_out][c_out] = zsub(bj[a_out][c_out], sacc);
-    }
-    __syncthreads();
-    // x_j = Ds * bj, one output per thread, two accumulation chains
-    zd s0 = zzero(), s1 = zzero();
-    if (c_out < k) {
-#pragma unroll 8
-        for (int qq = 0; qq < 64; qq += 2) {
-            s0 = zfma(Ds[a_out][qq], bj[qq][c_out], s0);
-            s1 = zfma(Ds[a_out][qq + 1], bj[qq + 1][c_out], s1);
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * w + (lane >> 4) + 4 * v;
+            bd[2 * (i * PT_KC + (jc & 7)) + (jc >= 8 ? 1 : 0)] -= acc[v];
         }
     }
-    if (a_out < m && c_out < k) X[(o + a_out) + ldx * c_out] = zadd(s0, s1);
+    __syncthreads();
+    // x_j = Ds bj, written straight to X (again one owner lane per real / imaginary part)
+    pt_dv4 xo = {0.0, 0.0, 0.0, 0.0};
+    pt_mfma_tile(xo, Ds, bj, w, lane);
+    {
+        double* xd = reinterpret_cast<double*>(X);
+        const int jc = lane & 15;
+        const int c = jc & 7;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * w + (lane >> 4) + 4 * v;
+            if (i < m && c < k) xd[2 * ((o + i) + ldx * c) + (jc >= 8 ? 1 : 0)] = xo[v];
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         __threadfence();
