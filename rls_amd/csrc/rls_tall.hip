@@ -593,11 +593,7 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         const zd* Di = Dinv + (int64_t)j * 4096;
         zd dv[PER];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int idx = tid + PT_THREADS * u;
-            const int a = idx & 63, c = idx >> 6;
-            dv[u] = upper ? Di[a + 64 * c] : Di[c + 64 * a];
-        }
+        for (int u = 0; u < PER; ++u) dv[u] = Di[tid + PT_THREADS * u];   // coalesced
         zd bv[PT_KC * 64 / PT_THREADS];
 #pragma unroll
         for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
@@ -608,8 +604,9 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int idx = tid + PT_THREADS * u;
-            const int a = idx & 63, c = idx >> 6;
-            Ds[a][c] = upper ? dv[u] : zconj(dv[u]);
+            const int r = idx & 63, c = idx >> 6;       // Dinv_j[r][c]
+            if (upper) Ds[r][c] = dv[u];
+            else Ds[c][r] = zconj(dv[u]);
         }
 #pragma unroll
         for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
