@@ -765,6 +765,82 @@ k_zgemm_tiled(int64_t n, TallOperands ops, int Ktot, const zd* __restrict__ B, i
         }
     }
 }
+
+// The same product on the FP64 matrix cores: workgroup tile 64 rows x 64 complex columns,
+// wave w owns rows 16w .. 16w+15 and eight 16 x 16 real tiles, each the [Re | Im] of 8 complex
+// columns: D = Ar [Br | Bi] + Ai [-Bi | Br] with v_mfma_f64_16x16x4f64 (lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16]; D[(l / 16) + 4 v][l % 16] lands in acc[v]).
+typedef double zg_dv4 __attribute__((ext_vector_type(4)));
+constexpr int ZGM_BK = 8;    // k depth of one LDS stage
+__global__ void __launch_bounds__(256)
+k_zgemm_mfma(int64_t n, TallOperands ops, int Ktot, const zd* __restrict__ B, int64_t ldb, int N,
+             zd* __restrict__ Y, int64_t ldy) {
+    __shared__ zd As[ZGM_BK][ZG_BM + 1];
+    __shared__ zd Bs[ZGM_BK][ZG_BN + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * ZG_BM;
+    const int c0 = blockIdx.y * ZG_BN;
+    zg_dv4 acc[8];
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) acc[ct] = zg_dv4{0.0, 0.0, 0.0, 0.0};
+    int qoff[TALL_MAX_OPS + 1];
+    qoff[0] = 0;
+    for (int q = 0; q < ops.nops; ++q) qoff[q + 1] = qoff[q] + ops.K[q];
+    const int ia = 16 * w + (lane & 15), kq = lane >> 4, jc = lane & 15;
+    for (int k0 = 0; k0 < Ktot; k0 += ZGM_BK) {
+        constexpr int PU = ZGM_BK * 64 / 256;
+        zd av[PU], bv[PU];
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int e = tid + 256 * u;
+            const int i = e & 63, kk = e >> 6;
+            const int kg = k0 + kk;
+            const int64_t row = r0 + i;
+            av[u] = zzero();
+            if (kg < Ktot && row < n) {
+                int q = 0;
+                while (kg >= qoff[q + 1]) ++q;
+                av[u] = ops.A[q][row + ops.lda[q] * (kg - qoff[q])];
+            }
+            const int kb = e % ZGM_BK, j = e / ZGM_BK;
+            const int kgb = k0 + kb, cg = c0 + j;
+            bv[u] = (kgb < Ktot && cg < N) ? B[kgb + ldb * cg] : zzero();
+        }
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int e = tid + 256 * u;
+            As[e >> 6][e & 63] = av[u];
+            Bs[e % ZGM_BK][e / ZGM_BK] = bv[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < ZGM_BK; ks += 4) {
+            const zd a = As[ks + kq][ia];
+#pragma unroll
+            for (int ct = 0; ct < 8; ++ct) {
+                const zd xb = Bs[ks + kq][8 * ct + (jc & 7)];
+                const double b1 = (jc < 8) ? xb.x : xb.y;
+                const double b2 = (jc < 8) ? -xb.y : xb.x;
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b1, acc[ct], 0, 0, 0);
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b2, acc[ct], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    double* yd = reinterpret_cast<double*>(Y);
+    const int part = (jc >= 8) ? 1 : 0;
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        const int cg = c0 + 8 * ct + (jc & 7);
+        if (cg >= N) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = r0 + 16 * w + (lane >> 4) + 4 * v;
+            if (row < n) yd[2 * (row + ldy * cg) + part] = acc[ct][v];
+        }
+    }
+}
 }  // namespace
 
 void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
@@ -772,7 +848,7 @@ void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, cons
     if (N <= 0 || n <= 0) return;
     int Ktot = 0;
     for (int q = 0; q < ops.nops; ++q) Ktot += ops.K[q];
-    hipLaunchKernelGGL(k_zgemm_tiled, dim3(cdiv(n, ZG_BM), cdiv(N, ZG_BN)), dim3(256), 0, st, n, ops, Ktot,
+    hipLaunchKernelGGL(k_zgemm_mfma, dim3(cdiv(n, ZG_BM), cdiv(N, ZG_BN)), dim3(256), 0, st, n, ops, Ktot,
                        B, ldb, N, Y, ldy);
     LAUNCH_CHECK();
 }
