@@ -267,12 +267,13 @@ class Operator:
 
 
 class RecycledSpace:
-    """Recycle-space store (src/core/recySpace.jl:40-120), implict_recycle=True layout.
+    """Recycle-space store (src/core/recySpace.jl:40-120).
 
-    Column blocks of width r; J4 aliases V when orth_S (recySpace.jl:90-91)."""
+    Column blocks of width r; J4 aliases V when orth_S (recySpace.jl:90-91).  With
+    implict_recycle=False only V, J5, J6 are maintained (the "light" cache, 48-52)."""
 
     def __init__(self, n: int, p_max: int, r: int, p_min: int = 1, orth_S: bool = True,
-                 implict_update: bool = False):
+                 implict_update: bool = False, implict_recycle: bool = True):
         if p_max <= 0:
             raise ValueError(f"p_max must be positive, got {p_max}")
         if not (0 <= p_min <= p_max):
@@ -280,6 +281,7 @@ class RecycledSpace:
         pr = p_max * r
         c = np.complex128
         self.orth_S, self.implict_update = orth_S, implict_update
+        self.implict_recycle = implict_recycle
         self.p_max, self.p_min, self.p, self.n, self.r = p_max, p_min, 0, n, r
         self.pts = np.zeros(p_max, c)
         self.V = np.zeros((n, pr), c)
@@ -305,9 +307,11 @@ def _shift_blocks(rs: RecycledSpace, l: int) -> None:
     src = slice(l * r, p * r)
     dst = slice(0, (p - l) * r)
     rs.pts[: p - l] = rs.pts[l:p].copy()
-    arrays = [rs.V, rs.J1, rs.J2, rs.J3, rs.J5, rs.J6]
-    if not rs.orth_S:
-        arrays.append(rs.J4)
+    arrays = [rs.V, rs.J5, rs.J6]
+    if rs.implict_recycle:
+        arrays += [rs.J1, rs.J2, rs.J3]
+        if not rs.orth_S:
+            arrays.append(rs.J4)
     for A in arrays:
         A[:, dst] = A[:, src].copy()
     mats = [rs.L, rs.H1, rs.H2] + ([] if rs.orth_S else [rs.H3])
@@ -331,8 +335,33 @@ def _add_blocks(rs: RecycledSpace, X: np.ndarray, M: Operator, S: Operator, z: c
         rs.V[:, new] = Q
         rs.L[new, new] = R
     Vn = rs.V[:, new]
-    rs.J5[:, new] = M.mul(Vn)                        # mul_coupled_ms! RLSPreconditioner.jl:38-49
-    rs.J6[:, new] = S.mul(Vn)
+    if getattr(M, "coupled", None) is not None and M.coupled is S.coupled:
+        # M = M0 F, S = S0 F: F V once (mul_coupled_ms! RLSPreconditioner.jl:52-76)
+        FV = M.coupled(Vn)
+        rs.J5[:, new] = M.left(FV)
+        rs.J6[:, new] = S.left(FV)
+    else:
+        rs.J5[:, new] = M.mul(Vn)                    # mul_coupled_ms! RLSPreconditioner.jl:38-49
+        rs.J6[:, new] = S.mul(Vn)
+    if not rs.implict_recycle:                       # light cache (recySpace.jl:420-449)
+        J5n, J6n = rs.J5[:, new], rs.J6[:, new]
+        rs.H1[new, new] = J5n.conj().T @ J5n
+        rs.H2[new, new] = J5n.conj().T @ J6n
+        rs.H3[new, new] = np.eye(r) if rs.orth_S else J6n.conj().T @ J6n
+        if p > 0:
+            rs.H1[old, new] = rs.J5[:, old].conj().T @ J5n
+            rs.H1[new, old] = rs.H1[old, new].conj().T
+            rs.H2[old, new] = rs.J5[:, old].conj().T @ J6n
+            rs.H2[new, old] = J5n.conj().T @ rs.J6[:, old]
+            if rs.orth_S:
+                rs.H3[old, new] = 0
+                rs.H3[new, old] = 0
+            else:
+                rs.H3[old, new] = rs.J6[:, old].conj().T @ J6n
+                rs.H3[new, old] = rs.H3[old, new].conj().T
+        rs.pts[p] = z
+        rs.p = p + 1
+        return
     rs.J1[:, new] = M.rmul(rs.J5[:, new])            # recySpace.jl:377
     rs.J2[:, new] = M.rmul(rs.J6[:, new])            # recySpace.jl:380
     rs.J3[:, new] = S.rmul(rs.J5[:, new])            # recySpace.jl:385
@@ -384,13 +413,17 @@ def _block_unitary_update(rs: RecycledSpace, S: Operator) -> None:
         rot_cols(rs.V)
         rot_cols(rs.J5)
         rot_cols(rs.J6)
-        rot_cols(rs.J1)
-        rot_cols(rs.J2)
-        if rs.implict_update:
+        if not rs.implict_recycle:
+            pass
+        elif rs.implict_update:
+            rot_cols(rs.J1)
+            rot_cols(rs.J2)
             rot_cols(rs.J3)
             if not rs.orth_S:
                 rot_cols(rs.J4)
         else:
+            rot_cols(rs.J1)
+            rot_cols(rs.J2)
             rs.J3[:, bp] = S.rmul(rs.J5[:, bp])
             rs.J3[:, bi] = S.rmul(rs.J5[:, bi])
             if not rs.orth_S:
@@ -613,6 +646,7 @@ class LobpcgResult:
     X: np.ndarray          # keep.S[:, 1:r]  (Ritz vectors)
     W: np.ndarray          # last residual block
     res1: float            # ||W[:,1]|| at the last check
+    CX: np.ndarray = None  # keep.CS[:, 1:r] = C X at return
 
 
 def lobpcg_svd(Cop: Operator, X0: Optional[np.ndarray], K, ctrl: LobpcgSvdControl,
@@ -661,13 +695,13 @@ def lobpcg_svd(Cop: Operator, X0: Optional[np.ndarray], K, ctrl: LobpcgSvdContro
         res1 = float(np.linalg.norm(W[:, 0]))
         if check_conv_svd(W, sig, tol, side, normC, ctrl.conv_range, ctrl.conv_thresh_type):
             ret = sig[0] if side == "S" else 1.0 / sig[0]
-            return LobpcgResult(ret, i, sig, X, W, res1)
+            return LobpcgResult(ret, i, sig, X, W, res1, CX)
         if K is not None:
             W = K(W)
         W = rls_orth(np.hstack([X, P]), W, ot)
         CW = Cop.mul(W)
     ret = sig[0] if side == "S" else 1.0 / sig[0]
-    return LobpcgResult(ret, ctrl.maxit, sig, X, W, res1)
+    return LobpcgResult(ret, ctrl.maxit, sig, X, W, res1, CX)
 
 
 # --------------------------------------------------------------------------------------

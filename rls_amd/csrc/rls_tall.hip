@@ -693,83 +693,15 @@ void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt,
 }
 
 // ---------------------------------------------------------------------------------------
-// Register/LDS-tiled complex FP64 GEMM for the batched recycle products:
+// Complex FP64 GEMM for the batched recycle products on the FP64 matrix cores:
 //   Y (n x N, ld ldy) = [A_0 .. A_{q-1}] (n x K) * B (K x N, ld ldb)
-// Workgroup tile 64 rows x 64 columns, K step 8; 256 threads, each a 4 x 4 block of outputs
-// (rows ty + 16 i, columns tx + 16 j), so every A element fetched from LDS feeds 4 columns
-// and every B element 4 rows.  The operand list is the same as k_tall_gemm's.
+// The operand list is the same as k_tall_gemm's.  Workgroup tile 64 rows x 64 complex
+// columns, wave w owns rows 16w .. 16w+15 and eight 16 x 16 real tiles, each the [Re | Im]
+// of 8 complex columns: D = Ar [Br | Bi] + Ai [-Bi | Br] with v_mfma_f64_16x16x4f64 (lane l
+// holds A[l % 16][l / 16], B[l / 16][l % 16]; D[(l / 16) + 4 v][l % 16] lands in acc[v]).
 // ---------------------------------------------------------------------------------------
 namespace {
-constexpr int ZG_BM = 64, ZG_BN = 64, ZG_BK = 8;
-__global__ void __launch_bounds__(256)
-k_zgemm_tiled(int64_t n, TallOperands ops, int Ktot, const zd* __restrict__ B, int64_t ldb, int N,
-              zd* __restrict__ Y, int64_t ldy) {
-    __shared__ zd As[ZG_BK][ZG_BM + 1];
-    __shared__ zd Bs[ZG_BK][ZG_BN + 1];
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int64_t r0 = (int64_t)blockIdx.x * ZG_BM;
-    const int c0 = blockIdx.y * ZG_BN;
-    zd acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = zzero();
-    // operand boundaries
-    int qoff[TALL_MAX_OPS + 1];
-    qoff[0] = 0;
-    for (int q = 0; q < ops.nops; ++q) qoff[q + 1] = qoff[q] + ops.K[q];
-    for (int k0 = 0; k0 < Ktot; k0 += ZG_BK) {
-        // A tile: 64 rows x 8 k  (512 elements, 2 per thread), coalesced along rows
-        for (int e = tid; e < ZG_BM * ZG_BK; e += 256) {
-            const int i = e & 63, kk = e >> 6;
-            const int kg = k0 + kk;
-            zd v = zzero();
-            const int64_t row = r0 + i;
-            if (kg < Ktot && row < n) {
-                int q = 0;
-                while (kg >= qoff[q + 1]) ++q;
-                v = ops.A[q][row + ops.lda[q] * (kg - qoff[q])];
-            }
-            As[kk][i] = v;
-        }
-        // B tile: 8 k x 64 columns
-        for (int e = tid; e < ZG_BK * ZG_BN; e += 256) {
-            const int kk = e & 7, j = e >> 3;
-            const int kg = k0 + kk, cg = c0 + j;
-            Bs[kk][j] = (kg < Ktot && cg < N) ? B[kg + ldb * cg] : zzero();
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < ZG_BK; ++kk) {
-            zd a[4], bb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t row = r0 + ty + 16 * i;
-        if (row >= n) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int cg = c0 + tx + 16 * j;
-            if (cg < N) Y[row + ldy * cg] = acc[i][j];
-        }
-    }
-}
-
-// The same product on the FP64 matrix cores: workgroup tile 64 rows x 64 complex columns,
-// wave w owns rows 16w .. 16w+15 and eight 16 x 16 real tiles, each the [Re | Im] of 8 complex
-// columns: D = Ar [Br | Bi] + Ai [-Bi | Br] with v_mfma_f64_16x16x4f64 (lane l holds
-// A[l % 16][l / 16], B[l / 16][l % 16]; D[(l / 16) + 4 v][l % 16] lands in acc[v]).
+constexpr int ZG_BM = 64, ZG_BN = 64;
 typedef double zg_dv4 __attribute__((ext_vector_type(4)));
 constexpr int ZGM_BK = 8;    // k depth of one LDS stage
 __global__ void __launch_bounds__(256)

@@ -13,6 +13,8 @@ configuration descriptions with numpy PCG64 seeds owned by this project.
 * ``sprand``          sparse N(0,1) at uniform positions, scaled to a target 2-norm.
 * ``banded_cfd``      banded + 1% off-band sparse, scaled to a target 2-norm (af23560-like).
 * ``basor``           the reference's complex Toeplitz test (example/basor/basor.jl:6-11).
+* ``skewlap3d``       3-D skew Laplacian of example/skewlap3d/skewlap3d.jl:11-30: on an
+                      (N-1)^3 grid, sum of kron products of N^2 tridiag(1.5, -2, 0.5).
 """
 from __future__ import annotations
 
@@ -57,6 +59,36 @@ def laser_landau(n: int, F: float = 40 * np.pi) -> np.ndarray:
     sw = np.sqrt(w)
     d = x[:, None] - x[None, :]
     return np.sqrt(1j * F) * (sw[:, None] * sw[None, :]) * np.exp(-1j * np.pi * F * d * d)
+
+
+def skewlap3d(N: int) -> sp.csr_matrix:
+    """(N-1)^3 x (N-1)^3 real nonsymmetric operator (example/skewlap3d/skewlap3d.jl:11-30):
+    D = N^2 tridiag(sub 1.5, diag -2, super 0.5) of size N-1, A = I(x)I(x)D + I(x)D(x)I + D(x)I(x)I."""
+    if N < 3:
+        raise ValueError("N must be >= 3")
+    m = N - 1
+    D = (N * N) * sp.diags([np.full(m - 1, 1.5), np.full(m, -2.0), np.full(m - 1, 0.5)],
+                           [-1, 0, 1], format="csr")
+    Im = sp.identity(m, format="csr")
+    A = sp.kron(Im, sp.kron(Im, D)) + sp.kron(Im, sp.kron(D, Im)) + sp.kron(D, sp.kron(Im, Im))
+    return A.astype(np.complex128).tocsr()
+
+
+def norm2_estimate(C: sp.spmatrix, seed: int = 1, iters: int = 300) -> float:
+    """||C||_2 by power iteration on C^H C."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(C.shape[1]).astype(np.complex128)
+    v /= np.linalg.norm(v)
+    CH = C.conj().T.tocsr()
+    s = 0.0
+    for _ in range(iters):
+        w = CH @ (C @ v)
+        s_new = float(np.sqrt(np.linalg.norm(w)))
+        v = w / np.linalg.norm(w)
+        if abs(s_new - s) < 1e-12 * s_new:
+            return s_new
+        s = s_new
+    return s
 
 
 def _scale_to_norm(C: sp.spmatrix, target: float, seed: int) -> sp.spmatrix:
