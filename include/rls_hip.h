@@ -169,6 +169,65 @@ int rls_sigma_grad_pattern(rls_ctx* ctx, int64_t first, int64_t count, int64_t n
  * out[(k + n - 1) + (2 n - 1) * b] = sum over i - j = k of Re(conj(u_b[i]) v_b[j]). */
 int rls_sigma_grad_toeplitz(rls_ctx* ctx, int64_t first, int64_t count, double* out);
 
+/* ---- psPrecRecy (src/core/psPrecRecy.jl:617-1042): preconditioner and device operators ----
+ *
+ * Threshold incomplete Cholesky of the shifted normal matrix, built on the host as the
+ * reference does: get_hsl_LLTP (src/preconditioner/hsl_preconditioner.jl:159-252) with the
+ * MI28 driver ichsd / ictkl3_new! (ichsd.jl:110-258, ictkl3_new.jl:1-272).  N is the lower
+ * triangle of (C - zI)^H (C - zI) in 0-based CSC (rows of each column >= the column, the
+ * diagonal present), p the column ordering (0-based, p[k] = original index at position k;
+ * the reference's qr(C).cpiv), tau1 = hsl_tol, tau2 = hsl_cntl_1, alpha = hsl_cntl_2,
+ * lsize = hsl_icntl_1, rsize = hsl_icntl_2 (psPrecRecy.jl:445-454).  The factor L satisfies
+ * P^T N P ~= L L^H with (P X)[i] = X[invp[i]] (RLSMatrices.jl:356-365). */
+typedef struct rls_csc_lower {
+    int64_t n;
+    const int64_t* colptr;   /* n + 1 */
+    const int64_t* rowind;   /* colptr[n] */
+    const double* values;    /* colptr[n] complex */
+} rls_csc_lower;
+typedef struct rls_ic rls_ic;
+#define RLS_IC_E_ARG RLS_E_ARG
+int rls_ic_build(const rls_csc_lower* N, const int64_t* p, double tau1, double tau2, double alpha,
+                 int64_t lsize, int64_t rsize, rls_ic** out);
+const char* rls_ic_last_error(void);
+/* n, nnz(L), the diagonal shift finally used and the number of shift increases (ichsd.jl:213-248) */
+int rls_ic_info(const rls_ic* f, int64_t* n, int64_t* nnz, double* alpha, int* n_shifts);
+/* L in 0-based CSC (colptr n + 1, rowind / values nnz), diagonal first in every column */
+int rls_ic_copy(const rls_ic* f, int64_t* colptr, int64_t* rowind, double* values);
+void rls_ic_free(rls_ic* f);
+
+/* Upload L (as CSR of L and of L^H, with the dependency analysis of the sync-free solve) and
+ * p to the device of `ctx`; the context must hold C (rls_set_csr) of the same order.
+ * Replaces SparseLLTP's construction (src/preconditioner/RLSPreconditioner.jl:15-25). */
+int rls_set_ic(rls_ctx* ctx, const rls_ic* f);
+
+/* Launch the context's kernels on the caller's stream (a hipStream_t; NULL restores the
+ * context's own stream), so a host driver that keeps blocks in its own device allocations
+ * orders them with its other work.  No counterpart in the reference. */
+int rls_set_stream(rls_ctx* ctx, void* stream);
+
+/* Y = op(X) for device blocks X (n x k, leading dimension ldx) and Y (n x k, ldy), Y != X,
+ * enqueued on the context's stream (no synchronisation):
+ *   RLS_OP_SHIFTED    (C - zI) X            ShiftedMat mul!      RLSMatrices.jl:24-76
+ *   RLS_OP_SHIFTED_H  (C - zI)^H X          ShiftedMat adjoint
+ *   RLS_OP_INVF       P L^{-H} X            SparseLLTP.invF      RLSPreconditioner.jl:15-25
+ *   RLS_OP_INVF_H     L^{-1} P^T X          invF'
+ *   RLS_OP_KINV       P L^{-H} L^{-1} P^T X ldiv!(K, B)          RLSPreconditioner.jl:79-110
+ *   RLS_OP_LINV       L^{-1} X              InvMat(L)            psPrecRecy.jl:388-402
+ *   RLS_OP_LINV_H     L^{-H} X              InvMat(L)'
+ * Bounded waits of the triangular solves report through rls_dev_sync. */
+#define RLS_OP_SHIFTED 0
+#define RLS_OP_SHIFTED_H 1
+#define RLS_OP_INVF 2
+#define RLS_OP_INVF_H 3
+#define RLS_OP_KINV 4
+#define RLS_OP_LINV 5
+#define RLS_OP_LINV_H 6
+int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* X, int64_t ldx, int k,
+                  double* Y, int64_t ldy);
+/* Wait for the context's stream and report a triangular-solve wait that exceeded its bound. */
+int rls_dev_sync(rls_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,17 @@ EXPORTED = [
     "rls_set_csr",
     "rls_set_factor_callback",
     "rls_set_sparse_factor",
+    "rls_ic_build",
+    "rls_ic_last_error",
+    "rls_ic_info",
+    "rls_ic_copy",
+    "rls_ic_free",
+    "rls_set_ic",
+    "rls_set_stream",
+    "rls_dev_apply",
+    "rls_dev_sync",
 ]
+RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 
 
@@ -93,6 +103,11 @@ class Csr(C.Structure):
     _fields_ = [("n", C.c_int64), ("rowptr", C.c_void_p), ("colind", C.c_void_p), ("values", C.c_void_p)]
 
 
+class CscLower(C.Structure):
+    """rls_csc_lower: 0-based CSC lower triangle with interleaved complex values."""
+    _fields_ = [("n", C.c_int64), ("colptr", C.c_void_p), ("rowind", C.c_void_p), ("values", C.c_void_p)]
+
+
 FACTOR_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double)
 
 _lib = None
@@ -135,10 +150,23 @@ def load() -> C.CDLL:
                        ("rls_set_csr", [vp, C.POINTER(Csr)]),
                        ("rls_set_factor_callback", [vp, FACTOR_FN, vp]),
                        ("rls_set_sparse_factor", [vp, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
-                                                  C.POINTER(Csr), vp, vp])):
+                                                  C.POINTER(Csr), vp, vp]),
+                       ("rls_ic_build", [C.POINTER(CscLower), vp, C.c_double, C.c_double, C.c_double, i64,
+                                         i64, C.POINTER(vp)]),
+                       ("rls_ic_info", [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int)]),
+                       ("rls_ic_copy", [vp, vp, vp, vp]),
+                       ("rls_set_ic", [vp, vp]),
+                       ("rls_set_stream", [vp, vp]),
+                       ("rls_dev_apply", [vp, C.c_int, C.c_double, C.c_double, vp, i64, C.c_int, vp, i64]),
+                       ("rls_dev_sync", [vp])):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_int
+    lib.rls_ic_last_error.argtypes = []
+    lib.rls_ic_last_error.restype = C.c_char_p
+    lib.rls_ic_free.argtypes = [vp]
+    lib.rls_ic_free.restype = None
     _lib = lib
     return lib
 

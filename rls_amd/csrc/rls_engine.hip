@@ -17,6 +17,7 @@
 // the first failure is refined, the space updated, and the next window starts after it.
 #include "../../include/rls_hip.h"
 #include "rls_common.h"
+#include "rls_ic_internal.h"
 #include "rls_kernels.h"
 
 #include <algorithm>
@@ -61,7 +62,8 @@ uint64_t now_ns() {
 
 struct rls_ctx {
     int device = 0;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;       // stream all work is enqueued on
+    hipStream_t own_st = nullptr;   // the context's own stream (st unless rls_set_stream)
     std::string err;
     // operator: dense upper-triangular Schur factor T, or (sparse path) C in CSR with the CSR
     // of C^H, and the host-supplied factor Pr (C - zI) Pc = L U of the current refine shift
@@ -80,6 +82,11 @@ struct rls_ctx {
     DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block)]
     DArr<int> sp_err;
     unsigned int sp_epoch = 0;
+    // psPrecRecy preconditioner K = P L L^H P^T: CSR of L and of L^H, p and invp
+    SpTri icL, icLH;
+    DArr<int64_t> ic_p, ic_invp;
+    bool have_ic = false;
+    DArr<zd> ic_tmp;
     rls_factor_fn factor_cb = nullptr;
     void* factor_user = nullptr;
     bool factor_ready = false;
@@ -175,7 +182,10 @@ struct rls_ctx {
         DArr<int64_t>* is[] = {&crp, &cci, &hrp, &hci, &perm_r, &perm_c};
         for (auto* a : is) a->release();
         cv.release(); hv.release(); sp_tmp.release(); sp_cnt.release(); sp_err.release();
-        for (SpTri* f : {&fL, &fU, &fLH, &fUH}) { f->rp.release(); f->ci.release(); f->v.release(); f->dep.release(); }
+        for (SpTri* f : {&fL, &fU, &fLH, &fUH, &icL, &icLH}) {
+            f->rp.release(); f->ci.release(); f->v.release(); f->dep.release();
+        }
+        ic_p.release(); ic_invp.release(); ic_tmp.release();
         tri_ctr.release();
         tri_flags.release();
         tri_err.release();
@@ -187,7 +197,7 @@ struct rls_ctx {
         info.release();
         rr.release();
         coff.release();
-        if (st) (void)hipStreamDestroy(st);
+        if (own_st) (void)hipStreamDestroy(own_st);
     }
 
     void sync() { HIP_TRY(hipStreamSynchronize(st)); }
@@ -1019,10 +1029,11 @@ int rls_ctx_create(int device, rls_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return RLS_E_HIP;
     rls_ctx* c = new rls_ctx();
     c->device = device;
-    if (hipStreamCreate(&c->st) != hipSuccess) {
+    if (hipStreamCreate(&c->own_st) != hipSuccess) {
         delete c;
         return RLS_E_HIP;
     }
+    c->st = c->own_st;
     *out = c;
     return RLS_OK;
 }
@@ -1184,6 +1195,117 @@ int rls_set_sparse_factor(rls_ctx* ctx, const rls_csr* L, const rls_csr* U, cons
         upload(ctx, ctx->perm_c, perm_c, (size_t)n);
         ctx->sync();
         ctx->factor_ready = true;
+        return RLS_OK;
+    });
+}
+
+int rls_set_ic(rls_ctx* ctx, const rls_ic* f) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "rls_set_ic: no sparse operator set (rls_set_csr)"};
+        if (!f || f->n != ctx->n) throw RlsError{RLS_E_ARG, "rls_set_ic: factor order differs from C"};
+        const int64_t n = f->n;
+        const int64_t nnz = f->colptr[n];
+        // CSR of L^H is the CSC of L read row-wise with conjugated values; CSR of L by transposing
+        std::vector<cplx> vh(nnz);
+        for (int64_t e = 0; e < nnz; ++e) vh[e] = std::conj(f->values[e]);
+        std::vector<int64_t> rp(n + 1, 0), ci(nnz);
+        std::vector<cplx> vl(nnz);
+        for (int64_t e = 0; e < nnz; ++e) rp[f->rowind[e] + 1]++;
+        for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+        std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t e = f->colptr[j]; e < f->colptr[j + 1]; ++e) {
+                const int64_t q = fill[f->rowind[e]]++;
+                ci[q] = j;
+                vl[q] = f->values[e];
+            }
+        std::vector<int64_t> invp(n);
+        for (int64_t k = 0; k < n; ++k) invp[f->p[k]] = k;
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
+        rls_csr Lc{n, rp.data(), ci.data(), reinterpret_cast<const double*>(vl.data())};
+        rls_csr LHc{n, f->colptr.data(), f->rowind.data(), reinterpret_cast<const double*>(vh.data())};
+        upload_tri(ctx, ctx->icL, &Lc, true, "IC factor L");
+        upload_tri(ctx, ctx->icLH, &LHc, false, "IC factor L^H");
+        upload(ctx, ctx->ic_p, f->p.data(), (size_t)n);
+        upload(ctx, ctx->ic_invp, invp.data(), (size_t)n);
+        ctx->sync();
+        ctx->have_ic = true;
+        return RLS_OK;
+    });
+}
+
+int rls_set_stream(rls_ctx* ctx, void* stream) {
+    return guarded(ctx, [&]() -> int {
+        ctx->sync();
+        ctx->st = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own_st;
+        return RLS_OK;
+    });
+}
+
+int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* Xd, int64_t ldx, int k,
+                  double* Yd, int64_t ldy) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "rls_dev_apply: no sparse operator set (rls_set_csr)"};
+        const int64_t n = ctx->n;
+        if (k < 0 || ldx < n || ldy < n || (k > 0 && (!Xd || !Yd)))
+            throw RlsError{RLS_E_ARG, "rls_dev_apply: bad block shape"};
+        if (Xd == Yd && k > 0) throw RlsError{RLS_E_ARG, "rls_dev_apply: Y must not alias X"};
+        if (k == 0) return RLS_OK;
+        if (op >= RLS_OP_INVF && !ctx->have_ic) throw RlsError{RLS_E_STATE, "rls_dev_apply: no IC factor set"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        const zd* X = reinterpret_cast<const zd*>(Xd);
+        zd* Y = reinterpret_cast<zd*>(Yd);
+        hipStream_t st = ctx->st;
+        auto solve = [&](rls_ctx::SpTri& F, bool lower, zd* B, int kk) { ctx->sp_solve(F, lower, false, B, kk); };
+        switch (op) {
+        case RLS_OP_SHIFTED:
+            launch_csr_spmm(st, n, ctx->crp.p, ctx->cci.p, ctx->cv.p, zmk(z_re, z_im), X, ldx, k, Y, ldy);
+            break;
+        case RLS_OP_SHIFTED_H:
+            launch_csr_spmm(st, n, ctx->hrp.p, ctx->hci.p, ctx->hv.p, zmk(z_re, -z_im), X, ldx, k, Y, ldy);
+            break;
+        default: {
+            ctx->ic_tmp.ensure((size_t)n * k);
+            zd* t = ctx->ic_tmp.p;
+            switch (op) {
+            case RLS_OP_INVF:       // P L^{-H} X: t = X, t <- L^{-H} t, Y[i] = t[invp[i]]
+                launch_copy2d(st, n, k, X, ldx, t, n);
+                solve(ctx->icLH, false, t, k);
+                launch_permute_rows(st, n, ctx->ic_invp.p, t, n, Y, ldy, k, false);
+                break;
+            case RLS_OP_INVF_H:     // L^{-1} P^T X: t[i] = X[p[i]], t <- L^{-1} t
+                launch_permute_rows(st, n, ctx->ic_p.p, X, ldx, t, n, k, false);
+                solve(ctx->icL, true, t, k);
+                launch_copy2d(st, n, k, t, n, Y, ldy);
+                break;
+            case RLS_OP_KINV:
+                launch_permute_rows(st, n, ctx->ic_p.p, X, ldx, t, n, k, false);
+                solve(ctx->icL, true, t, k);
+                solve(ctx->icLH, false, t, k);
+                launch_permute_rows(st, n, ctx->ic_invp.p, t, n, Y, ldy, k, false);
+                break;
+            case RLS_OP_LINV:
+            case RLS_OP_LINV_H:
+                launch_copy2d(st, n, k, X, ldx, t, n);
+                if (op == RLS_OP_LINV) solve(ctx->icL, true, t, k);
+                else solve(ctx->icLH, false, t, k);
+                launch_copy2d(st, n, k, t, n, Y, ldy);
+                break;
+            default:
+                throw RlsError{RLS_E_ARG, "rls_dev_apply: unknown operator"};
+            }
+        }
+        }
+        return RLS_OK;
+    });
+}
+
+int rls_dev_sync(rls_ctx* ctx) {
+    return guarded(ctx, [&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
+        ctx->check_tri_err();
         return RLS_OK;
     });
 }
