@@ -121,6 +121,13 @@ def load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rls_amd.build` "
                            "(there is no CPU fallback)")
+    # One HIP runtime per process: torch bundles its own libamdhip64.so.7.  Loading torch
+    # first lets the library bind to that already-loaded runtime (same SONAME) instead of
+    # bringing in a second copy, which then fails to open the device once torch has.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     lib.rls_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]

@@ -979,9 +979,12 @@ void upload_tri(rls_ctx* ctx, rls_ctx::SpTri& F, const rls_csr* A, bool lower, c
     const int64_t n = ctx->n;
     check_csr(A, n, what);
     for (int64_t i = 0; i < n; ++i)
-        for (int64_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e)
+        for (int64_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e) {
             if (lower ? A->colind[e] > i : A->colind[e] < i)
                 throw RlsError{RLS_E_ARG, std::string(what) + ": entry outside the triangle"};
+            if (e > A->rowptr[i] && A->colind[e] <= A->colind[e - 1])
+                throw RlsError{RLS_E_ARG, std::string(what) + ": row columns must be sorted and distinct"};
+        }
     const int RB = sptrsv_block_rows();
     const int nb = (int)cdiv(n, RB);
     std::vector<int> dep(nb);
@@ -999,6 +1002,7 @@ void upload_tri(rls_ctx* ctx, rls_ctx::SpTri& F, const rls_csr* A, bool lower, c
     upload(ctx, F.ci, A->colind, (size_t)nnz);
     upload(ctx, F.v, reinterpret_cast<const zd*>(A->values), (size_t)nnz);
     upload(ctx, F.dep, dep.data(), (size_t)nb);
+    ctx->sync();   // the copies read the local vector dep
 }
 
 template <typename F>

@@ -62,9 +62,10 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 // Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
 // ascending order, upper: descending).  One workgroup per block of SP_RB rows, started in
 // dependency order by a ticket counter.  A block first waits for every earlier block its
-// rows reference (host analysis: dep_lo[b] for lower / dep_hi[b] for upper), then solves
-// its rows in order, one wave per right-hand-side pair, lanes splitting each row's entries.
-// The wait is bounded: on overrun *err is raised and the workgroup exits.
+// rows reference (host analysis: dep_lo[b] for lower / dep_hi[b] for upper), then forms all
+// its rows' sums over earlier blocks in parallel and solves the small triangle of in-block
+// entries from LDS, one wave per right-hand-side pair.  The wait is bounded: on overrun
+// *err is raised and the workgroup exits.
 constexpr int SP_RB = 32;
 
 template <bool LOWER>
@@ -75,6 +76,9 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     __shared__ unsigned int s_ticket;
     __shared__ int s_ok;
     __shared__ zd xs[SP_RB][8];
+    __shared__ zd D[SP_RB][SP_RB + 1];
+    __shared__ zd ext[SP_RB][8];
+    __shared__ zd sdg[SP_RB];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int nblk = (int)((n + SP_RB - 1) / SP_RB);
@@ -99,46 +103,65 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     __syncthreads();
     if (!s_ok) return;
-    // wave w handles right-hand sides c = 2w, 2w+1 (k <= 8)
-    const int c0 = 2 * w;
-    for (int rr = 0; rr < rows; ++rr) {
-        const int li = LOWER ? rr : rows - 1 - rr;
-        const int64_t i = r0 + li;
-        const int64_t e0 = rp[i], e1 = rp[i + 1];
-        zd s0 = zzero(), s1 = zzero(), dg = zmk(1.0, 0.0);
-        for (int64_t e = e0 + lane; e < e1; e += 64) {
-            const int64_t j = ci[e];
-            const zd a = v[e];
-            if (j == i) { dg = a; continue; }
-            zd x0, x1;
-            if (j >= r0 && j < r0 + rows) {
-                x0 = (c0 < k) ? xs[j - r0][c0] : zzero();
-                x1 = (c0 + 1 < k) ? xs[j - r0][c0 + 1] : zzero();
-            } else {
-                x0 = (c0 < k) ? X[j + ldx * c0] : zzero();
-                x1 = (c0 + 1 < k) ? X[j + ldx * (c0 + 1)] : zzero();
+    // Phase A, all rows of the block at once (8 threads per row): the row's sum over the
+    // entries outside the block (those solution rows are final), subtracted from the
+    // right-hand side, and the diagonal; the entries inside the block go to a dense tile D
+    // (row li, column j - r0).  Rows are sorted with distinct columns (checked on upload).
+    for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
+    xs[tid >> 3][tid & 7] = zzero();
+    __syncthreads();
+    {
+        const int li = tid >> 3, sub = tid & 7;
+        zd acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = zzero();
+        double dr = 0.0, di = 0.0;
+        if (li < rows) {
+            const int64_t i = r0 + li;
+            const int64_t e1 = rp[i + 1];
+            for (int64_t e = rp[i] + sub; e < e1; e += 8) {
+                const int64_t j = ci[e];
+                const zd a = v[e];
+                if (j == i) { dr = a.x; di = a.y; continue; }
+                if (j >= r0 && j < r0 + rows) { D[li][j - r0] = a; continue; }
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < k) acc[c] = zfma(a, X[j + ldx * c], acc[c]);
             }
-            s0 = zfma(a, x0, s0);
-            s1 = zfma(a, x1, s1);
         }
-        double a0 = s0.x, a1 = s0.y, a2 = s1.x, a3 = s1.y, dr = dg.x, di = dg.y;
-        for (int o = 32; o > 0; o >>= 1) {
-            a0 += __shfl_xor(a0, o); a1 += __shfl_xor(a1, o);
-            a2 += __shfl_xor(a2, o); a3 += __shfl_xor(a3, o);
+        for (int o = 4; o > 0; o >>= 1) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                acc[c].x += __shfl_xor(acc[c].x, o);
+                acc[c].y += __shfl_xor(acc[c].y, o);
+            }
+            dr += __shfl_xor(dr, o);
+            di += __shfl_xor(di, o);
         }
-        // the diagonal entry sits in exactly one lane (or none for a unit diagonal)
-        int hasd = !(dr == 1.0 && di == 0.0);
-        for (int o = 32; o > 0; o >>= 1) {
-            const double pr = __shfl_xor(dr, o), pi = __shfl_xor(di, o);
-            const int ph = __shfl_xor(hasd, o);
-            if (!hasd && ph) { dr = pr; di = pi; hasd = 1; }
+        if (sub == 0 && li < rows) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                ext[li][c] = (c < k) ? zsub(X[(r0 + li) + ldx * c], acc[c]) : zzero();
+            sdg[li] = unit_diag ? zmk(1.0, 0.0) : zmk(dr, di);
         }
-        const zd diag = unit_diag ? zmk(1.0, 0.0) : zmk(dr, di);
-        if (lane == 0) {
-            if (c0 < k) xs[li][c0] = zdiv(zsub(X[i + ldx * c0], zmk(a0, a1)), diag);
-            if (c0 + 1 < k) xs[li][c0 + 1] = zdiv(zsub(X[i + ldx * (c0 + 1)], zmk(a2, a3)), diag);
+    }
+    __syncthreads();
+    // Phase B: the block's rows in dependency order from the tile; wave w solves columns 2w
+    // (lanes 0..31) and 2w+1 (lanes 32..63), lane l taking tile column l (entries outside the
+    // triangle are zero).
+    {
+        const int cw = 2 * w + (lane >> 5), l = lane & 31;
+        for (int rr = 0; rr < rows; ++rr) {
+            const int li = LOWER ? rr : rows - 1 - rr;
+            zd sacc = zzero();
+            if (l < rows && cw < k) sacc = zmul(D[li][l], xs[l][cw]);
+            for (int o = 16; o > 0; o >>= 1) {
+                sacc.x += __shfl_xor(sacc.x, o);
+                sacc.y += __shfl_xor(sacc.y, o);
+            }
+            if (l == 0 && cw < k) xs[li][cw] = zdiv(zsub(ext[li][cw], sacc), sdg[li]);
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (int idx = tid; idx < rows * 8; idx += 256) {
         const int li = idx % rows, c = idx / rows;

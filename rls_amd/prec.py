@@ -245,10 +245,16 @@ class PrecSolver:
         """n x k column-major complex block."""
         return self.torch.empty((k, self.n), dtype=self.torch.complex128, device=self.dev).T
 
+    def _ld(self, X) -> int:
+        """Leading dimension of a column-major block (a single column has any stride)."""
+        return X.stride(1) if X.shape[1] > 1 else self.n
+
     def _cm(self, X):
-        if X.stride(0) == 1 and X.stride(1) >= self.n:
+        if X.stride(0) == 1 and self._ld(X) >= self.n:
             return X
-        return X.T.contiguous().T
+        Y = self.empty(X.shape[1])
+        Y.copy_(X)
+        return Y
 
     def apply(self, op: int, X, z: complex = 0.0):
         X = self._cm(X)
@@ -256,8 +262,8 @@ class PrecSolver:
         Y = self.empty(k)
         if k:
             self.ctx.check(self.ctx.lib.rls_dev_apply(self.ctx.h, op, float(z.real), float(z.imag),
-                                                      C.c_void_p(X.data_ptr()), X.stride(1), k,
-                                                      C.c_void_p(Y.data_ptr()), Y.stride(1)))
+                                                      C.c_void_p(X.data_ptr()), self._ld(X), k,
+                                                      C.c_void_p(Y.data_ptr()), self._ld(Y)))
         return Y
 
     def sync(self):
