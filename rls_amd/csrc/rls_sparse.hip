@@ -78,7 +78,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     __shared__ zd xs[SP_RB][8];
     __shared__ zd D[SP_RB][SP_RB + 1];
     __shared__ zd ext[SP_RB][8];
-    __shared__ zd sdg[SP_RB];
+    __shared__ zd sdg[SP_RB];   // reciprocal diagonal
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int nblk = (int)((n + SP_RB - 1) / SP_RB);
@@ -88,17 +88,24 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     const int b = LOWER ? t : nblk - 1 - t;
     const int64_t r0 = (int64_t)b * SP_RB;
     const int rows = (int)min((int64_t)SP_RB, n - r0);
-    // wait for the earlier blocks this block reads
-    if (tid == 0) {
+    // wait for the earlier blocks this block reads: wave 0 polls 64 of their flags per step
+    // (one lane per block), the other waves wait at the barrier
+    if (w == 0) {
         const int d = dep[b];
         const int lo = LOWER ? d : b + 1, hi = LOWER ? b : d + 1;
-        for (int q = lo; q < hi && s_ok; ++q) {
-            unsigned int spins = 0;
-            while (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1u << 24)) { s_ok = 0; atomicExch(err, 1); break; }
+        bool ok = true;
+        for (int q0 = lo; q0 < hi; q0 += 64) {
+            const int q = q0 + lane;
+            if (q < hi && ok) {
+                unsigned int spins = 0;
+                while (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1u << 24)) { ok = false; atomicExch(err, 1); break; }
+                }
             }
+            if (__any(!ok)) break;
         }
+        if (!ok) s_ok = 0;
         __threadfence();
     }
     __syncthreads();
@@ -119,14 +126,27 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
         if (li < rows) {
             const int64_t i = r0 + li;
             const int64_t e1 = rp[i + 1];
-            for (int64_t e = rp[i] + sub; e < e1; e += 8) {
-                const int64_t j = ci[e];
-                const zd a = v[e];
-                if (j == i) { dr = a.x; di = a.y; continue; }
-                if (j >= r0 && j < r0 + rows) { D[li][j - r0] = a; continue; }
+            // four entries per step: their index / value loads, then their X loads, are
+            // issued back to back
+            for (int64_t e = rp[i] + sub; e < e1; e += 32) {
+                int64_t jj[4];
+                zd aa[4];
 #pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    if (c < k) acc[c] = zfma(a, X[j + ldx * c], acc[c]);
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t ee = e + 8 * u;
+                    jj[u] = (ee < e1) ? ci[ee] : -1;
+                    aa[u] = (ee < e1) ? v[ee] : zzero();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t j = jj[u];
+                    if (j < 0) continue;
+                    if (j == i) { dr = aa[u].x; di = aa[u].y; continue; }
+                    if (j >= r0 && j < r0 + rows) { D[li][j - r0] = aa[u]; continue; }
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (c < k) acc[c] = zfma(aa[u], X[j + ldx * c], acc[c]);
+                }
             }
         }
         for (int o = 4; o > 0; o >>= 1) {
@@ -142,7 +162,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 ext[li][c] = (c < k) ? zsub(X[(r0 + li) + ldx * c], acc[c]) : zzero();
-            sdg[li] = unit_diag ? zmk(1.0, 0.0) : zmk(dr, di);
+            sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));   // 1 / diagonal
         }
     }
     __syncthreads();
@@ -159,10 +179,13 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                 sacc.x += __shfl_xor(sacc.x, o);
                 sacc.y += __shfl_xor(sacc.y, o);
             }
-            if (l == 0 && cw < k) xs[li][cw] = zdiv(zsub(ext[li][cw], sacc), sdg[li]);
-            __syncthreads();
+            if (l == 0 && cw < k) xs[li][cw] = zmul(zsub(ext[li][cw], sacc), sdg[li]);
+            // a column pair never leaves its wave, and one wave's LDS accesses complete in
+            // order, so only the compiler has to be kept from reordering across rows
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    __syncthreads();
     for (int idx = tid; idx < rows * 8; idx += 256) {
         const int li = idx % rows, c = idx / rows;
         if (c < k) X[(r0 + li) + ldx * c] = xs[li][c];
