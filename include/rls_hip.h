@@ -130,7 +130,7 @@ int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_contr
  * (flops and bytes counted from the launch shapes) is accumulated. */
 #define RLS_KF_GEMM 0     /* batched recycled-Rayleigh-Ritz products (tiled complex GEMM) */
 #define RLS_KF_HEEV 1     /* per-shift Hermitian eigensolve of the projected matrix      */
-#define RLS_KF_TRSM 2     /* shifted triangular solve with the Schur factor               */
+#define RLS_KF_TRSM 2     /* shifted triangular solves: Schur factor, sparse LU or IC factor */
 #define RLS_KF_COUNT 3
 int rls_kernel_timing_enable(rls_ctx* ctx, int enable);
 int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,

@@ -76,6 +76,7 @@ struct rls_ctx {
         DArr<int64_t> rp, ci;
         DArr<zd> v;
         DArr<int> dep;
+        int64_t nnz = 0;
     };
     SpTri fL, fU, fLH, fUH;
     DArr<int64_t> perm_r, perm_c;
@@ -393,14 +394,18 @@ struct rls_ctx {
         }
         for (int c0 = 0; c0 < k; c0 += 8) {
             const int kc = std::min(8, k - c0);
+            kt_begin(RLS_KF_TRSM);
             launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X + (int64_t)n * c0, n, kc,
                           sp_cnt.p, sp_cnt.p + 1, ++sp_epoch, sp_err.p);
+            // algorithmic: the factor once (values, column indices, row pointers) + the block
+            // read and written
+            kt_end(RLS_KF_TRSM, 8.0 * (double)F.nnz * kc,
+                   24.0 * (double)F.nnz + 8.0 * (double)(n + 1) + 32.0 * (double)n * kc);
         }
     }
     // X <- A^{-1} X or A^{-H} X for A = C - zI = Pr^T L U Pc^T (sparse path)
     void sp_apply(zd* X, int k, bool inverse_h) {
         sp_tmp.ensure((size_t)n * k);
-        kt_begin(RLS_KF_TRSM);
         if (!inverse_h) {
             launch_permute_rows(st, n, perm_r.p, X, n, sp_tmp.p, n, k, true);
             sp_solve(fL, true, true, sp_tmp.p, k);
@@ -412,7 +417,6 @@ struct rls_ctx {
             sp_solve(fLH, false, true, sp_tmp.p, k);
             launch_permute_rows(st, n, perm_r.p, sp_tmp.p, n, X, n, k, false);
         }
-        kt_end(RLS_KF_TRSM, 0.0, 0.0);
     }
     void apply_C(zd* X, int k) {    // X <- (M - zS)^{-H} X
         if (sparse) sp_apply(X, k, true);
@@ -1002,6 +1006,7 @@ void upload_tri(rls_ctx* ctx, rls_ctx::SpTri& F, const rls_csr* A, bool lower, c
     upload(ctx, F.ci, A->colind, (size_t)nnz);
     upload(ctx, F.v, reinterpret_cast<const zd*>(A->values), (size_t)nnz);
     upload(ctx, F.dep, dep.data(), (size_t)nb);
+    F.nnz = nnz;
     ctx->sync();   // the copies read the local vector dep
 }
 

@@ -276,6 +276,16 @@ class PrecSolver:
     def to_dev(self, X: np.ndarray):
         return self.torch.from_numpy(np.ascontiguousarray(X, dtype=np.complex128)).to(self.dev)
 
+    def enable_kernel_timing(self, on: bool = True):
+        """HIP-event timing of the triangular-solve launches (family "trsm") with their
+        algorithmic bytes; read with kernel_timing()."""
+        from . import TriSolver
+        TriSolver.enable_kernel_timing(self, on)
+
+    def kernel_timing(self) -> dict:
+        from . import TriSolver
+        return TriSolver.kernel_timing(self)
+
 
 class _Op:
     """Square operator with products mul (A X) and rmul (A^H Y)."""
