@@ -197,7 +197,6 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __shared__ zd pw[1024];
     __shared__ int s_rr;
     __shared__ double s_part[HEEV_THREADS];
-    __shared__ zd s_acc[HEEV_THREADS / 256][256];
 
     if (b == 0 && tid == 0) g_heev_phase[0] = wall_clock64();
     // 1. copy with Hermitian completion from the upper triangle
@@ -253,49 +252,62 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
         const bool has_t = !(t.x == 0.0 && t.y == 0.0);
-        // 2-D split: thread (row gi, column group gg); groups reduce through LDS.  Columns
-        // are taken HB at a time with all HB loads issued before any is used, so a thread
-        // keeps HB memory requests in flight instead of one.
+        // each row of A22 belongs to a pair of adjacent lanes (even / odd columns), combined
+        // with one shuffle: no LDS reduction and no barrier inside the pass.  Columns are
+        // taken HB at a time with all HB loads issued before any is used, so a thread keeps
+        // HB memory requests in flight instead of one (HB = 4 keeps the chunk's operands in
+        // registers without spilling at 1024 threads).
         {
-            const int RG = 256;
-            const int NG = nthr / RG;
-            const int gi = tid % RG, gg = tid / RG;
-            constexpr int HB = 8;
-            for (int i0 = 0; i0 < m; i0 += RG) {
-                const int i = i0 + gi;
+            constexpr int HB = 4;
+            const int h = tid & 1;
+            const int rows_per_pass = nthr >> 1;
+            for (int i0 = 0; i0 < m; i0 += rows_per_pass) {
+                const int i = i0 + (tid >> 1);
                 zd acc0 = zzero(), acc1 = zzero();
                 if (i < m) {
-                    const zd pvi = pending ? pv[i + 1] : zzero();
-                    const zd pwi = pending ? pw[i + 1] : zzero();
-                    for (int jb = gg; jb < m; jb += HB * NG) {
-                        zd a[HB];
+                    // negated so the rank-2 update is two fused complex multiply-adds
+                    const zd npvi = pending ? zmk(-pv[i + 1].x, -pv[i + 1].y) : zzero();
+                    const zd npwi = pending ? zmk(-pw[i + 1].x, -pw[i + 1].y) : zzero();
+                    // full chunks: no bounds checks, every global and LDS load of the chunk
+                    // issued before the arithmetic
+                    int jb = h;
+                    for (; jb + 2 * (HB - 1) < m; jb += 2 * HB) {
+                        zd a[HB], sj[HB], pwj[HB], pvj[HB];
 #pragma unroll
-                        for (int u = 0; u < HB; ++u) {
-                            const int j = jb + u * NG;
-                            a[u] = (j < m) ? A22[i + (int64_t)n * j] : zzero();
+                        for (int u = 0; u < HB; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
+#pragma unroll
+                        for (int u = 0; u < HB; ++u) sj[u] = sv[jb + 2 * u];
+                        if (pending) {
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) {
+                                pwj[u] = pw[jb + 2 * u + 1];
+                                pvj[u] = pv[jb + 2 * u + 1];
+                            }
+#pragma unroll
+                            for (int u = 0; u < HB; ++u)
+                                a[u] = zfma(npwi, zconj(pvj[u]), zfma(npvi, zconj(pwj[u]), a[u]));
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) A22[i + (int64_t)n * (jb + 2 * u)] = a[u];
                         }
 #pragma unroll
-                        for (int u = 0; u < HB; ++u) {
-                            const int j = jb + u * NG;
-                            if (j < m) {
-                                if (pending) {
-                                    a[u] = zsub(a[u], zadd(zmul(pvi, zconj(pw[j + 1])), zmul(pwi, zconj(pv[j + 1]))));
-                                    A22[i + (int64_t)n * j] = a[u];
-                                }
-                                if (u & 1) acc1 = zfma(a[u], sv[j], acc1);
-                                else acc0 = zfma(a[u], sv[j], acc0);
-                            }
+                        for (int u = 0; u < HB; u += 2) {
+                            acc0 = zfma(a[u], sj[u], acc0);
+                            acc1 = zfma(a[u + 1], sj[u + 1], acc1);
                         }
                     }
+                    for (; jb < m; jb += 2) {
+                        zd a = A22[i + (int64_t)n * jb];
+                        if (pending) {
+                            a = zfma(npwi, zconj(pv[jb + 1]), zfma(npvi, zconj(pw[jb + 1]), a));
+                            A22[i + (int64_t)n * jb] = a;
+                        }
+                        acc0 = zfma(a, sv[jb], acc0);
+                    }
                 }
-                s_acc[gg][gi] = zadd(acc0, acc1);
-                __syncthreads();
-                if (gg == 0 && i < m) {
-                    zd sacc = s_acc[0][gi];
-                    for (int g2 = 1; g2 < NG; ++g2) sacc = zadd(sacc, s_acc[g2][gi]);
-                    sw[i] = zmul(t, sacc);
-                }
-                __syncthreads();
+                zd acc = zadd(acc0, acc1);
+                acc.x += __shfl_xor(acc.x, 1);
+                acc.y += __shfl_xor(acc.y, 1);
+                if (h == 0 && i < m) sw[i] = zmul(t, acc);
             }
         }
         __syncthreads();
