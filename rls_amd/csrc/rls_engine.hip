@@ -392,16 +392,14 @@ struct rls_ctx {
             HIP_TRY(hipMemsetAsync(sp_err.p, 0, sizeof(int), st));
             sp_epoch = 0;
         }
-        for (int c0 = 0; c0 < k; c0 += 8) {
-            const int kc = std::min(8, k - c0);
-            kt_begin(RLS_KF_TRSM);
-            launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X + (int64_t)n * c0, n, kc,
-                          sp_cnt.p, sp_cnt.p + 1, ++sp_epoch, sp_err.p);
-            // algorithmic: the factor once (values, column indices, row pointers) + the block
-            // read and written
-            kt_end(RLS_KF_TRSM, 8.0 * (double)F.nnz * kc,
-                   24.0 * (double)F.nnz + 8.0 * (double)(n + 1) + 32.0 * (double)n * kc);
-        }
+        kt_begin(RLS_KF_TRSM);
+        launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, n, k, sp_cnt.p, sp_cnt.p + 1,
+                      ++sp_epoch, sp_err.p);
+        // algorithmic: the factor once per 8-column chunk (values, column indices, row
+        // pointers) + the block read and written
+        const double chunks = (double)((k + 7) / 8);
+        kt_end(RLS_KF_TRSM, 8.0 * (double)F.nnz * k,
+               chunks * (24.0 * (double)F.nnz + 8.0 * (double)(n + 1)) + 32.0 * (double)n * k);
     }
     // X <- A^{-1} X or A^{-H} X for A = C - zI = Pr^T L U Pc^T (sparse path)
     void sp_apply(zd* X, int k, bool inverse_h) {

@@ -116,7 +116,8 @@ void launch_csr_spmm(hipStream_t st, int64_t n, const int64_t* rp, const int64_t
 // Y[i] = X[p[i]] (gather) or Y[p[i]] = X[i] (scatter), k columns
 void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t ldx, zd* Y,
                          int64_t ldy, int k, bool scatter);
-// X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), k <= 8.
+// X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), any k (in
+// chunks of 8 columns inside one launch).
 // dep[b] (b = block of sptrsv_block_rows() rows): lower -> the first block any row of b
 // references, upper -> the last.  counter / flags (one per block) start at zero; epoch grows
 // by one per solve; *err is raised if a wait overran its bound.

@@ -110,87 +110,92 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     __syncthreads();
     if (!s_ok) return;
-    // Phase A, all rows of the block at once (8 threads per row): the row's sum over the
-    // entries outside the block (those solution rows are final), subtracted from the
-    // right-hand side, and the diagonal; the entries inside the block go to a dense tile D
-    // (row li, column j - r0).  Rows are sorted with distinct columns (checked on upload).
-    for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
-    xs[tid >> 3][tid & 7] = zzero();
-    __syncthreads();
-    {
-        const int li = tid >> 3, sub = tid & 7;
-        zd acc[8];
+    // the right-hand sides in chunks of 8 columns, all after the one dependency wait
+    for (int c0 = 0; c0 < k; c0 += 8) {
+        const int kc = min(8, k - c0);
+        zd* Xc = X + ldx * c0;
+        // Phase A, all rows of the block at once (8 threads per row): the row's sum over the
+        // entries outside the block (those solution rows are final), subtracted from the
+        // right-hand side, and the diagonal; the entries inside the block go to a dense tile D
+        // (row li, column j - r0).  Rows are sorted with distinct columns (checked on upload).
+        for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
+        xs[tid >> 3][tid & 7] = zzero();
+        __syncthreads();
+        {
+            const int li = tid >> 3, sub = tid & 7;
+            zd acc[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = zzero();
-        double dr = 0.0, di = 0.0;
-        if (li < rows) {
-            const int64_t i = r0 + li;
-            const int64_t e1 = rp[i + 1];
-            // four entries per step: their index / value loads, then their X loads, are
-            // issued back to back
-            for (int64_t e = rp[i] + sub; e < e1; e += 32) {
-                int64_t jj[4];
-                zd aa[4];
+            for (int c = 0; c < 8; ++c) acc[c] = zzero();
+            double dr = 0.0, di = 0.0;
+            if (li < rows) {
+                const int64_t i = r0 + li;
+                const int64_t e1 = rp[i + 1];
+                // four entries per step: their index / value loads, then their X loads, are
+                // issued back to back
+                for (int64_t e = rp[i] + sub; e < e1; e += 32) {
+                    int64_t jj[4];
+                    zd aa[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t ee = e + 8 * u;
-                    jj[u] = (ee < e1) ? ci[ee] : -1;
-                    aa[u] = (ee < e1) ? v[ee] : zzero();
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t ee = e + 8 * u;
+                        jj[u] = (ee < e1) ? ci[ee] : -1;
+                        aa[u] = (ee < e1) ? v[ee] : zzero();
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t j = jj[u];
+                        if (j < 0) continue;
+                        if (j == i) { dr = aa[u].x; di = aa[u].y; continue; }
+                        if (j >= r0 && j < r0 + rows) { D[li][j - r0] = aa[u]; continue; }
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                            if (c < kc) acc[c] = zfma(aa[u], Xc[j + ldx * c], acc[c]);
+                    }
                 }
+            }
+            for (int o = 4; o > 0; o >>= 1) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t j = jj[u];
-                    if (j < 0) continue;
-                    if (j == i) { dr = aa[u].x; di = aa[u].y; continue; }
-                    if (j >= r0 && j < r0 + rows) { D[li][j - r0] = aa[u]; continue; }
-#pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        if (c < k) acc[c] = zfma(aa[u], X[j + ldx * c], acc[c]);
+                for (int c = 0; c < 8; ++c) {
+                    acc[c].x += __shfl_xor(acc[c].x, o);
+                    acc[c].y += __shfl_xor(acc[c].y, o);
                 }
+                dr += __shfl_xor(dr, o);
+                di += __shfl_xor(di, o);
             }
-        }
-        for (int o = 4; o > 0; o >>= 1) {
+            if (sub == 0 && li < rows) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                acc[c].x += __shfl_xor(acc[c].x, o);
-                acc[c].y += __shfl_xor(acc[c].y, o);
+                for (int c = 0; c < 8; ++c)
+                    ext[li][c] = (c < kc) ? zsub(Xc[(r0 + li) + ldx * c], acc[c]) : zzero();
+                sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));   // 1 / diagonal
             }
-            dr += __shfl_xor(dr, o);
-            di += __shfl_xor(di, o);
         }
-        if (sub == 0 && li < rows) {
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                ext[li][c] = (c < k) ? zsub(X[(r0 + li) + ldx * c], acc[c]) : zzero();
-            sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));   // 1 / diagonal
-        }
-    }
-    __syncthreads();
-    // Phase B: the block's rows in dependency order from the tile; wave w solves columns 2w
-    // (lanes 0..31) and 2w+1 (lanes 32..63), lane l taking tile column l (entries outside the
-    // triangle are zero).
-    {
-        const int cw = 2 * w + (lane >> 5), l = lane & 31;
-        for (int rr = 0; rr < rows; ++rr) {
-            const int li = LOWER ? rr : rows - 1 - rr;
-            zd sacc = zzero();
-            if (l < rows && cw < k) sacc = zmul(D[li][l], xs[l][cw]);
-            for (int o = 16; o > 0; o >>= 1) {
-                sacc.x += __shfl_xor(sacc.x, o);
-                sacc.y += __shfl_xor(sacc.y, o);
+        __syncthreads();
+        // Phase B: the block's rows in dependency order from the tile; wave w solves columns 2w
+        // (lanes 0..31) and 2w+1 (lanes 32..63), lane l taking tile column l (entries outside the
+        // triangle are zero).
+        {
+            const int cw = 2 * w + (lane >> 5), l = lane & 31;
+            for (int rr = 0; rr < rows; ++rr) {
+                const int li = LOWER ? rr : rows - 1 - rr;
+                zd sacc = zzero();
+                if (l < rows && cw < kc) sacc = zmul(D[li][l], xs[l][cw]);
+                for (int o = 16; o > 0; o >>= 1) {
+                    sacc.x += __shfl_xor(sacc.x, o);
+                    sacc.y += __shfl_xor(sacc.y, o);
+                }
+                if (l == 0 && cw < kc) xs[li][cw] = zmul(zsub(ext[li][cw], sacc), sdg[li]);
+                // a column pair never leaves its wave, and one wave's LDS accesses complete in
+                // order, so only the compiler has to be kept from reordering across rows
+                __builtin_amdgcn_wave_barrier();
             }
-            if (l == 0 && cw < k) xs[li][cw] = zmul(zsub(ext[li][cw], sacc), sdg[li]);
-            // a column pair never leaves its wave, and one wave's LDS accesses complete in
-            // order, so only the compiler has to be kept from reordering across rows
-            __builtin_amdgcn_wave_barrier();
         }
+        __syncthreads();
+        for (int idx = tid; idx < rows * 8; idx += 256) {
+            const int li = idx % rows, c = idx / rows;
+            if (c < kc) Xc[(r0 + li) + ldx * c] = xs[li][c];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int idx = tid; idx < rows * 8; idx += 256) {
-        const int li = idx % rows, c = idx / rows;
-        if (c < k) X[(r0 + li) + ldx * c] = xs[li][c];
-    }
-    __syncthreads();
     if (tid == 0) {
         __threadfence();
         __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -226,7 +231,7 @@ int sptrsv_block_rows() { return SP_RB; }
 void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
                    bool unit_diag, const int* dep, zd* X, int64_t ldx, int k, unsigned int* counter,
                    unsigned int* flags, unsigned int epoch, int* err) {
-    if (k > 8) throw RlsError{-1, "sptrsv: k > 8 unsupported"};
+    if (k <= 0) return;
     const int nblk = (int)cdiv(n, SP_RB);
     if (lower)
         hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(256), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
