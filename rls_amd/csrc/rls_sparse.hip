@@ -61,12 +61,17 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 
 // Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
 // ascending order, upper: descending).  One workgroup per block of SP_RB rows, started in
-// dependency order by a ticket counter.  A block first waits for every earlier block its
-// rows reference (host analysis: dep_lo[b] for lower / dep_hi[b] for upper), then forms all
-// its rows' sums over earlier blocks in parallel and solves the small triangle of in-block
-// entries from LDS, one wave per right-hand-side pair.  The wait is bounded: on overrun
-// *err is raised and the workgroup exits.
+// dependency order by a ticket counter.  Everything that does not depend on other blocks is
+// done before the block's dependency wait, which a block started early spends idle: each
+// row's entries are staged in LDS (8 threads per row, SP_SLOTS entries per thread), the
+// entries inside the block go to a dense tile D and the reciprocal diagonal is formed.  Then
+// the block waits for every earlier block its rows reference (host analysis: dep_lo[b] for
+// lower / dep_hi[b] for upper), forms all its rows' sums over earlier blocks in parallel
+// (only the loads of X remain; entries beyond the staged slots are read from the factor)
+// and solves the small in-block triangle from LDS, one wave per right-hand-side pair.  The
+// wait is bounded: on overrun *err is raised and the workgroup exits.
 constexpr int SP_RB = 32;
+constexpr int SP_SLOTS = 16;   // staged entries per thread (128 per row)
 
 template <bool LOWER>
 __global__ void __launch_bounds__(256)
@@ -78,16 +83,55 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     __shared__ zd xs[SP_RB][8];
     __shared__ zd D[SP_RB][SP_RB + 1];
     __shared__ zd ext[SP_RB][8];
-    __shared__ zd sdg[SP_RB];   // reciprocal diagonal
+    __shared__ zd sdg[SP_RB];                 // reciprocal diagonal
+    __shared__ int s_j[SP_SLOTS][256];        // staged out-of-block column (-1: none)
+    __shared__ zd s_a[SP_SLOTS][256];         // and its value
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int nblk = (int)((n + SP_RB - 1) / SP_RB);
     if (tid == 0) { s_ticket = atomicAdd(counter, 1u); s_ok = 1; }
+    for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
     __syncthreads();
     const int t = (int)(s_ticket % (unsigned)nblk);
     const int b = LOWER ? t : nblk - 1 - t;
     const int64_t r0 = (int64_t)b * SP_RB;
     const int rows = (int)min((int64_t)SP_RB, n - r0);
+    const int li = tid >> 3, sub = tid & 7;   // row of the block and entry phase of this thread
+    const int64_t ri = r0 + li;
+    int64_t e_over = 0, e_end = 0;            // entries of this thread left for after the wait
+    // staging (independent of other blocks)
+    {
+        double dr = 0.0, di = 0.0;
+        int slot = 0;
+        if (li < rows) {
+            const int64_t e1 = rp[ri + 1];
+            int64_t e = rp[ri] + sub;
+            for (; e < e1 && slot < SP_SLOTS; e += 8) {
+                const int64_t j = ci[e];
+                const zd a = v[e];
+                if (j == ri) { dr = a.x; di = a.y; continue; }
+                if (j >= r0 && j < r0 + rows) { D[li][j - r0] = a; continue; }
+                s_j[slot][tid] = (int)j;
+                s_a[slot][tid] = a;
+                ++slot;
+            }
+            // entries past the staged slots: the diagonal and in-block ones are placed now,
+            // the others read again after the wait
+            e_over = e;
+            e_end = e1;
+            for (; e < e1; e += 8) {
+                const int64_t j = ci[e];
+                if (j == ri) { dr = v[e].x; di = v[e].y; }
+                else if (j >= r0 && j < r0 + rows) D[li][j - r0] = v[e];
+            }
+        }
+        for (int q = slot; q < SP_SLOTS; ++q) s_j[q][tid] = -1;
+        for (int o = 4; o > 0; o >>= 1) {
+            dr += __shfl_xor(dr, o);
+            di += __shfl_xor(di, o);
+        }
+        if (sub == 0 && li < rows) sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));
+    }
     // wait for the earlier blocks this block reads: wave 0 polls 64 of their flags per step
     // (one lane per block), the other waves wait at the barrier
     if (w == 0) {
@@ -110,48 +154,40 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     __syncthreads();
     if (!s_ok) return;
-    // the right-hand sides in chunks of 8 columns, all after the one dependency wait
+    // the right-hand sides in chunks of 8 columns
     for (int c0 = 0; c0 < k; c0 += 8) {
         const int kc = min(8, k - c0);
         zd* Xc = X + ldx * c0;
-        // Phase A, all rows of the block at once (8 threads per row): the row's sum over the
-        // entries outside the block (those solution rows are final), subtracted from the
-        // right-hand side, and the diagonal; the entries inside the block go to a dense tile D
-        // (row li, column j - r0).  Rows are sorted with distinct columns (checked on upload).
-        for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
         xs[tid >> 3][tid & 7] = zzero();
-        __syncthreads();
+        // all rows' sums over earlier blocks: staged entries two at a time (their X loads in
+        // flight together), then any entries past the staged slots
         {
-            const int li = tid >> 3, sub = tid & 7;
             zd acc[8];
 #pragma unroll
             for (int c = 0; c < 8; ++c) acc[c] = zzero();
-            double dr = 0.0, di = 0.0;
-            if (li < rows) {
-                const int64_t i = r0 + li;
-                const int64_t e1 = rp[i + 1];
-                // four entries per step: their index / value loads, then their X loads, are
-                // issued back to back
-                for (int64_t e = rp[i] + sub; e < e1; e += 32) {
-                    int64_t jj[4];
-                    zd aa[4];
+            for (int q = 0; q < SP_SLOTS; q += 2) {
+                const int j0 = s_j[q][tid], j1 = s_j[q + 1][tid];
+                if (j0 < 0) break;
+                const zd a0 = s_a[q][tid];
+                const zd a1 = (j1 >= 0) ? s_a[q + 1][tid] : zzero();
+                const int jj1 = (j1 >= 0) ? j1 : j0;
+                zd x0[8], x1[8];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int64_t ee = e + 8 * u;
-                        jj[u] = (ee < e1) ? ci[ee] : -1;
-                        aa[u] = (ee < e1) ? v[ee] : zzero();
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int64_t j = jj[u];
-                        if (j < 0) continue;
-                        if (j == i) { dr = aa[u].x; di = aa[u].y; continue; }
-                        if (j >= r0 && j < r0 + rows) { D[li][j - r0] = aa[u]; continue; }
-#pragma unroll
-                        for (int c = 0; c < 8; ++c)
-                            if (c < kc) acc[c] = zfma(aa[u], Xc[j + ldx * c], acc[c]);
-                    }
+                for (int c = 0; c < 8; ++c) {
+                    x0[c] = (c < kc) ? Xc[j0 + ldx * c] : zzero();
+                    x1[c] = (c < kc) ? Xc[jj1 + ldx * c] : zzero();
                 }
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[c] = zfma(a1, x1[c], zfma(a0, x0[c], acc[c]));
+                if (j1 < 0) break;
+            }
+            for (int64_t e = e_over; e < e_end; e += 8) {
+                const int64_t j = ci[e];
+                if (j == ri || (j >= r0 && j < r0 + rows)) continue;
+                const zd a = v[e];
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < kc) acc[c] = zfma(a, Xc[j + ldx * c], acc[c]);
             }
             for (int o = 4; o > 0; o >>= 1) {
 #pragma unroll
@@ -159,40 +195,36 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                     acc[c].x += __shfl_xor(acc[c].x, o);
                     acc[c].y += __shfl_xor(acc[c].y, o);
                 }
-                dr += __shfl_xor(dr, o);
-                di += __shfl_xor(di, o);
             }
             if (sub == 0 && li < rows) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
-                    ext[li][c] = (c < kc) ? zsub(Xc[(r0 + li) + ldx * c], acc[c]) : zzero();
-                sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));   // 1 / diagonal
+                    ext[li][c] = (c < kc) ? zsub(Xc[ri + ldx * c], acc[c]) : zzero();
             }
         }
         __syncthreads();
-        // Phase B: the block's rows in dependency order from the tile; wave w solves columns 2w
-        // (lanes 0..31) and 2w+1 (lanes 32..63), lane l taking tile column l (entries outside the
-        // triangle are zero).
+        // the in-block triangle, column by column: wave w solves columns 2w (lanes 0..31) and
+        // 2w+1 (lanes 32..63); lane r keeps row r's running right-hand side in a register, the
+        // newly solved unknown is broadcast with one shuffle and every later row subtracts
+        // its tile entry times it (entries outside the triangle are zero)
         {
-            const int cw = 2 * w + (lane >> 5), l = lane & 31;
-            for (int rr = 0; rr < rows; ++rr) {
-                const int li = LOWER ? rr : rows - 1 - rr;
-                zd sacc = zzero();
-                if (l < rows && cw < kc) sacc = zmul(D[li][l], xs[l][cw]);
-                for (int o = 16; o > 0; o >>= 1) {
-                    sacc.x += __shfl_xor(sacc.x, o);
-                    sacc.y += __shfl_xor(sacc.y, o);
-                }
-                if (l == 0 && cw < kc) xs[li][cw] = zmul(zsub(ext[li][cw], sacc), sdg[li]);
-                // a column pair never leaves its wave, and one wave's LDS accesses complete in
-                // order, so only the compiler has to be kept from reordering across rows
-                __builtin_amdgcn_wave_barrier();
+            const int cw = 2 * w + (lane >> 5), r = lane & 31;
+            zd er = (r < rows && cw < kc) ? ext[r][cw] : zzero();
+            for (int q = 0; q < rows; ++q) {
+                const int l = LOWER ? q : rows - 1 - q;
+                zd xl;
+                xl.x = __shfl(er.x, l, 32);
+                xl.y = __shfl(er.y, l, 32);
+                xl = zmul(xl, sdg[l]);
+                if (r == l) er = xl;
+                else if (LOWER ? (r > l) : (r < l)) er = zsub(er, zmul(D[r][l], xl));
             }
+            if (r < rows && cw < kc) xs[r][cw] = er;
         }
         __syncthreads();
         for (int idx = tid; idx < rows * 8; idx += 256) {
-            const int li = idx % rows, c = idx / rows;
-            if (c < kc) Xc[(r0 + li) + ldx * c] = xs[li][c];
+            const int lr = idx % rows, c = idx / rows;
+            if (c < kc) Xc[(r0 + lr) + ldx * c] = xs[lr][c];
         }
         __syncthreads();
     }
