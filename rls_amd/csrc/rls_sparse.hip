@@ -63,7 +63,7 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 // ascending order, upper: descending).  One workgroup per block of SP_RB rows, started in
 // dependency order by a ticket counter.  Everything that does not depend on other blocks is
 // done before the block's dependency wait, which a block started early spends idle: each
-// row's entries are staged in LDS (8 threads per row, SP_SLOTS entries per thread), the
+// row's entries are staged in LDS (SP_TPR threads per row, SP_SLOTS entries each), the
 // entries inside the block go to a dense tile D and the reciprocal diagonal is formed.  Then
 // the block waits for every earlier block its rows reference (host analysis: dep_lo[b] for
 // lower / dep_hi[b] for upper), forms all its rows' sums over earlier blocks in parallel
@@ -71,10 +71,12 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 // and solves the small in-block triangle from LDS, one wave per right-hand-side pair.  The
 // wait is bounded: on overrun *err is raised and the workgroup exits.
 constexpr int SP_RB = 32;
-constexpr int SP_SLOTS = 16;   // staged entries per thread (128 per row)
+constexpr int SP_THREADS = 512;                // workgroup size
+constexpr int SP_TPR = SP_THREADS / SP_RB;      // threads per row (16)
+constexpr int SP_SLOTS = 8;                     // staged entries per thread (128 per row)
 
 template <bool LOWER>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SP_THREADS)
 k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
          const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t ldx,
          int k, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
@@ -84,19 +86,19 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     __shared__ zd D[SP_RB][SP_RB + 1];
     __shared__ zd ext[SP_RB][8];
     __shared__ zd sdg[SP_RB];                 // reciprocal diagonal
-    __shared__ int s_j[SP_SLOTS][256];        // staged out-of-block column (-1: none)
-    __shared__ zd s_a[SP_SLOTS][256];         // and its value
+    __shared__ int s_j[SP_SLOTS][SP_THREADS];   // staged out-of-block column (-1: none)
+    __shared__ zd s_a[SP_SLOTS][SP_THREADS];    // and its value
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int nblk = (int)((n + SP_RB - 1) / SP_RB);
     if (tid == 0) { s_ticket = atomicAdd(counter, 1u); s_ok = 1; }
-    for (int q = tid; q < SP_RB * (SP_RB + 1); q += 256) (&D[0][0])[q] = zzero();
+    for (int q = tid; q < SP_RB * (SP_RB + 1); q += SP_THREADS) (&D[0][0])[q] = zzero();
     __syncthreads();
     const int t = (int)(s_ticket % (unsigned)nblk);
     const int b = LOWER ? t : nblk - 1 - t;
     const int64_t r0 = (int64_t)b * SP_RB;
     const int rows = (int)min((int64_t)SP_RB, n - r0);
-    const int li = tid >> 3, sub = tid & 7;   // row of the block and entry phase of this thread
+    const int li = tid / SP_TPR, sub = tid % SP_TPR;   // row of the block, entry phase
     const int64_t ri = r0 + li;
     int64_t e_over = 0, e_end = 0;            // entries of this thread left for after the wait
     // staging (independent of other blocks)
@@ -106,7 +108,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
         if (li < rows) {
             const int64_t e1 = rp[ri + 1];
             int64_t e = rp[ri] + sub;
-            for (; e < e1 && slot < SP_SLOTS; e += 8) {
+            for (; e < e1 && slot < SP_SLOTS; e += SP_TPR) {
                 const int64_t j = ci[e];
                 const zd a = v[e];
                 if (j == ri) { dr = a.x; di = a.y; continue; }
@@ -119,14 +121,14 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             // the others read again after the wait
             e_over = e;
             e_end = e1;
-            for (; e < e1; e += 8) {
+            for (; e < e1; e += SP_TPR) {
                 const int64_t j = ci[e];
                 if (j == ri) { dr = v[e].x; di = v[e].y; }
                 else if (j >= r0 && j < r0 + rows) D[li][j - r0] = v[e];
             }
         }
         for (int q = slot; q < SP_SLOTS; ++q) s_j[q][tid] = -1;
-        for (int o = 4; o > 0; o >>= 1) {
+        for (int o = SP_TPR / 2; o > 0; o >>= 1) {
             dr += __shfl_xor(dr, o);
             di += __shfl_xor(di, o);
         }
@@ -158,7 +160,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     for (int c0 = 0; c0 < k; c0 += 8) {
         const int kc = min(8, k - c0);
         zd* Xc = X + ldx * c0;
-        xs[tid >> 3][tid & 7] = zzero();
+        if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
         // all rows' sums over earlier blocks: staged entries two at a time (their X loads in
         // flight together), then any entries past the staged slots
         {
@@ -181,7 +183,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                 for (int c = 0; c < 8; ++c) acc[c] = zfma(a1, x1[c], zfma(a0, x0[c], acc[c]));
                 if (j1 < 0) break;
             }
-            for (int64_t e = e_over; e < e_end; e += 8) {
+            for (int64_t e = e_over; e < e_end; e += SP_TPR) {
                 const int64_t j = ci[e];
                 if (j == ri || (j >= r0 && j < r0 + rows)) continue;
                 const zd a = v[e];
@@ -189,7 +191,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                 for (int c = 0; c < 8; ++c)
                     if (c < kc) acc[c] = zfma(a, Xc[j + ldx * c], acc[c]);
             }
-            for (int o = 4; o > 0; o >>= 1) {
+            for (int o = SP_TPR / 2; o > 0; o >>= 1) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     acc[c].x += __shfl_xor(acc[c].x, o);
@@ -203,11 +205,11 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             }
         }
         __syncthreads();
-        // the in-block triangle, column by column: wave w solves columns 2w (lanes 0..31) and
-        // 2w+1 (lanes 32..63); lane r keeps row r's running right-hand side in a register, the
+        // the in-block triangle, column by column: wave w < 4 solves columns 2w (lanes 0..31)
+        // and 2w+1 (lanes 32..63); lane r keeps row r's running right-hand side in a register, the
         // newly solved unknown is broadcast with one shuffle and every later row subtracts
         // its tile entry times it (entries outside the triangle are zero)
-        {
+        if (w < 4) {
             const int cw = 2 * w + (lane >> 5), r = lane & 31;
             zd er = (r < rows && cw < kc) ? ext[r][cw] : zzero();
             for (int q = 0; q < rows; ++q) {
@@ -222,7 +224,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             if (r < rows && cw < kc) xs[r][cw] = er;
         }
         __syncthreads();
-        for (int idx = tid; idx < rows * 8; idx += 256) {
+        for (int idx = tid; idx < rows * 8; idx += SP_THREADS) {
             const int lr = idx % rows, c = idx / rows;
             if (c < kc) Xc[(r0 + lr) + ldx * c] = xs[lr][c];
         }
@@ -266,10 +268,10 @@ void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* 
     if (k <= 0) return;
     const int nblk = (int)cdiv(n, SP_RB);
     if (lower)
-        hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(256), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
+        hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
                            ldx, k, counter, flags, epoch, err);
     else
-        hipLaunchKernelGGL(k_sptrsv<false>, dim3(nblk), dim3(256), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
+        hipLaunchKernelGGL(k_sptrsv<false>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
                            ldx, k, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
