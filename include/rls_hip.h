@@ -228,6 +228,17 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
 /* Wait for the context's stream and report a triangular-solve wait that exceeded its bound. */
 int rls_dev_sync(rls_ctx* ctx);
 
+/* Dense block products on device blocks with the context's kernels, enqueued on its stream
+ * (the tall-skinny contractions of fastRaylRitz.jl:382-427, recySpace.jl:344-453 and the
+ * LOBPCG basis updates, lobpcgSvd.jl:195-220, whose reference calls are BLAS mul!):
+ *   rls_dev_gemm: Y (n x N) = beta Y + [A_0 .. A_{nops-1}] (n x sum K) * B (sum K x N),
+ *                 1 <= nops <= 4, n = the context's order;
+ *   rls_dev_gram: C (ka x kb) = A^H B for A (n x ka), B (n x kb). */
+int rls_dev_gemm(rls_ctx* ctx, int nops, const double* const* A, const int64_t* lda, const int32_t* K,
+                 const double* B, int64_t ldb, int N, double beta, double* Y, int64_t ldy);
+int rls_dev_gram(rls_ctx* ctx, const double* A, int64_t lda, int ka, const double* B, int64_t ldb, int kb,
+                 double* C, int64_t ldc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,8 @@ EXPORTED = [
     "rls_set_stream",
     "rls_dev_apply",
     "rls_dev_sync",
+    "rls_dev_gemm",
+    "rls_dev_gram",
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
@@ -166,7 +168,9 @@ def load() -> C.CDLL:
                        ("rls_set_ic", [vp, vp]),
                        ("rls_set_stream", [vp, vp]),
                        ("rls_dev_apply", [vp, C.c_int, C.c_double, C.c_double, vp, i64, C.c_int, vp, i64]),
-                       ("rls_dev_sync", [vp])):
+                       ("rls_dev_sync", [vp]),
+                       ("rls_dev_gemm", [vp, C.c_int, vp, vp, vp, vp, i64, C.c_int, C.c_double, vp, i64]),
+                       ("rls_dev_gram", [vp, vp, i64, C.c_int, vp, i64, C.c_int, vp, i64])):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_int

@@ -1308,6 +1308,49 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
     });
 }
 
+int rls_dev_gemm(rls_ctx* ctx, int nops, const double* const* A, const int64_t* lda, const int32_t* K,
+                 const double* Bd, int64_t ldb, int N, double beta, double* Yd, int64_t ldy) {
+    return guarded(ctx, [&]() -> int {
+        const int64_t n = ctx->n;
+        if (n <= 0) throw RlsError{RLS_E_STATE, "rls_dev_gemm: no operator set"};
+        if (nops < 1 || nops > 4 || !A || !lda || !K || N < 0 || (N > 0 && (!Bd || !Yd)) || ldy < n)
+            throw RlsError{RLS_E_ARG, "rls_dev_gemm: bad arguments"};
+        TallOperands ops{};
+        int Ktot = 0;
+        for (int q = 0; q < nops; ++q) {
+            if (!A[q] || lda[q] < n || K[q] < 0) throw RlsError{RLS_E_ARG, "rls_dev_gemm: bad operand"};
+            ops.A[q] = reinterpret_cast<const zd*>(A[q]);
+            ops.lda[q] = lda[q];
+            ops.K[q] = K[q];
+            Ktot += K[q];
+        }
+        ops.nops = nops;
+        if (ldb < std::max(Ktot, 1)) throw RlsError{RLS_E_ARG, "rls_dev_gemm: ldb < sum K"};
+        if (N == 0) return RLS_OK;
+        HIP_TRY(hipSetDevice(ctx->device));
+        const zd* B = reinterpret_cast<const zd*>(Bd);
+        zd* Y = reinterpret_cast<zd*>(Yd);
+        if (beta == 0.0 && N >= 32) launch_zgemm_tiled(ctx->st, n, ops, B, ldb, N, Y, ldy);
+        else launch_tall_gemm(ctx->st, n, ops, B, ldb, 0, N, Y, ldy, 0, beta, 1);
+        return RLS_OK;
+    });
+}
+
+int rls_dev_gram(rls_ctx* ctx, const double* Ad, int64_t lda, int ka, const double* Bd, int64_t ldb, int kb,
+                 double* Cd, int64_t ldc) {
+    return guarded(ctx, [&]() -> int {
+        const int64_t n = ctx->n;
+        if (n <= 0) throw RlsError{RLS_E_STATE, "rls_dev_gram: no operator set"};
+        if (ka < 0 || kb < 0 || lda < n || ldb < n || ldc < std::max(ka, 1) || (ka && kb && (!Ad || !Bd || !Cd)))
+            throw RlsError{RLS_E_ARG, "rls_dev_gram: bad arguments"};
+        if (ka == 0 || kb == 0) return RLS_OK;
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->gram_op(reinterpret_cast<const zd*>(Ad), lda, ka, reinterpret_cast<const zd*>(Bd), ldb, kb,
+                     reinterpret_cast<zd*>(Cd), ldc);
+        return RLS_OK;
+    });
+}
+
 int rls_dev_sync(rls_ctx* ctx) {
     return guarded(ctx, [&]() -> int {
         HIP_TRY(hipSetDevice(ctx->device));

@@ -295,20 +295,70 @@ class _Op:
         self.coupled, self.left = coupled, left
 
 
+# ---- tall-skinny products on the library's kernels ----
+class _Blas:
+    """Products of n-row device blocks on the library's kernels (rls_dev_gemm: the FP64-MFMA
+    GEMM or the tall GEMM; rls_dev_gram: split-row Grams), in the place of BLAS mul! on
+    tall-skinny shapes."""
+
+    def __init__(self, ps: "PrecSolver"):
+        self.ps = ps
+        self.lib = ps.ctx.lib
+
+    def _small_cm(self, B):
+        """Column-major copy of a small matrix and its leading dimension."""
+        if B.stride(0) == 1 and (B.shape[1] <= 1 or B.stride(1) >= B.shape[0]):
+            return B, max(B.stride(1) if B.shape[1] > 1 else B.shape[0], 1)
+        Bc = B.T.contiguous().T
+        return Bc, max(B.shape[0], 1)
+
+    def mm(self, As, B):
+        """[A_0 .. A_q] (n x sum K) @ B (sum K x N) -> n x N."""
+        ps = self.ps
+        As = [ps._cm(A) for A in As if A.shape[1] > 0]
+        N = B.shape[1]
+        Y = ps.empty(N)
+        if N == 0:
+            return Y
+        if not As:                       # an empty basis (e.g. no old blocks): the product is 0
+            return Y.zero_()
+        if sum(A.shape[1] for A in As) != B.shape[0]:
+            raise ValueError("inner dimensions of the block product do not match")
+        Bc, ldb = self._small_cm(B)
+        q = len(As)
+        ptrs = (C.c_void_p * q)(*[A.data_ptr() for A in As])
+        lda = (C.c_int64 * q)(*[ps._ld(A) for A in As])
+        K = (C.c_int32 * q)(*[A.shape[1] for A in As])
+        ps.ctx.check(self.lib.rls_dev_gemm(ps.ctx.h, q, ptrs, lda, K, C.c_void_p(Bc.data_ptr()), ldb, N, 0.0,
+                                           C.c_void_p(Y.data_ptr()), ps._ld(Y)))
+        return Y
+
+    def gram(self, A, B):
+        """A^H B for n-row blocks A (n x ka), B (n x kb) -> ka x kb."""
+        import torch
+        ps = self.ps
+        A, B = ps._cm(A), ps._cm(B)
+        ka, kb = A.shape[1], B.shape[1]
+        Cm = torch.empty((kb, ka), dtype=torch.complex128, device=ps.dev).T
+        if ka and kb:
+            ps.ctx.check(self.lib.rls_dev_gram(ps.ctx.h, C.c_void_p(A.data_ptr()), ps._ld(A), ka,
+                                               C.c_void_p(B.data_ptr()), ps._ld(B), kb,
+                                               C.c_void_p(Cm.data_ptr()), max(ka, 1)))
+        return Cm
+
+
+_LA: Optional[_Blas] = None   # set by the driver for the solve in progress
+
+
 # ---- small dense linear algebra on the device (RLSLinearAlgebra.jl counterparts) ----
-def _qr(X):
-    import torch
-    Q, R = torch.linalg.qr(X, mode="reduced")
-    return Q, R
-
-
 def _qr_shiftchol(X, itmax: int = 3):
-    """Shifted Cholesky-QR, <= itmax passes (RLSLinearAlgebra.jl:725-796)."""
+    """Shifted Cholesky-QR, <= itmax passes (RLSLinearAlgebra.jl:725-796), with the Grams and
+    the X R^{-1} products on the library's kernels."""
     import torch
     m, n = X.shape
     conv_tol = 2.0 * math.sqrt(n) * EPS
     Rtot = None
-    G = X.conj().T @ X
+    G = _LA.gram(X, X)
     eye = torch.eye(n, dtype=X.dtype, device=X.device)
     it = 1
     while it <= itmax:
@@ -319,16 +369,23 @@ def _qr_shiftchol(X, itmax: int = 3):
             continue
         Rk = Lc.conj().T
         Rtot = Rk.clone() if Rtot is None else Rk @ Rtot
-        X = torch.linalg.solve_triangular(Rk, X, upper=True, left=False)
-        G = X.conj().T @ X
+        X = _LA.mm([X], torch.linalg.solve_triangular(Rk, eye, upper=True))
+        G = _LA.gram(X, X)
         if float(torch.linalg.norm(G - eye)) < conv_tol:
             break
         it += 1
     return X, Rtot
 
 
+def _qr(X):
+    """Thin QR of a tall block.  Householder QR (rls_qr! with :lapack) is taken as CholQR
+    passes, as in the library's psTriRecy engine: R differs by diagonal phases, which no
+    caller observes."""
+    return _qr_shiftchol(X)
+
+
 def _qr_typed(X, orth_type):
-    return _qr(X) if orth_type == "lapack" else _qr_shiftchol(X)
+    return _qr_shiftchol(X)
 
 
 def _svd(A, ascending: bool):
@@ -353,7 +410,7 @@ def _eigh_range(A, il: int, iu: int):
 def _orth(V, X, orth_type):
     """Two-pass block Gram-Schmidt + QR (rls_orth!, RLSLinearAlgebra.jl:1127-1167)."""
     for _ in range(2):
-        X = X - V @ (V.conj().T @ X)
+        X = X - _LA.mm([V], _LA.gram(V, X))
         X, _ = _qr_typed(X, orth_type)
     return X
 
@@ -365,15 +422,15 @@ def _orth_robust(V, X, miniter: int = 2, maxiter: int = 4):
     tol = EPS * math.sqrt(float(V.shape[1]))
     Wtot = Rtot = None
     for it in range(1, maxiter + 1):
-        Wk = V.conj().T @ X
-        X = X - V @ Wk
+        Wk = _LA.gram(V, X)
+        X = X - _LA.mm([V], Wk)
         X, Rk = _qr(X)
         if it == 1:
             Wtot, Rtot = Wk, Rk
         else:
             Wtot = Wtot + Wk @ Rtot
             Rtot = Rk @ Rtot
-        if it >= miniter and float(torch.linalg.norm(V.conj().T @ X)) <= tol:
+        if it >= miniter and float(torch.linalg.norm(_LA.gram(V, X))) <= tol:
             break
     return X, Wtot, Rtot
 
@@ -381,6 +438,13 @@ def _orth_robust(V, X, miniter: int = 2, maxiter: int = 4):
 def _affine(z, A1, A2, A3, A4, shift=0.0):
     """A1 - z A2 - conj(z) A3 + (|z|^2 + shift) A4 (affine_conbine!, fastRaylRitz.jl:6-28)."""
     return A1 - z * A2 - z.conjugate() * A3 + (abs(z) ** 2 + shift) * A4
+
+
+def _affine_coef(z, G):
+    """[G; -z G; -conj(z) G; |z|^2 G]: affine_conbine!(J1..J4) G as one product with the four
+    blocks side by side."""
+    import torch
+    return torch.cat([G, -z * G, -z.conjugate() * G, (abs(z) ** 2) * G], 0)
 
 
 class _Space:
@@ -457,19 +521,19 @@ class _Space:
         J5n, J6n = self.J5[:, new], self.J6[:, new]
         eye = torch.eye(r, dtype=torch.complex128, device=Vn.device)
         if not self.implict_recycle:                    # light cache (recySpace.jl:420-449)
-            self.H1[new, new] = J5n.conj().T @ J5n
-            self.H2[new, new] = J5n.conj().T @ J6n
-            self.H3[new, new] = eye if self.orth_S else J6n.conj().T @ J6n
+            self.H1[new, new] = _LA.gram(J5n, J5n)
+            self.H2[new, new] = _LA.gram(J5n, J6n)
+            self.H3[new, new] = eye if self.orth_S else _LA.gram(J6n, J6n)
             if p > 0:
-                self.H1[old, new] = self.J5[:, old].conj().T @ J5n
+                self.H1[old, new] = _LA.gram(self.J5[:, old], J5n)
                 self.H1[new, old] = self.H1[old, new].conj().T
-                self.H2[old, new] = self.J5[:, old].conj().T @ J6n
-                self.H2[new, old] = J5n.conj().T @ self.J6[:, old]
+                self.H2[old, new] = _LA.gram(self.J5[:, old], J6n)
+                self.H2[new, old] = _LA.gram(J5n, self.J6[:, old])
                 if self.orth_S:
                     self.H3[old, new] = 0
                     self.H3[new, old] = 0
                 else:
-                    self.H3[old, new] = self.J6[:, old].conj().T @ J6n
+                    self.H3[old, new] = _LA.gram(self.J6[:, old], J6n)
                     self.H3[new, old] = self.H3[old, new].conj().T
         else:
             self.J1[:, new] = M.rmul(J5n)
@@ -478,19 +542,19 @@ class _Space:
                 self.J3[:, new] = S.rmul(J5n)
             if not self.orth_S:
                 self.J4[:, new] = S.rmul(J6n)
-            self.H1[new, new] = Vn.conj().T @ self.J1[:, new]
-            self.H2[new, new] = Vn.conj().T @ self.J2[:, new]
-            self.H3[new, new] = eye if self.orth_S else Vn.conj().T @ self.J4[:, new]
+            self.H1[new, new] = _LA.gram(Vn, self.J1[:, new])
+            self.H2[new, new] = _LA.gram(Vn, self.J2[:, new])
+            self.H3[new, new] = eye if self.orth_S else _LA.gram(Vn, self.J4[:, new])
             if p > 0:
-                self.H1[old, new] = self.J1[:, old].conj().T @ Vn
-                self.H2[old, new] = self.J3[:, old].conj().T @ Vn
+                self.H1[old, new] = _LA.gram(self.J1[:, old], Vn)
+                self.H2[old, new] = _LA.gram(self.J3[:, old], Vn)
                 self.H1[new, old] = self.H1[old, new].conj().T
-                self.H2[new, old] = Vn.conj().T @ self.J2[:, old]
+                self.H2[new, old] = _LA.gram(Vn, self.J2[:, old])
                 if self.orth_S:
                     self.H3[old, new] = 0
                     self.H3[new, old] = 0
                 else:
-                    self.H3[old, new] = self.J4[:, old].conj().T @ Vn
+                    self.H3[old, new] = _LA.gram(self.J4[:, old], Vn)
                     self.H3[new, old] = self.H3[old, new].conj().T
         self.pts[p] = z
         self.p = p + 1
@@ -522,9 +586,9 @@ class _Space:
             self.L[bp, :pr] = U[:r, :r] @ Lp + U[:r, r:] @ Li
             self.L[bi, :pr] = U[r:, :r] @ Lp + U[r:, r:] @ Li
             for A in rot:
-                Ap, Ai = A[:, bp].clone(), A[:, bi].clone()
-                A[:, bp] = Ap @ Uh[:r, :r] + Ai @ Uh[r:, :r]
-                A[:, bi] = Ap @ Uh[:r, r:] + Ai @ Uh[r:, r:]
+                AU = _LA.mm([A[:, bp], A[:, bi]], Uh)     # [A_p A_i] U^H, n x 2r
+                A[:, bp] = AU[:, :r]
+                A[:, bi] = AU[:, r:]
             if self.implict_recycle and not self.implict_update:
                 if not self.s_identity:
                     self.J3[:, bp] = S.rmul(self.J5[:, bp])
@@ -587,7 +651,7 @@ def _find_rr_dim(Hz, r: int, tau: float):
 def _fast_svd_rr(sp_: _Space, z, shift: float, tau: float, M: _Op = None, S: _Op = None):
     """fastSvdRaylRitz (fastRaylRitz.jl:99-220): implicit on the cached J blocks, or explicit
     with M - zS (USE_EXPLICIT_MS).  Returns (r_, X, W, sigma)."""
-    import torch
+    import torch  # noqa: F811
     p, r = sp_.p, sp_.r
     pr = p * r
     H1, H2, H3 = sp_.H1[:pr, :pr], sp_.H2[:pr, :pr], sp_.H3[:pr, :pr]
@@ -595,25 +659,25 @@ def _fast_svd_rr(sp_: _Space, z, shift: float, tau: float, M: _Op = None, S: _Op
     r_, G = _find_rr_dim(Hz, r, tau)
     G1 = G[:, :r_]
     if M is not None:
-        X_tmp = sp_.V[:, :pr] @ G1
+        X_tmp = _LA.mm([sp_.V[:, :pr]], G1)
         MX = M.mul(X_tmp) - z * S.mul(X_tmp)
         _, R_ = _qr(MX)
         sig, G_ = _svd(R_, ascending=True)
         G2 = G_[:, :r]
         sigma = sig[:r]
-        X = X_tmp @ G2
+        X = _LA.mm([X_tmp], G2)
         MX = M.mul(X) - z * S.mul(X)
         W = M.rmul(MX) - z.conjugate() * S.rmul(MX) - X * (sigma ** 2)[None, :]
         return r_, X, W, sigma
-    CzX_ = (sp_.J5[:, :pr] - z * sp_.J6[:, :pr]) @ G1
+    CzX_ = _LA.mm([sp_.J5[:, :pr], sp_.J6[:, :pr]], torch.cat([G1, -z * G1], 0))
     _, R_ = _qr(CzX_)
     sig, G_ = _svd(R_, ascending=True)
     G2 = G_[:, :r]
     sigma = sig[:r]
     Gt = G1 @ G2 if r_ < pr else G2
-    work = _affine(z, sp_.J1[:, :pr], sp_.J2[:, :pr], sp_.J3[:, :pr], sp_.J4[:, :pr])
-    X = sp_.V[:, :pr] @ Gt
-    W = work @ Gt - X * (sigma ** 2)[None, :]
+    X = _LA.mm([sp_.V[:, :pr]], Gt)
+    W = _LA.mm([sp_.J1[:, :pr], sp_.J2[:, :pr], sp_.J3[:, :pr], sp_.J4[:, :pr]], _affine_coef(z, Gt))
+    W = W - X * (sigma ** 2)[None, :]
     return r_, X, W, sigma
 
 
@@ -625,10 +689,10 @@ def _fast_rr(sp_: _Space, z):
     Hz = _affine(z, H1, H2, H2.conj().T, H3)
     lam, G = _eigh_range(Hz, 1, min(2 * r, pr))
     lam, G_ = lam[:r], G[:, :r]
-    X = sp_.V[:, :pr] @ G_
-    E = sp_.J6[:, :pr] @ G_
-    work = _affine(z, sp_.J1[:, :pr], sp_.J2[:, :pr], sp_.J3[:, :pr], sp_.J4[:, :pr])
-    W = work @ G_ - X * lam[None, :]
+    X = _LA.mm([sp_.V[:, :pr]], G_)
+    E = _LA.mm([sp_.J6[:, :pr]], G_)
+    W = _LA.mm([sp_.J1[:, :pr], sp_.J2[:, :pr], sp_.J3[:, :pr], sp_.J4[:, :pr]], _affine_coef(z, G_))
+    W = W - X * lam[None, :]
     return X, E, W, lam
 
 
@@ -689,8 +753,8 @@ def _lobpcg(Cop: _Op, X0, K, r: int, tol: float, side: str, normC: float, conv_r
     X, _ = _qr_typed(X0, orth_type)
     CX = Cop.mul(X)
     Q, R, G, sig, lam = _svd_rr(CX, r, side, orth_type)
-    X = X @ G
-    CX = Q @ (R @ G)
+    X = _LA.mm([X], G)
+    CX = _LA.mm([Q], R @ G)
     P = resid(X, CX, lam[:r])
     if K is not None:
         P = K(P)
@@ -699,14 +763,14 @@ def _lobpcg(Cop: _Op, X0, K, r: int, tol: float, side: str, normC: float, conv_r
     W = CW = None
     for i in range(1, maxit + 1):
         if i == 1:
-            S_, CS = torch.cat([X, P], 1), torch.cat([CX, CP], 1)
+            CS = torch.cat([CX, CP], 1)
             Q, R, G, sig, lam = _svd_rr(CS, r, side, orth_type)
-            XP, CXP = S_ @ G, Q @ (R @ G)
+            XP, CXP = _LA.mm([X, P], G), _LA.mm([Q], R @ G)
         else:
-            S_, CS = torch.cat([X, P, W], 1), torch.cat([CX, CP, CW], 1)
+            CS = torch.cat([CX, CP, CW], 1)
             Q, R, G, sig, lam = _svd_rr(CS, r, side, orth_type)
             G2 = G[:, :2 * r]
-            XP, CXP = S_ @ G2, Q @ (R @ G2)
+            XP, CXP = _LA.mm([X, P, W], G2), _LA.mm([Q], R @ G2)
         X, P = XP[:, :r], XP[:, r:]
         CX, CP = CXP[:, :r], CXP[:, r:]
         W = resid(X, CX, lam[:r])
@@ -733,10 +797,15 @@ def psPrecRecy(C, rec: Recursion, control: PsPrecRecyControl = None, verbose: in
     perm: column ordering p (0-based); default column_ordering(C).  rand_block(n, r, tag)
     overrides the library's start blocks (for comparisons with another implementation)."""
     import torch
+    global _LA
     ctl = control or PsPrecRecyControl()
     ps = solver or PrecSolver(C, device)
-    with torch.cuda.device(ps.dev), torch.cuda.stream(ps.stream):
-        return _drive(ps, rec, ctl, verbose, perm, rand_block)
+    _LA = _Blas(ps)
+    try:
+        with torch.cuda.device(ps.dev), torch.cuda.stream(ps.stream):
+            return _drive(ps, rec, ctl, verbose, perm, rand_block)
+    finally:
+        _LA = None
 
 
 def _drive(ps: PrecSolver, rec: Recursion, ctl: PsPrecRecyControl, verbose: int, perm, rand_block):
@@ -916,12 +985,12 @@ def _drive(ps: PrecSolver, rec: Recursion, ctl: PsPrecRecyControl, verbose: int,
         # two-level preconditioner (_get_pred_precond_ad!, psPrecRecy.jl:465-485;
         # SparseLLTP_AD ldiv!, RLSPreconditioner.jl:229-262)
         t0 = time.perf_counter_ns()
-        Lg = torch.linalg.cholesky(CX_pred.conj().T @ CX_pred)
+        Lg = torch.linalg.cholesky(_LA.gram(CX_pred, CX_pred))
         E = E_pred
 
         def K_ad(B, E=E, Lg=Lg):
-            coeff = torch.cholesky_solve(E.conj().T @ B, Lg)
-            return ps.apply(O.RLS_OP_KINV, B) + E @ coeff
+            coeff = torch.cholesky_solve(_LA.gram(E, B), Lg)
+            return ps.apply(O.RLS_OP_KINV, B) + _LA.mm([E], coeff)
         st.precond_time_ns += time.perf_counter_ns() - t0
         t0 = time.perf_counter_ns()
         res = _lobpcg(Cz, X_main, K_ad, r_main, tol_main, "S", ctl.main_normC, (1, 1), maxit=ctl.maxit)
