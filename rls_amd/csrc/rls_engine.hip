@@ -383,6 +383,7 @@ struct rls_ctx {
         if (rc != 0 || !factor_ready)
             throw RlsError{RLS_E_NUMERIC, "factor callback failed for a refine shift"};
     }
+    // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
         const int nb = (int)cdiv(n, sptrsv_block_rows());
         if (!sp_cnt.p || sp_cnt.n < (size_t)nb + 1) {
@@ -393,7 +394,7 @@ struct rls_ctx {
             sp_epoch = 0;
         }
         kt_begin(RLS_KF_TRSM);
-        launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, n, k, sp_cnt.p, sp_cnt.p + 1,
+        launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, sp_cnt.p, sp_cnt.p + 1,
                       ++sp_epoch, sp_err.p);
         // algorithmic: the factor once per 8-column chunk (values, column indices, row
         // pointers) + the block read and written
@@ -404,16 +405,17 @@ struct rls_ctx {
     // X <- A^{-1} X or A^{-H} X for A = C - zI = Pr^T L U Pc^T (sparse path)
     void sp_apply(zd* X, int k, bool inverse_h) {
         sp_tmp.ensure((size_t)n * k);
+        // the solves run on a row-major copy (a row's right-hand sides in one cache line)
         if (!inverse_h) {
-            launch_permute_rows(st, n, perm_r.p, X, n, sp_tmp.p, n, k, true);
+            launch_permute_rows_strided(st, n, perm_r.p, X, 1, n, sp_tmp.p, k, 1, k, true);
             sp_solve(fL, true, true, sp_tmp.p, k);
             sp_solve(fU, false, false, sp_tmp.p, k);
-            launch_permute_rows(st, n, perm_c.p, sp_tmp.p, n, X, n, k, false);
+            launch_permute_rows_strided(st, n, perm_c.p, sp_tmp.p, k, 1, X, 1, n, k, false);
         } else {
-            launch_permute_rows(st, n, perm_c.p, X, n, sp_tmp.p, n, k, true);
+            launch_permute_rows_strided(st, n, perm_c.p, X, 1, n, sp_tmp.p, k, 1, k, true);
             sp_solve(fUH, true, false, sp_tmp.p, k);
             sp_solve(fLH, false, true, sp_tmp.p, k);
-            launch_permute_rows(st, n, perm_r.p, sp_tmp.p, n, X, n, k, false);
+            launch_permute_rows_strided(st, n, perm_r.p, sp_tmp.p, k, 1, X, 1, n, k, false);
         }
     }
     void apply_C(zd* X, int k) {    // X <- (M - zS)^{-H} X
@@ -1276,28 +1278,29 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
             ctx->ic_tmp.ensure((size_t)n * k);
             zd* t = ctx->ic_tmp.p;
             switch (op) {
+            // t is row-major (element (i, c) at i k + c): a row's right-hand sides share a line
             case RLS_OP_INVF:       // P L^{-H} X: t = X, t <- L^{-H} t, Y[i] = t[invp[i]]
-                launch_copy2d(st, n, k, X, ldx, t, n);
+                launch_permute_rows_strided(st, n, nullptr, X, 1, ldx, t, k, 1, k, false);
                 solve(ctx->icLH, false, t, k);
-                launch_permute_rows(st, n, ctx->ic_invp.p, t, n, Y, ldy, k, false);
+                launch_permute_rows_strided(st, n, ctx->ic_invp.p, t, k, 1, Y, 1, ldy, k, false);
                 break;
             case RLS_OP_INVF_H:     // L^{-1} P^T X: t[i] = X[p[i]], t <- L^{-1} t
-                launch_permute_rows(st, n, ctx->ic_p.p, X, ldx, t, n, k, false);
+                launch_permute_rows_strided(st, n, ctx->ic_p.p, X, 1, ldx, t, k, 1, k, false);
                 solve(ctx->icL, true, t, k);
-                launch_copy2d(st, n, k, t, n, Y, ldy);
+                launch_permute_rows_strided(st, n, nullptr, t, k, 1, Y, 1, ldy, k, false);
                 break;
             case RLS_OP_KINV:
-                launch_permute_rows(st, n, ctx->ic_p.p, X, ldx, t, n, k, false);
+                launch_permute_rows_strided(st, n, ctx->ic_p.p, X, 1, ldx, t, k, 1, k, false);
                 solve(ctx->icL, true, t, k);
                 solve(ctx->icLH, false, t, k);
-                launch_permute_rows(st, n, ctx->ic_invp.p, t, n, Y, ldy, k, false);
+                launch_permute_rows_strided(st, n, ctx->ic_invp.p, t, k, 1, Y, 1, ldy, k, false);
                 break;
             case RLS_OP_LINV:
             case RLS_OP_LINV_H:
-                launch_copy2d(st, n, k, X, ldx, t, n);
+                launch_permute_rows_strided(st, n, nullptr, X, 1, ldx, t, k, 1, k, false);
                 if (op == RLS_OP_LINV) solve(ctx->icL, true, t, k);
                 else solve(ctx->icLH, false, t, k);
-                launch_copy2d(st, n, k, t, n, Y, ldy);
+                launch_permute_rows_strided(st, n, nullptr, t, k, 1, Y, 1, ldy, k, false);
                 break;
             default:
                 throw RlsError{RLS_E_ARG, "rls_dev_apply: unknown operator"};

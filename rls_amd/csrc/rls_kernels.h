@@ -113,15 +113,20 @@ void launch_grad_toeplitz(hipStream_t st, int64_t n, const zd* U, const zd* V, d
 // 0-based), X and Y n x k column-major.
 void launch_csr_spmm(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, zd s,
                      const zd* X, int64_t ldx, int k, zd* Y, int64_t ldy);
-// Y[i] = X[p[i]] (gather) or Y[p[i]] = X[i] (scatter), k columns
+// Y[i] = X[p[i]] (gather) or Y[p[i]] = X[i] (scatter), k columns, column-major blocks
 void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t ldx, zd* Y,
                          int64_t ldy, int k, bool scatter);
+// the same for any layouts: element (i, c) of X at i * xrs + c * xcs, of Y at i * yrs + c * ycs;
+// p = nullptr copies (a layout change)
+void launch_permute_rows_strided(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t xrs,
+                                 int64_t xcs, zd* Y, int64_t yrs, int64_t ycs, int k, bool scatter);
 // X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), any k (in
-// chunks of 8 columns inside one launch).
+// chunks of 8 columns inside one launch); element (j, c) of X at j * rs + c * cs (row-major
+// blocks, rs = k, cs = 1, keep a row's right-hand sides in one cache line).
 // dep[b] (b = block of sptrsv_block_rows() rows): lower -> the first block any row of b
 // references, upper -> the last.  counter / flags (one per block) start at zero; epoch grows
 // by one per solve; *err is raised if a wait overran its bound.
 int sptrsv_block_rows();
 void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
-                   bool unit_diag, const int* dep, zd* X, int64_t ldx, int k, unsigned int* counter,
+                   bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k, unsigned int* counter,
                    unsigned int* flags, unsigned int epoch, int* err);

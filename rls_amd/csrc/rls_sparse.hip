@@ -49,14 +49,17 @@ k_csr_spmm(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict_
     }
 }
 
-// Row permutation: Y[i, c] = X[p[i], c]  (gather) or Y[p[i], c] = X[i, c] (scatter)
+// Row permutation between blocks of any layout (element (i, c) at i * rs + c * cs):
+// Y[i, c] = X[p[i], c]  (gather) or Y[p[i], c] = X[i, c] (scatter); p = nullptr copies.
 __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const zd* __restrict__ X,
-                               int64_t ldx, zd* __restrict__ Y, int64_t ldy, int scatter) {
+                               int64_t xrs, int64_t xcs, zd* __restrict__ Y, int64_t yrs, int64_t ycs,
+                               int scatter) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     if (i >= n) return;
-    if (scatter) Y[p[i] + ldy * c] = X[i + ldx * c];
-    else Y[i + ldy * c] = X[p[i] + ldx * c];
+    const int64_t q = p ? p[i] : i;
+    if (scatter) Y[q * yrs + ycs * c] = X[i * xrs + xcs * c];
+    else Y[i * yrs + ycs * c] = X[q * xrs + xcs * c];
 }
 
 // Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
@@ -78,8 +81,8 @@ constexpr int SP_SLOTS = 8;                     // staged entries per thread (12
 template <bool LOWER>
 __global__ void __launch_bounds__(SP_THREADS)
 k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
-         const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t ldx,
-         int k, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
+         const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t rs,
+         int64_t cs, int k, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
     __shared__ unsigned int s_ticket;
     __shared__ int s_ok;
     __shared__ zd xs[SP_RB][8];
@@ -159,7 +162,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     // the right-hand sides in chunks of 8 columns
     for (int c0 = 0; c0 < k; c0 += 8) {
         const int kc = min(8, k - c0);
-        zd* Xc = X + ldx * c0;
+        zd* Xc = X + cs * c0;
         if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
         // all rows' sums over earlier blocks: staged entries two at a time (their X loads in
         // flight together), then any entries past the staged slots
@@ -176,20 +179,33 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                 zd x0[8], x1[8];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    x0[c] = (c < kc) ? Xc[j0 + ldx * c] : zzero();
-                    x1[c] = (c < kc) ? Xc[jj1 + ldx * c] : zzero();
+                    x0[c] = (c < kc) ? Xc[j0 * rs + cs * c] : zzero();
+                    x1[c] = (c < kc) ? Xc[jj1 * rs + cs * c] : zzero();
                 }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) acc[c] = zfma(a1, x1[c], zfma(a0, x0[c], acc[c]));
                 if (j1 < 0) break;
             }
-            for (int64_t e = e_over; e < e_end; e += SP_TPR) {
-                const int64_t j = ci[e];
-                if (j == ri || (j >= r0 && j < r0 + rows)) continue;
-                const zd a = v[e];
+            // entries past the staged slots (long rows), four at a time: their index / value
+            // loads, then their X loads, issued back to back
+            for (int64_t e = e_over; e < e_end; e += 4 * SP_TPR) {
+                int64_t jj[4];
+                zd aa[4];
 #pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    if (c < kc) acc[c] = zfma(a, Xc[j + ldx * c], acc[c]);
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t ee = e + (int64_t)SP_TPR * u;
+                    int64_t j = (ee < e_end) ? ci[ee] : -1;
+                    if (j == ri || (j >= r0 && j < r0 + rows)) j = -1;
+                    jj[u] = j;
+                    aa[u] = (j >= 0) ? v[ee] : zzero();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (jj[u] < 0) continue;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (c < kc) acc[c] = zfma(aa[u], Xc[jj[u] * rs + cs * c], acc[c]);
+                }
             }
             for (int o = SP_TPR / 2; o > 0; o >>= 1) {
 #pragma unroll
@@ -201,7 +217,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             if (sub == 0 && li < rows) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
-                    ext[li][c] = (c < kc) ? zsub(Xc[ri + ldx * c], acc[c]) : zzero();
+                    ext[li][c] = (c < kc) ? zsub(Xc[ri * rs + cs * c], acc[c]) : zzero();
             }
         }
         __syncthreads();
@@ -226,7 +242,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
         __syncthreads();
         for (int idx = tid; idx < rows * 8; idx += SP_THREADS) {
             const int lr = idx % rows, c = idx / rows;
-            if (c < kc) Xc[(r0 + lr) + ldx * c] = xs[lr][c];
+            if (c < kc) Xc[(r0 + lr) * rs + cs * c] = xs[lr][c];
         }
         __syncthreads();
     }
@@ -254,24 +270,29 @@ void launch_csr_spmm(hipStream_t st, int64_t n, const int64_t* rp, const int64_t
 
 void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t ldx, zd* Y,
                          int64_t ldy, int k, bool scatter) {
-    if (k <= 0) return;
-    hipLaunchKernelGGL(k_permute_rows, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, p, X, ldx, Y, ldy,
-                       scatter ? 1 : 0);
+    launch_permute_rows_strided(st, n, p, X, 1, ldx, Y, 1, ldy, k, scatter);
+}
+
+void launch_permute_rows_strided(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t xrs,
+                                 int64_t xcs, zd* Y, int64_t yrs, int64_t ycs, int k, bool scatter) {
+    if (k <= 0 || n <= 0) return;
+    hipLaunchKernelGGL(k_permute_rows, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, p, X, xrs, xcs, Y, yrs,
+                       ycs, scatter ? 1 : 0);
     LAUNCH_CHECK();
 }
 
 int sptrsv_block_rows() { return SP_RB; }
 
 void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
-                   bool unit_diag, const int* dep, zd* X, int64_t ldx, int k, unsigned int* counter,
+                   bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k, unsigned int* counter,
                    unsigned int* flags, unsigned int epoch, int* err) {
     if (k <= 0) return;
     const int nblk = (int)cdiv(n, SP_RB);
     if (lower)
         hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
-                           ldx, k, counter, flags, epoch, err);
+                           rs, cs, k, counter, flags, epoch, err);
     else
         hipLaunchKernelGGL(k_sptrsv<false>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
-                           ldx, k, counter, flags, epoch, err);
+                           rs, cs, k, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
