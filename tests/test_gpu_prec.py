@@ -103,3 +103,51 @@ def test_ps_prec_recy_explicit_main_recycle():
     assert np.max(np.abs(sig - tr) / tr) < 1e-4
     assert st.n_pts == ost.n_pts and np.array_equal(st.r_recy, ost.r_recy)
     assert np.max(np.abs(sig - osig) / osig) < 1e-6
+
+
+def test_device_block_products_match_torch():
+    """rls_dev_gemm (MFMA GEMM for >= 32 columns, tall GEMM below) and rls_dev_gram against
+    plain products, with operand lists, column slices and an empty operand."""
+    import torch
+    A = WL.skewlap3d(9)                       # n = 512
+    n = A.shape[0]
+    ps = RP.PrecSolver(A)
+    la = RP._Blas(ps)
+    g = torch.Generator().manual_seed(0)
+    rnd = lambda *sh: torch.complex(torch.randn(*sh, generator=g, dtype=torch.float64),
+                                    torch.randn(*sh, generator=g, dtype=torch.float64))
+    with torch.cuda.stream(ps.stream):
+        big = rnd(100, n).to(ps.dev).T                     # column-major n x 100
+        assert big.shape == (n, 100) and big.stride() == (1, n)
+        V, J = big[:, :40], big[:, 40:100]
+        for N in (6, 40):
+            B = rnd(100, N).to(ps.dev)
+            got = la.mm([V, J], B).cpu()
+            want = (torch.cat([V, J], 1) @ B).cpu()
+            assert torch.linalg.norm(got - want) / torch.linalg.norm(want) < 1e-13
+        empty = big[:, :0]
+        B = rnd(40, 7).to(ps.dev)
+        got = la.mm([empty, V], B).cpu()
+        assert torch.linalg.norm(got - (V @ B).cpu()) / torch.linalg.norm((V @ B).cpu()) < 1e-13
+        G = la.gram(V, J).cpu()
+        Gw = (V.conj().T @ J).cpu()
+        assert torch.linalg.norm(G - Gw) / torch.linalg.norm(Gw) < 1e-13
+        ps.sync()
+
+
+def test_ps_prec_recy_wide_prediction_block():
+    """r_pred = 12: the prediction stage's solves carry more than 8 right-hand sides, so each
+    triangular solve runs several column chunks after one dependency wait."""
+    A, rec = _small_case()
+    n = A.shape[0]
+    normC = WL.norm2_estimate(A)
+    octl, gctl = _controls(normC, r_pred=12, p_pred=4)
+    perm = RP.column_ordering(A)
+    rb = lambda nn, rr, tag: R.initial_block(nn, rr, 31 + tag)
+    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, rand_block=rb)
+    osig, ost = OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb)
+    Ad = A.toarray()
+    tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
+    assert np.max(np.abs(sig - tr) / tr) < 1e-4
+    assert st.n_pts == ost.n_pts and st.n_pred_pts == ost.n_pred_pts
+    assert np.max(np.abs(sig - osig) / osig) < 1e-6
