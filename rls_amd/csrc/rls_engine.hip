@@ -68,7 +68,7 @@ struct rls_ctx {
     // operator: dense upper-triangular Schur factor T, or (sparse path) C in CSR with the CSR
     // of C^H, and the host-supplied factor Pr (C - zI) Pc = L U of the current refine shift
     int64_t n = 0;
-    DArr<zd> T;
+    DArr<zd> T, TH;   // dense upper-triangular operator and its adjoint (column-major, ld n)
     bool sparse = false;
     DArr<int64_t> crp, cci, hrp, hci;
     DArr<zd> cv, hv;
@@ -98,7 +98,7 @@ struct rls_ctx {
     DArr<zd> V, J1, J2, J5, L, H1, H2;
     std::vector<cplx> pts;
     // LOBPCG blocks
-    DArr<zd> S, CS, S2, CS2, Wt, Dinv, trsm_tmp, trmm_part;
+    DArr<zd> S, CS, S2, CS2, Wt, tri_F, trsm_tmp, trmm_part;
     // small / reduction workspaces
     DArr<zd> part, gram, Gs, Rm, Rinv, Racc, Rtmp, Wtot, Wk, Wneg, U;
     DArr<double> sig, lam, offI, nrm2;
@@ -172,7 +172,7 @@ struct rls_ctx {
     }
 
     ~rls_ctx() {
-        DArr<zd>* zs[] = {&T, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &Dinv, &trsm_tmp, &trmm_part,
+        DArr<zd>* zs[] = {&T, &TH, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &tri_F, &trsm_tmp, &trmm_part,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
                           &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev,
                           &vkeep, &zkeep, &Qs, &gY, &gU, &gUq, &gVq};
@@ -319,7 +319,7 @@ struct rls_ctx {
         S2.ensure((size_t)n * 3 * r); CS2.ensure((size_t)n * 3 * r);
         Wt.ensure((size_t)n * 3 * r);
         trsm_tmp.ensure((size_t)n * 3 * r);
-        Dinv.ensure((size_t)cdiv(n, 64) * 4096);
+        tri_F.ensure(tri_factor_elems(n));
         U.ensure((size_t)std::max(pmax, 1) * 4 * r * r);
     }
 
@@ -343,7 +343,7 @@ struct rls_ctx {
             int kc = std::min(8, k - c0);
             ++tri_epoch;
             kt_begin(RLS_KF_TRSM);
-            launch_trsm_persistent(st, n, T.p, n, Dinv.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
+            launch_trsm_persistent(st, n, T.p, TH.p, n, tri_F.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
                                    tri_flags.p, tri_epoch, tri_err.p);
             // algorithmic: the triangle of T once + the RHS block read and written
             kt_end(RLS_KF_TRSM, 8.0 * 0.5 * (double)n * n * kc,
@@ -374,7 +374,7 @@ struct rls_ctx {
     // sparse -> the host's factor callback, which uploads L, U (rls_set_sparse_factor)
     void refine_factor(cplx z) {
         if (!sparse) {
-            launch_tri_diag_inv(st, n, T.p, n, zmk(z.real(), z.imag()), Dinv.p);
+            launch_tri_factors(st, n, T.p, n, zmk(z.real(), z.imag()), tri_F.p);
             return;
         }
         if (!factor_cb) throw RlsError{RLS_E_STATE, "sparse operator: no factor callback set (rls_set_factor_callback)"};
@@ -1120,6 +1120,9 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         ctx->T.ensure((size_t)n * n);
         HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
                                  hipMemcpyHostToDevice, ctx->st));
+        // T^H for the lower solves, which then read column blocks like the upper ones
+        ctx->TH.ensure((size_t)n * n);
+        launch_conj_transpose(ctx->st, (int)n, (int)n, ctx->T.p, n, ctx->TH.p, n);
         ctx->sync();
         return RLS_OK;
     });
@@ -1157,6 +1160,7 @@ int rls_set_csr(rls_ctx* ctx, const rls_csr* C) {
             }
         ctx->n = n;
         ctx->T.release();
+        ctx->TH.release();
         ctx->sparse = true;
         ctx->resumable = false;
         ctx->have_q = false;
@@ -1517,9 +1521,9 @@ int rls_test_trsm(int64_t n, const double* T, double zr, double zi, int k, int u
         if (rls_set_tri_dense(ctx, T, n, n) != RLS_OK) throw RlsError{RLS_E_ARG, ctx->err};
         DArr<zd> dX;
         dX.ensure((size_t)n * k);
-        ctx->Dinv.ensure((size_t)cdiv(n, 64) * 4096);
+        ctx->tri_F.ensure(tri_factor_elems(n));
         ctx->h2d(dX.p, reinterpret_cast<const zd*>(X), (size_t)n * k);
-        launch_tri_diag_inv(ctx->st, n, ctx->T.p, n, zmk(zr, zi), ctx->Dinv.p);
+        launch_tri_factors(ctx->st, n, ctx->T.p, n, zmk(zr, zi), ctx->tri_F.p);
         // a solve of a scratch copy first, so the checked solve runs with the next epoch
         DArr<zd> dY;
         dY.ensure((size_t)n * k);
@@ -1611,8 +1615,8 @@ extern "C" int rls_test_trsm_time(int64_t n, const double* T, int k, int upper, 
         std::vector<cplx> X0((size_t)n * k);
         for (size_t e = 0; e < X0.size(); ++e) X0[e] = cplx(u01_splitmix(7, 2 * e), u01_splitmix(7, 2 * e + 1));
         ctx->h2d(dX.p, reinterpret_cast<zd*>(X0.data()), X0.size());
-        ctx->Dinv.ensure((size_t)cdiv(n, 64) * 4096);
-        launch_tri_diag_inv(ctx->st, n, ctx->T.p, n, zmk(0.3, -0.2), ctx->Dinv.p);
+        ctx->tri_F.ensure(tri_factor_elems(n));
+        launch_tri_factors(ctx->st, n, ctx->T.p, n, zmk(0.3, -0.2), ctx->tri_F.p);
         ctx->tri_solve(dX.p, k, upper != 0);
         ctx->sync();
         hipEvent_t e0, e1;

@@ -36,8 +36,9 @@ void launch_scale_sub(hipStream_t st, int64_t n, zd* Y, int64_t ldy, const zd* X
 // Dense triangular operators with T upper triangular (n x n, ld) and shift z:
 //   trmm:   Y = T X              (upper, no shift)
 //   trmmh:  Y = T^H X
-//   diag_inv: Dinv_j = (T_jj - z I)^{-1} for 64 x 64 diagonal blocks
-//   (the solves with T - zI are launch_trsm_persistent below)
+//   tri_factors: the block factors of T - zI for launch_trsm_persistent: F holds
+//   tri_factor_elems(n) complex values, [D_j^{-1} | D_j^{-1} T_{j,j+1} | (T_{j-1,j} D_j^{-1})^H]
+//   over the 64 x 64 blocks j (each block column-major)
 // part: workspace of trmm_part_elems(n) complex values
 size_t trmm_part_elems(int64_t n);
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
@@ -45,7 +46,8 @@ void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, cons
 void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                         int64_t ldx, int k, zd* Y, int64_t ldy);
 constexpr int TRI_NB = 64;
-void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* Dinv);
+size_t tri_factor_elems(int64_t n);
+void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* F);
 
 
 // Block rotations of the recycle space: for i = 0..p-2 (block index i, newest block pb),
@@ -86,10 +88,11 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B);
 void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
                        int B);
-// Persistent single-launch shifted triangular solve (see rls_tall.hip); counter/flags are
-// device words owned by the caller (flags: one per 64-row block), epoch increases per call.
-void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv, zd* X,
-                            int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
+// Persistent single-launch shifted triangular solve (see rls_tall.hip) with T, its adjoint TH
+// (both n x n, ldt) and the block factors F of launch_tri_factors; counter/flags are device
+// words owned by the caller (flags: one per 64-row block), epoch increases per call.
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, const zd* TH, int64_t ldt, const zd* F,
+                            zd* X, int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
                             unsigned int epoch, int* err);
 void read_heev_phases(long long* out);
 // Tiled FP64 complex GEMM Y = [A_q] * B for wide B (batched recycle products).

@@ -1283,9 +1283,9 @@ void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, i
 // D[i + ldd*j] = conj(S[j + lds*i])  (m x k block of D from the k x m block of S)
 __global__ void k_conj_transpose(int m, int k, const zd* __restrict__ S, int64_t lds,
                                  zd* __restrict__ D, int64_t ldd) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * k) return;
-    const int i = idx % m, j = idx / m;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * k) return;
+    const int64_t i = idx % m, j = idx / m;
     D[i + ldd * j] = zconj(S[j + lds * i]);
 }
 

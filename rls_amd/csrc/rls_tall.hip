@@ -259,10 +259,13 @@ k_trmmh_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __rest
     }
 }
 
-// Inverse of each 64 x 64 diagonal block of T - zI (upper triangular).
+// Inverse of each 64 x 64 diagonal block of T - zI (upper triangular; a partial last block
+// is padded with the identity).  Thread c computes column c by back substitution
+// (x_a = (delta_ac - sum_{q=a+1..c} D[a][q] x_q) / D[a][a]) with the column in LDS.
 __global__ void __launch_bounds__(64)
 k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __restrict__ Dinv) {
     __shared__ zd D[64][65];
+    __shared__ zd Xs[64][65];
     const int jb = blockIdx.x;
     const int64_t o = (int64_t)jb * 64;
     const int m = (int)min((int64_t)64, n - o);
@@ -276,18 +279,21 @@ k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __res
             v = (a == c) ? zmk(1.0, 0.0) : zzero();
         }
         D[a][c] = v;
+        Xs[a][c] = zzero();
     }
     __syncthreads();
-    // column c of the inverse: back substitution
-    zd x[64];
-    for (int a = 0; a < 64; ++a) x[a] = zzero();
     for (int a = c; a >= 0; --a) {
-        zd s = (a == c) ? zmk(1.0, 0.0) : zzero();
-        for (int q = a + 1; q <= c; ++q) s = zsub(s, zmul(D[a][q], x[q]));
-        x[a] = zdiv(s, D[a][a]);
+        zd s0 = (a == c) ? zmk(1.0, 0.0) : zzero(), s1 = zzero();
+        int q = a + 1;
+        for (; q + 1 <= c; q += 2) {
+            s0 = zsub(s0, zmul(D[a][q], Xs[q][c]));
+            s1 = zsub(s1, zmul(D[a][q + 1], Xs[q + 1][c]));
+        }
+        if (q <= c) s0 = zsub(s0, zmul(D[a][q], Xs[q][c]));
+        Xs[a][c] = zdiv(zadd(s0, s1), D[a][a]);
     }
     zd* out = Dinv + (int64_t)jb * 64 * 64;
-    for (int a = 0; a < 64; ++a) out[a + 64 * c] = x[a];
+    for (int a = 0; a < 64; ++a) out[a + 64 * c] = Xs[a][c];
 }
 
 
@@ -412,11 +418,6 @@ void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, con
     LAUNCH_CHECK();
 }
 
-void launch_tri_diag_inv(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* Dinv) {
-    hipLaunchKernelGGL(k_tri_diag_inv, dim3(cdiv(n, 64)), dim3(64), 0, st, n, T, ldt, z, Dinv);
-    LAUNCH_CHECK();
-}
-
 
 void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int narr, int64_t ld,
                            int r, int p, const zd* U) {
@@ -508,7 +509,12 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 //   upper = 0:  X <- (T - zI)^{-H} X        (forward substitution with the lower factor)
 // One workgroup per 64-row block.  Blocks are claimed through a ticket counter in start
 // order, so a workgroup only ever waits for blocks claimed by workgroups that are already
-// running.  Block j publishes its solution by storing `epoch` into flag[j] after a device
+// running.  Block j first folds in every dependency but the last (its neighbour j+1 or
+// j-1) and forms c_j = D_j^{-1} (b_j - sum T_jq x_q); the neighbour's solution then only
+// costs one 64 x 64 product with the precomputed chain block E_j (upper: D_j^{-1} T_{j,j+1};
+// lower: (T_{j-1,j} D_j^{-1})^H): x_j = c_j - E_j x_nb, which keeps the diagonal-block
+// solve off the dependency chain.  The lower solve reads the off-diagonal blocks from TH =
+// T^H (column-major), so both directions stream column blocks of their factor.  Block j publishes its solution by storing `epoch` into flag[j] after a device
 // fence; readers spin (bounded) on an agent-scope atomic load, then fence.  The counter is
 // never reset: tickets of solve e are [e*nblk, (e+1)*nblk) because launches on one stream
 // run in order.  A spin that exceeds its bound sets *err and the kernel still terminates.
@@ -525,6 +531,7 @@ typedef double pt_dv4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void pt_mfma_tile(pt_dv4& acc, const zd (*A)[65], const zd (*B)[PT_KC], int w,
                                              int l) {
     const int i = 16 * w + (l & 15), kq = l >> 4, jc = l & 15;
+    pt_dv4 acc2 = {0.0, 0.0, 0.0, 0.0};   // second, independent chain: Ai [-Bi | Br]
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) {
         const int kc = 4 * ks + kq;
@@ -533,14 +540,16 @@ __device__ __forceinline__ void pt_mfma_tile(pt_dv4& acc, const zd (*A)[65], con
         const double b1 = (jc < 8) ? xb.x : xb.y;
         const double b2 = (jc < 8) ? -xb.y : xb.x;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b2, acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b2, acc2, 0, 0, 0);
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += acc2[v];
 }
 
 __global__ void __launch_bounds__(PT_THREADS)
-k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
-                  zd* X, int64_t ldx, int k, int upper, unsigned int* counter, unsigned int* flags,
-                  unsigned int epoch, int* err) {
+k_trsm_persistent(int64_t n, const zd* __restrict__ Tsrc, int64_t ldt, const zd* __restrict__ Dinv,
+                  const zd* __restrict__ Ech, zd* X, int64_t ldx, int k, int upper, unsigned int* counter,
+                  unsigned int* flags, unsigned int epoch, int* err) {
     constexpr int PER = 4096 / PT_THREADS;      // T / Dinv block elements per thread
     __shared__ zd Ts[64][65];
     __shared__ zd Ds[64][65];
@@ -560,9 +569,8 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
     const int j = upper ? nblk - 1 - t : t;
     const int64_t o = (int64_t)j * 64;
     const int m = (int)min((int64_t)64, n - o);
-    // T block of dependency kb into registers, all loads issued before any is consumed.
-    // upper: Ts[a][c] = T[o + a, ko + c] (lanes over a); lower: Ts[a][c] = conj(T[ko + c, o + a])
-    // (lanes over c, the contiguous index).
+    // block (j, kb) of the factor into registers, all loads issued before any is consumed:
+    // Ts[a][c] = Tsrc[o + a, ko + c] with Tsrc = T (upper) or T^H (lower); lanes over a
     zd pre[PER];
     auto load_T = [&](int kb) {
         const int64_t ko = (int64_t)kb * 64;
@@ -571,23 +579,29 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         for (int u = 0; u < PER; ++u) {
             const int idx = tid + PT_THREADS * u;
             const int i0 = idx & 63, i1 = idx >> 6;
-            if (upper) pre[u] = (i0 < m && i1 < mk) ? T[(o + i0) + ldt * (ko + i1)] : zzero();
-            else pre[u] = (i1 < m && i0 < mk) ? zconj(T[(ko + i0) + ldt * (o + i1)]) : zzero();
+            pre[u] = (i0 < m && i1 < mk) ? Tsrc[(o + i0) + ldt * (ko + i1)] : zzero();
         }
     };
     auto store_T = [&]() {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int idx = tid + PT_THREADS * u;
-            const int i0 = idx & 63, i1 = idx >> 6;
-            if (upper) Ts[i0][i1] = pre[u];
-            else Ts[i1][i0] = pre[u];
+            Ts[idx & 63][idx >> 6] = pre[u];
         }
     };
-    // dependencies: upper -> blocks j+1..nblk-1 (descending), lower -> blocks 0..j-1
+    // chain block E_j of block j (column-major) into the same registers
+    auto load_E = [&]() {
+        const zd* E = Ech + (int64_t)j * 4096;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) pre[u] = E[tid + PT_THREADS * u];
+    };
+    // dependencies: upper -> blocks j+1..nblk-1 (descending), lower -> blocks 0..j-1; all but
+    // the last go through T blocks, the last (the neighbour) through the chain block
     const int ndep = upper ? (nblk - 1 - j) : j;
+    const int nfar = ndep > 0 ? ndep - 1 : 0;
     auto dep = [&](int q) { return upper ? nblk - 1 - q : q; };
-    if (ndep > 0) load_T(dep(0));
+    if (nfar > 0) load_T(dep(0));
+    else if (ndep > 0) load_E();
     // diagonal-block inverse (upper: Dinv_j, lower: Dinv_j^H) and right-hand side
     {
         const zd* Di = Dinv + (int64_t)j * 4096;
@@ -614,14 +628,11 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
             bj[idx & 63][idx >> 6] = bv[u];
         }
     }
-    pt_dv4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int q = 0; q < ndep; ++q) {
-        const int kb = dep(q);
+    // wait for block kb's solution (bounded spin; on expiry *err is set and s_bail raised),
+    // then stage it in xs
+    auto await_x = [&](int kb) {
         const int64_t ko = (int64_t)kb * 64;
         const int mk = (int)min((int64_t)64, n - ko);
-        __syncthreads();                       // the previous block's products are done with Ts
-        store_T();
-        if (q + 1 < ndep) load_T(dep(q + 1));  // in flight while this dependency is awaited
         if (tid == 0) {
             unsigned int spins = 0;
             while (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -631,22 +642,29 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
             __threadfence();
         }
         __syncthreads();
-        if (s_bail) break;
-        {
-            zd xv[PT_KC * 64 / PT_THREADS];
+        if (s_bail) return;
+        zd xv[PT_KC * 64 / PT_THREADS];
 #pragma unroll
-            for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
-                const int idx = tid + PT_THREADS * u;
-                const int a = idx & 63, c = idx >> 6;
-                xv[u] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
-            }
+        for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+            const int idx = tid + PT_THREADS * u;
+            const int a = idx & 63, c = idx >> 6;
+            xv[u] = (a < mk && c < k) ? X[(ko + a) + ldx * c] : zzero();
+        }
 #pragma unroll
-            for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
-                const int idx = tid + PT_THREADS * u;
-                xs[idx & 63][idx >> 6] = xv[u];
-            }
+        for (int u = 0; u < PT_KC * 64 / PT_THREADS; ++u) {
+            const int idx = tid + PT_THREADS * u;
+            xs[idx & 63][idx >> 6] = xv[u];
         }
         __syncthreads();
+    };
+    pt_dv4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < nfar; ++q) {
+        __syncthreads();                       // the previous block's products are done with Ts
+        store_T();
+        if (q + 1 < nfar) load_T(dep(q + 1));  // in flight while this dependency is awaited
+        else load_E();
+        await_x(dep(q));
+        if (s_bail) break;
         pt_mfma_tile(acc, Ts, xs, w, lane);   // acc += T[j, kb] x_kb on the matrix cores
     }
     // bj -= acc: each real or imaginary part has exactly one owner lane
@@ -661,9 +679,21 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         }
     }
     __syncthreads();
-    // x_j = Ds bj, written straight to X (again one owner lane per real / imaginary part)
+    // c_j = Ds bj; then x_j = c_j - E_j x_nb once the neighbour has published
     pt_dv4 xo = {0.0, 0.0, 0.0, 0.0};
     pt_mfma_tile(xo, Ds, bj, w, lane);
+    if (ndep > 0 && !s_bail) {
+        __syncthreads();                       // the last far product is done with Ts and xs
+        store_T();
+        await_x(dep(ndep - 1));
+        if (!s_bail) {
+            pt_dv4 ex = {0.0, 0.0, 0.0, 0.0};
+            pt_mfma_tile(ex, Ts, xs, w, lane);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) xo[v] -= ex[v];
+        }
+    }
+    // x_j written straight to X (one owner lane per real / imaginary part)
     {
         double* xd = reinterpret_cast<double*>(X);
         const int jc = lane & 15;
@@ -680,15 +710,76 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __
         __hip_atomic_store(&flags[j], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
+
+// Chain blocks of the shifted solve, 64 x 64 column-major each (blockIdx.y = variant):
+//   variant 0: EU_j = D_j^{-1} T_{j,j+1}   (j < nblk - 1; used by the upper solve)
+//   variant 1: EL_j = (T_{j-1,j} D_j^{-1})^H   (j >= 1; used by the lower solve)
+// D_j^{-1} from k_tri_diag_inv; T blocks above the diagonal, columns past n read as zero.
+__global__ void __launch_bounds__(PT_THREADS)
+k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
+                   zd* __restrict__ EU, zd* __restrict__ EL) {
+    __shared__ zd As[64][65];
+    __shared__ zd Bs[64][PT_KC];
+    const int j = blockIdx.x, var = blockIdx.y;
+    const int nblk = (int)((n + 63) / 64);
+    if (var == 0 ? j >= nblk - 1 : j == 0) return;     // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const zd* Di = Dinv + (int64_t)j * 4096;
+    const int64_t o = (int64_t)j * 64;
+    for (int idx = tid; idx < 4096; idx += PT_THREADS) {
+        const int r = idx & 63, c = idx >> 6;
+        zd v;
+        if (var == 0) v = Di[idx];
+        else v = (o + c < n) ? T[(o - 64 + r) + ldt * (o + c)] : zzero();
+        As[r][c] = v;
+    }
+    zd* out = (var == 0 ? EU : EL) + (int64_t)j * 4096;
+    double* od = reinterpret_cast<double*>(out);
+    for (int c0 = 0; c0 < 64; c0 += PT_KC) {
+        __syncthreads();                       // As stored / the previous chunk's products done
+        for (int idx = tid; idx < 64 * PT_KC; idx += PT_THREADS) {
+            const int r = idx & 63, c = c0 + (idx >> 6);
+            zd v;
+            if (var == 0) v = (o + 64 + c < n) ? T[(o + r) + ldt * (o + 64 + c)] : zzero();
+            else v = Di[r + 64 * c];
+            Bs[r][idx >> 6] = v;
+        }
+        __syncthreads();
+        pt_dv4 acc = {0.0, 0.0, 0.0, 0.0};
+        pt_mfma_tile(acc, As, Bs, w, lane);
+        const int jc = lane & 15;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * w + (lane >> 4) + 4 * v, c = c0 + (jc & 7);
+            if (var == 0) od[2 * (i + 64 * c) + (jc >= 8 ? 1 : 0)] = acc[v];
+            else od[2 * (c + 64 * i) + (jc >= 8 ? 1 : 0)] = (jc >= 8) ? -acc[v] : acc[v];   // adjoint
+        }
+    }
+}
 }  // namespace
 
-void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* Dinv, zd* X,
-                            int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
+size_t tri_factor_elems(int64_t n) { return (size_t)3 * cdiv(n, 64) * 4096; }
+
+void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* F) {
+    const int nblk = cdiv(n, 64);
+    zd* EU = F + (size_t)nblk * 4096;
+    zd* EL = EU + (size_t)nblk * 4096;
+    hipLaunchKernelGGL(k_tri_diag_inv, dim3(nblk), dim3(64), 0, st, n, T, ldt, z, F);
+    LAUNCH_CHECK();
+    if (nblk > 1) {
+        hipLaunchKernelGGL(k_tri_chain_blocks, dim3(nblk, 2), dim3(PT_THREADS), 0, st, n, T, ldt, F, EU, EL);
+        LAUNCH_CHECK();
+    }
+}
+
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, const zd* TH, int64_t ldt, const zd* F,
+                            zd* X, int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
                             unsigned int epoch, int* err) {
     if (k > PT_KC) throw RlsError{-1, "trsm: k > 8 unsupported"};
     const int nblk = cdiv(n, 64);
-    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, T, ldt, Dinv, X, ldx, k,
-                       upper ? 1 : 0, counter, flags, epoch, err);
+    const zd* Ech = F + (size_t)(upper ? 1 : 2) * nblk * 4096;
+    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, upper ? T : TH, ldt, F, Ech, X,
+                       ldx, k, upper ? 1 : 0, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
 
