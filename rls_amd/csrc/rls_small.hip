@@ -22,6 +22,7 @@
 namespace {
 
 constexpr int HEEV_THREADS = 1024;
+constexpr int HEEV_PMAX = 4;    // rank-2 updates deferred at most this many steps
 
 // Number of eigenvalues < x of the symmetric tridiagonal (d, e), e2 = e^2.  The pivot
 // reciprocal uses the hardware estimate refined by one Newton step (only signs are used).
@@ -193,8 +194,14 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __shared__ zd s_alpha;
     __shared__ zd sv[1024];    // current Householder vector v_k
     __shared__ zd sw[1024];    // w_k
-    __shared__ zd pv[1024];    // pending update vectors (v_{k-1}, w_{k-1})
-    __shared__ zd pw[1024];
+    // pending rank-2 updates, deferred up to npmax steps: slot q holds (v, w) of an earlier
+    // step k_q at absolute row indices (offset q * n); on rows / columns after k_q the true
+    // matrix is M - sum_q (v_q w_q^H + w_q v_q^H)
+    __shared__ zd PVs[2 * HEEV_THREADS];
+    __shared__ zd PWs[2 * HEEV_THREADS];
+    __shared__ double s_red[HEEV_THREADS / 64][1 + 4 * HEEV_PMAX];
+    __shared__ double s_tot[1 + 4 * HEEV_PMAX];
+    __shared__ zd s_cw[HEEV_PMAX], s_cv[HEEV_PMAX];
     __shared__ int s_rr;
     __shared__ double s_part[HEEV_THREADS];
 
@@ -206,35 +213,75 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         if (i == j) v.y = 0.0;
         M[i + (int64_t)n * j] = v;
     }
-    for (int i = tid; i < 1024; i += nthr) { pv[i] = zzero(); pw[i] = zzero(); }
     __syncthreads();
 
-    // 2. tridiagonalisation (zhetd2 lower form).  The rank-2 update of step k-1 is applied
-    //    lazily inside the matrix-vector pass of step k (one read + one write per element
-    //    per step); the column of step k gets it on the fly.
-    bool pending = false;
+    // 2. tridiagonalisation (zhetd2 lower form).  The rank-2 updates are deferred: a step
+    //    whose pending list is not full only reads the trailing matrix for A22 v and corrects
+    //    the product with the pending vectors, sum_q (v_q (w_q^H v) + w_q (v_q^H v)); a step
+    //    with a full list applies every pending update inside its pass (one read + one write
+    //    per element) and empties the list.  Per element and step that is one read plus one
+    //    write every npmax steps instead of a read and a write every step.  The column of
+    //    step k and its diagonal entry get the pending updates on the fly.
+    const int lane = tid & 63, wv = tid >> 6;
+    const int npmax = min(HEEV_PMAX, (2 * HEEV_THREADS) / n);    // >= 2 for n <= 1023
+    int np = 0;
     for (int k = 0; k < n - 1; ++k) {
         const int m = n - k - 1;                 // x = A[k+1:n, k]
         zd* x = M + (k + 1) + (int64_t)n * k;
-        // x' = x - (pending update of step k-1 restricted to column k, rows k+1..), kept in a
-        // register of thread i (m < blockDim); in the k-1 frame row k+1+i <-> index i+1 and
-        // column k <-> index 0
+        const bool flush = (np == npmax);
         zd xi = zzero();
         if (tid < m) {
             xi = x[tid];
-            if (pending)
-                xi = zsub(xi, zadd(zmul(pv[tid + 1], zconj(pw[0])), zmul(pw[tid + 1], zconj(pv[0]))));
+            const int g = k + 1 + tid;
+            for (int q = 0; q < np; ++q) {
+                const zd* V = PVs + q * n;
+                const zd* W = PWs + q * n;
+                xi = zsub(xi, zadd(zmul(V[g], zconj(W[k])), zmul(W[g], zconj(V[k]))));
+            }
         }
         if (tid == 0) {
             double dk = M[k + (int64_t)n * k].x;
-            if (pending) dk -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
+            for (int q = 0; q < np; ++q) {
+                const zd v = PVs[q * n + k], w = PWs[q * n + k];
+                dk -= 2.0 * (v.x * w.x + v.y * w.y);
+            }
             d[k] = dk;
             s_alpha = xi;
         }
-        double part = (tid >= 1 && tid < m) ? zabs2(xi) : 0.0;
-        double xn2 = block_sum(part, red);
+        // one block reduction: |x'|^2 over rows k+2.., and (unless this step flushes) the
+        // dots w_q^H x', v_q^H x' over the same rows
+        {
+            const bool in = (tid >= 1 && tid < m);
+            const int nred = flush ? 1 : 1 + 4 * np;
+            const double x2 = wave_sum(in ? zabs2(xi) : 0.0);
+            if (lane == 0) s_red[wv][0] = x2;
+            if (!flush) {
+                for (int q = 0; q < np; ++q) {
+                    zd aw = zzero(), av = zzero();
+                    if (in) {
+                        const int g = k + 1 + tid;
+                        aw = zcmul(PWs[q * n + g], xi);
+                        av = zcmul(PVs[q * n + g], xi);
+                    }
+                    aw.x = wave_sum(aw.x); aw.y = wave_sum(aw.y);
+                    av.x = wave_sum(av.x); av.y = wave_sum(av.y);
+                    if (lane == 0) {
+                        s_red[wv][1 + 4 * q] = aw.x; s_red[wv][2 + 4 * q] = aw.y;
+                        s_red[wv][3 + 4 * q] = av.x; s_red[wv][4 + 4 * q] = av.y;
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid < nred) {
+                double acc = 0.0;
+                for (int w2 = 0; w2 < nthr / 64; ++w2) acc += s_red[w2][tid];
+                s_tot[tid] = acc;
+            }
+            __syncthreads();
+        }
+        const double xn2 = s_tot[0];
         const zd alpha = s_alpha;
-        zd t = zzero();
+        zd t = zzero(), scal = zzero();
         double beta;
         if (xn2 == 0.0 && alpha.y == 0.0) {
             beta = alpha.x;
@@ -243,80 +290,117 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
             double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
             beta = (alpha.x >= 0.0) ? -nrm : nrm;
             t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
-            zd scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+            scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
             if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zmul(xi, scal);
+        }
+        // w_q^H v and v_q^H v over rows k+1.. (v = e_0 + scal x' on rows k+2..)
+        if (!flush && tid < np) {
+            const zd w0 = PWs[tid * n + k + 1], v0 = PVs[tid * n + k + 1];
+            s_cw[tid] = zadd(zconj(w0), zmul(scal, zmk(s_tot[1 + 4 * tid], s_tot[2 + 4 * tid])));
+            s_cv[tid] = zadd(zconj(v0), zmul(scal, zmk(s_tot[3 + 4 * tid], s_tot[4 + 4 * tid])));
         }
         if (tid >= 1 && tid < m) x[tid] = sv[tid];   // keep v for the back-transform
         if (tid == 0) { e[k] = beta; tau[k] = t; }
         __syncthreads();
-        // fused pass over A22 = M[k+1:, k+1:]: apply the pending update, accumulate A22 v
+        // pass over A22 = M[k+1:, k+1:]: (flush) apply the pending updates, and A22 v
         zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
         const bool has_t = !(t.x == 0.0 && t.y == 0.0);
         // each row of A22 belongs to a pair of adjacent lanes (even / odd columns), combined
         // with one shuffle: no LDS reduction and no barrier inside the pass.  Columns are
-        // taken HB at a time with all HB loads issued before any is used, so a thread keeps
-        // HB memory requests in flight instead of one (HB = 4 keeps the chunk's operands in
-        // registers without spilling at 1024 threads).
+        // taken HB at a time with all HB loads issued before any is used.
         {
             constexpr int HB = 4;
             const int h = tid & 1;
             const int rows_per_pass = nthr >> 1;
+            const zd* PVc = PVs + (k + 1);   // column-side pending values at local index
+            const zd* PWc = PWs + (k + 1);
             for (int i0 = 0; i0 < m; i0 += rows_per_pass) {
                 const int i = i0 + (tid >> 1);
                 zd acc0 = zzero(), acc1 = zzero();
                 if (i < m) {
-                    // negated so the rank-2 update is two fused complex multiply-adds
-                    const zd npvi = pending ? zmk(-pv[i + 1].x, -pv[i + 1].y) : zzero();
-                    const zd npwi = pending ? zmk(-pw[i + 1].x, -pw[i + 1].y) : zzero();
-                    // full chunks: no bounds checks, every global and LDS load of the chunk
-                    // issued before the arithmetic
-                    int jb = h;
-                    for (; jb + 2 * (HB - 1) < m; jb += 2 * HB) {
-                        zd a[HB], sj[HB], pwj[HB], pvj[HB];
+                    if (flush) {
+                        zd nrv[HEEV_PMAX], nrw[HEEV_PMAX];   // row side, negated
 #pragma unroll
-                        for (int u = 0; u < HB; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
+                        for (int q = 0; q < HEEV_PMAX; ++q) {
+                            nrv[q] = zzero(); nrw[q] = zzero();
+                            if (q < np) {
+                                const zd v = PVc[q * n + i], w = PWc[q * n + i];
+                                nrv[q] = zmk(-v.x, -v.y);
+                                nrw[q] = zmk(-w.x, -w.y);
+                            }
+                        }
+                        constexpr int HF = 2;   // narrower chunks: the row side of the updates stays in registers
+                        int jb = h;
+                        for (; jb + 2 * (HF - 1) < m; jb += 2 * HF) {
+                            zd a[HF], sj[HF];
 #pragma unroll
-                        for (int u = 0; u < HB; ++u) sj[u] = sv[jb + 2 * u];
-                        if (pending) {
+                            for (int u = 0; u < HF; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
 #pragma unroll
-                            for (int u = 0; u < HB; ++u) {
-                                pwj[u] = pw[jb + 2 * u + 1];
-                                pvj[u] = pv[jb + 2 * u + 1];
+                            for (int u = 0; u < HF; ++u) sj[u] = sv[jb + 2 * u];
+#pragma unroll
+                            for (int q = 0; q < HEEV_PMAX; ++q) {
+                                if (q < np) {
+                                    zd cw[HF], cv[HF];
+#pragma unroll
+                                    for (int u = 0; u < HF; ++u) {
+                                        cw[u] = PWc[q * n + jb + 2 * u];
+                                        cv[u] = PVc[q * n + jb + 2 * u];
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < HF; ++u)
+                                        a[u] = zfma(nrw[q], zconj(cv[u]), zfma(nrv[q], zconj(cw[u]), a[u]));
+                                }
                             }
 #pragma unroll
-                            for (int u = 0; u < HB; ++u)
-                                a[u] = zfma(npwi, zconj(pvj[u]), zfma(npvi, zconj(pwj[u]), a[u]));
+                            for (int u = 0; u < HF; ++u) A22[i + (int64_t)n * (jb + 2 * u)] = a[u];
 #pragma unroll
-                            for (int u = 0; u < HB; ++u) A22[i + (int64_t)n * (jb + 2 * u)] = a[u];
+                            for (int u = 0; u < HF; u += 2) {
+                                acc0 = zfma(a[u], sj[u], acc0);
+                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
+                            }
                         }
+                        for (; jb < m; jb += 2) {
+                            zd a = A22[i + (int64_t)n * jb];
 #pragma unroll
-                        for (int u = 0; u < HB; u += 2) {
-                            acc0 = zfma(a[u], sj[u], acc0);
-                            acc1 = zfma(a[u + 1], sj[u + 1], acc1);
-                        }
-                    }
-                    for (; jb < m; jb += 2) {
-                        zd a = A22[i + (int64_t)n * jb];
-                        if (pending) {
-                            a = zfma(npwi, zconj(pv[jb + 1]), zfma(npvi, zconj(pw[jb + 1]), a));
+                            for (int q = 0; q < HEEV_PMAX; ++q)
+                                if (q < np)
+                                    a = zfma(nrw[q], zconj(PVc[q * n + jb]), zfma(nrv[q], zconj(PWc[q * n + jb]), a));
                             A22[i + (int64_t)n * jb] = a;
+                            acc0 = zfma(a, sv[jb], acc0);
                         }
-                        acc0 = zfma(a, sv[jb], acc0);
+                    } else {
+                        int jb = h;
+                        for (; jb + 2 * (HB - 1) < m; jb += 2 * HB) {
+                            zd a[HB], sj[HB];
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) sj[u] = sv[jb + 2 * u];
+#pragma unroll
+                            for (int u = 0; u < HB; u += 2) {
+                                acc0 = zfma(a[u], sj[u], acc0);
+                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
+                            }
+                        }
+                        for (; jb < m; jb += 2) acc0 = zfma(A22[i + (int64_t)n * jb], sv[jb], acc0);
                     }
                 }
                 zd acc = zadd(acc0, acc1);
                 acc.x += __shfl_xor(acc.x, 1);
                 acc.y += __shfl_xor(acc.y, 1);
-                if (h == 0 && i < m) sw[i] = zmul(t, acc);
+                if (h == 0 && i < m) {
+                    if (!flush) {
+                        for (int q = 0; q < np; ++q)
+                            acc = zsub(acc, zadd(zmul(PVc[q * n + i], s_cw[q]), zmul(PWc[q * n + i], s_cv[q])));
+                    }
+                    sw[i] = zmul(t, acc);
+                }
             }
         }
         __syncthreads();
-        if (!has_t) {
-            pending = false;
-            __syncthreads();
-            continue;
-        }
-        // alpha2 = -1/2 tau (w^H v); w += alpha2 v
+        if (flush) np = 0;
+        if (!has_t) continue;
+        // alpha2 = -1/2 tau (w^H v); w += alpha2 v; (v, w) joins the pending list
         double whv_r = 0.0, whv_i = 0.0;
         if (tid < m) {
             const zd c = zcmul(sw[tid], sv[tid]);
@@ -325,16 +409,19 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         block_sum2(whv_r, whv_i, red2s);
         zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
         for (int i = tid; i < m; i += nthr) {
-            pw[i] = zfma(a2, sv[i], sw[i]);
-            pv[i] = sv[i];
+            PWs[np * n + k + 1 + i] = zfma(a2, sv[i], sw[i]);
+            PVs[np * n + k + 1 + i] = sv[i];
         }
-        pending = true;
+        ++np;
         __syncthreads();
     }
     if (tid == 0) {
-        zd dl = M[(n - 1) + (int64_t)n * (n - 1)];
-        if (pending) dl.x -= 2.0 * (pv[0].x * pw[0].x + pv[0].y * pw[0].y);
-        d[n - 1] = dl.x;
+        double dl = M[(n - 1) + (int64_t)n * (n - 1)].x;
+        for (int q = 0; q < np; ++q) {
+            const zd v = PVs[q * n + n - 1], w = PWs[q * n + n - 1];
+            dl -= 2.0 * (v.x * w.x + v.y * w.y);
+        }
+        d[n - 1] = dl;
         e[n - 1] = 0.0;
         tau[n - 1] = zzero();
     }
@@ -444,7 +531,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     //    eigenvalues are closer than ortol are re-orthogonalised (one wave per cluster).
     const double ortol = 1e-9 * tnorm;
     const double floorv = eps * tnorm + pivmin;
-    const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
+    const int nw = nthr >> 6;
     for (int it = 0; it < 3; ++it) {
         for (int j = tid; j < nv; j += nthr) {
             double* y = Y + (int64_t)j * n;
