@@ -1546,6 +1546,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     __shared__ zd sv[HW_NMAX], pv[HW_NMAX + 1], pw[HW_NMAX + 1], tau[HW_NMAX];
     __shared__ double d[HW_NMAX], e[HW_NMAX], e2[HW_NMAX], lam[HW_NMAX];
     __shared__ int s_c[64], s_idx[64];
+    __shared__ zd s_bdot[4][64];                     // back-transform: wave partials of v^H g
     const int b = blockIdx.x;
     const int lane = threadIdx.x & 63;
     const int tid0 = threadIdx.x;
@@ -1834,18 +1835,37 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             Gs[i + ldl * jj] = zmk(Y[idx], 0.0);
         }
         __syncthreads();
-        if (wave0 && j < nc) {
+        {
+            // lane = vector j; the four waves split the rows by absolute index (wave w owns
+            // rows R = w mod 4, so no wave reads a row another wave updates); the wave
+            // partials of v^H g meet in LDS and are summed in a fixed order
+            const int w = tid0 >> 6;
             zd* g = Gs + ldl * j;
             for (int k = n - 2; k >= 0; --k) {
                 const zd tk = tau[k];
-                if (tk.x == 0.0 && tk.y == 0.0) continue;
-                const int m = n - k - 1;
+                if (tk.x == 0.0 && tk.y == 0.0) continue;   // uniform over the workgroup
                 const zd* hv = Hh + poff(n, k) + 1;      // v[0] = 1, v[1:] at hv[1..m)
-                zd dot = g[k + 1];
-                for (int i = 1; i < m; ++i) dot = zcfma(hv[i], g[k + 1 + i], dot);
+                const int R0 = k + 1 + ((w - (k + 1)) & 3);
+                zd part = zzero();
+                if (j < nc) {
+                    for (int R = R0; R < n; R += 4) {
+                        const int i = R - (k + 1);
+                        const zd vi = (i == 0) ? zmk(1.0, 0.0) : hv[i];
+                        part = zcfma(vi, g[R], part);
+                    }
+                }
+                s_bdot[w][lane] = part;
+                __syncthreads();
+                const zd dot = zadd(zadd(s_bdot[0][lane], s_bdot[1][lane]), zadd(s_bdot[2][lane], s_bdot[3][lane]));
+                __syncthreads();                             // partials read before the next write
                 const zd f = zmul(tk, dot);
-                g[k + 1] = zsub(g[k + 1], f);
-                for (int i = 1; i < m; ++i) g[k + 1 + i] = zsub(g[k + 1 + i], zmul(hv[i], f));
+                if (j < nc) {
+                    for (int R = R0; R < n; R += 4) {
+                        const int i = R - (k + 1);
+                        const zd vi = (i == 0) ? zmk(1.0, 0.0) : hv[i];
+                        g[R] = zsub(g[R], zmul(vi, f));
+                    }
+                }
             }
         }
         __syncthreads();

@@ -31,7 +31,7 @@ def _herm(n, seed, spread=1.0):
     return (Q * lam) @ Q.conj().T, lam
 
 
-@pytest.mark.parametrize("n,nvec", [(12, 12), (90, 20), (360, 40), (200, 200)])
+@pytest.mark.parametrize("n,nvec", [(12, 12), (90, 20), (120, 120), (128, 100), (360, 40), (200, 200)])
 def test_heev_matches_lapack(n, nvec):
     _say(f"heev n={n} nvec={nvec}")
     A, _ = _herm(n, n)
