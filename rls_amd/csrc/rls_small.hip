@@ -1325,31 +1325,45 @@ k_block_rot_seq(const zd* __restrict__ L, int64_t ld, int p, int r, zd* __restri
 // Left application, column by column: for each column c < ncols and i = 0..p-2,
 // [A_pb; A_i](:, c) <- U_i [A_pb; A_i](:, c)   (lmul!(bu, A), RLSLinearAlgebra.jl:60-92).
 struct SmallArrays { zd* a[4]; };
+template <int R>
 __global__ void __launch_bounds__(256)
-k_block_rot_rows(int ncols, SmallArrays arr, int64_t ld, int r, int p, const zd* __restrict__ U) {
+k_block_rot_rows(int ncols, SmallArrays arr, int64_t ld, int p, const zd* __restrict__ U) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     zd* A = arr.a[blockIdx.y] + ld * c;
-    const int pb = p - 1, m = 2 * r;
-    zd ap[16], ai[16], np_[16];
-    for (int s2 = 0; s2 < r; ++s2) ap[s2] = A[pb * r + s2];
+    const int pb = p - 1;
+    constexpr int m = 2 * R;
+    zd ap[R], ai[R];
+#pragma unroll
+    for (int s2 = 0; s2 < R; ++s2) ap[s2] = A[pb * R + s2];
     for (int i = 0; i < pb; ++i) {
         const zd* Ui = U + (int64_t)i * m * m;
-        for (int s2 = 0; s2 < r; ++s2) ai[s2] = A[i * r + s2];
-        for (int t = 0; t < r; ++t) {
+#pragma unroll
+        for (int s2 = 0; s2 < R; ++s2) ai[s2] = A[i * R + s2];
+        zd np_[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
             zd s1 = zzero(), sb = zzero();
-            for (int q = 0; q < r; ++q) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
                 s1 = zfma(Ui[t + m * q], ap[q], s1);
-                s1 = zfma(Ui[t + m * (r + q)], ai[q], s1);
-                sb = zfma(Ui[(r + t) + m * q], ap[q], sb);
-                sb = zfma(Ui[(r + t) + m * (r + q)], ai[q], sb);
+                s1 = zfma(Ui[t + m * (R + q)], ai[q], s1);
+                sb = zfma(Ui[(R + t) + m * q], ap[q], sb);
+                sb = zfma(Ui[(R + t) + m * (R + q)], ai[q], sb);
             }
             np_[t] = s1;
-            A[i * r + t] = sb;
+            A[i * R + t] = sb;
         }
-        for (int q = 0; q < r; ++q) ap[q] = np_[q];
+#pragma unroll
+        for (int q = 0; q < R; ++q) ap[q] = np_[q];
     }
-    for (int s2 = 0; s2 < r; ++s2) A[pb * r + s2] = ap[s2];
+#pragma unroll
+    for (int s2 = 0; s2 < R; ++s2) A[pb * R + s2] = ap[s2];
+}
+
+template <int R>
+void launch_rot_rows_r(hipStream_t st, int ncols, const SmallArrays& sa, int64_t ld, int p, const zd* U) {
+    hipLaunchKernelGGL(k_block_rot_rows<R>, dim3(cdiv(ncols, 256), 3), dim3(256), 0, st, ncols, sa, ld, p, U);
 }
 
 void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, int p, int r, zd* U) {
@@ -1360,7 +1374,14 @@ void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, i
     const int pr = p * r;
     SmallArrays sa{};
     sa.a[0] = L; sa.a[1] = H1; sa.a[2] = H2;
-    hipLaunchKernelGGL(k_block_rot_rows, dim3(cdiv(pr, 256), 3), dim3(256), 0, st, pr, sa, ld, r, p, U);
+    switch (r) {
+#define RLS_ROT_CASE(R) case R: launch_rot_rows_r<R>(st, pr, sa, ld, p, U); break;
+        RLS_ROT_CASE(1) RLS_ROT_CASE(2) RLS_ROT_CASE(3) RLS_ROT_CASE(4) RLS_ROT_CASE(5) RLS_ROT_CASE(6)
+        RLS_ROT_CASE(7) RLS_ROT_CASE(8) RLS_ROT_CASE(9) RLS_ROT_CASE(10) RLS_ROT_CASE(11) RLS_ROT_CASE(12)
+        RLS_ROT_CASE(13) RLS_ROT_CASE(14) RLS_ROT_CASE(15) RLS_ROT_CASE(16)
+#undef RLS_ROT_CASE
+        default: throw RlsError{-1, "block rotation: bad block width"};
+    }
     LAUNCH_CHECK();
     // right application to H1, H2 (H <- H U^H), same kernel as the tall arrays
     zd* hs[2] = {H1, H2};

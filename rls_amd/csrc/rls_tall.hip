@@ -304,36 +304,55 @@ __global__ void k_copy_cols(int64_t n, const zd* __restrict__ S, int64_t lds, zd
     if (i < n) D[i + ldd * j] = S[i + lds * j];
 }
 
-// Column side of the block unitary rotations, one thread per row of one array.
+// Column side of the block unitary rotations, one thread per row of one array.  The block
+// width R is a template parameter so the carried row blocks stay in registers; the next
+// block's loads are issued before the current rotation.
 constexpr int ROT_RMAX = 16;
 struct RotArrays { zd* a[8]; };
+template <int R>
 __global__ void __launch_bounds__(256)
-k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int r, int p, const zd* __restrict__ U) {
+k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int p, const zd* __restrict__ U) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
     zd* A = arr.a[blockIdx.y] + row;
     const int pb = p - 1;
-    zd ap[ROT_RMAX], ai[ROT_RMAX], np_[ROT_RMAX];
-    for (int c = 0; c < r; ++c) ap[c] = A[ld * (pb * r + c)];
-    const int r2 = 2 * r;
+    constexpr int R2 = 2 * R;
+    zd ap[R], ai[R], an[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ap[c] = A[ld * (pb * R + c)];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ai[c] = A[ld * c];
     for (int i = 0; i < pb; ++i) {
-        const zd* Ui = U + (int64_t)i * r2 * r2;   // U (2r x 2r), we apply U^H from the right
-        for (int c = 0; c < r; ++c) ai[c] = A[ld * (i * r + c)];
+        if (i + 1 < pb) {
+#pragma unroll
+            for (int c = 0; c < R; ++c) an[c] = A[ld * ((i + 1) * R + c)];
+        }
+        const zd* Ui = U + (int64_t)i * R2 * R2;   // U (2R x 2R), applied as U^H from the right
         // [np ni] = [ap ai] * U^H ; (U^H)[s, t] = conj(U[t, s])
-        for (int t = 0; t < r; ++t) {
+        zd np_[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
             zd s1 = zzero(), s2 = zzero();
-            for (int s = 0; s < r; ++s) {
-                s1 = zfma(ap[s], zconj(Ui[t + r2 * s]), s1);
-                s1 = zfma(ai[s], zconj(Ui[t + r2 * (r + s)]), s1);
-                s2 = zfma(ap[s], zconj(Ui[(r + t) + r2 * s]), s2);
-                s2 = zfma(ai[s], zconj(Ui[(r + t) + r2 * (r + s)]), s2);
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                s1 = zfma(ap[s], zconj(Ui[t + R2 * s]), s1);
+                s1 = zfma(ai[s], zconj(Ui[t + R2 * (R + s)]), s1);
+                s2 = zfma(ap[s], zconj(Ui[(R + t) + R2 * s]), s2);
+                s2 = zfma(ai[s], zconj(Ui[(R + t) + R2 * (R + s)]), s2);
             }
             np_[t] = s1;
-            A[ld * (i * r + t)] = s2;
+            A[ld * (i * R + t)] = s2;
         }
-        for (int c = 0; c < r; ++c) ap[c] = np_[c];
+#pragma unroll
+        for (int c = 0; c < R; ++c) { ap[c] = np_[c]; ai[c] = an[c]; }
     }
-    for (int c = 0; c < r; ++c) A[ld * (pb * r + c)] = ap[c];
+#pragma unroll
+    for (int c = 0; c < R; ++c) A[ld * (pb * R + c)] = ap[c];
+}
+
+template <int R>
+void launch_rot_cols_r(hipStream_t st, int64_t n, const RotArrays& ra, int narr, int64_t ld, int p, const zd* U) {
+    hipLaunchKernelGGL(k_block_rot_cols<R>, dim3(cdiv(n, 256), narr), dim3(256), 0, st, n, ra, ld, p, U);
 }
 
 }  // namespace
@@ -426,8 +445,14 @@ void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int nar
     if (narr > 8) throw RlsError{-1, "block rotation: too many arrays"};
     RotArrays ra;
     for (int i = 0; i < narr; ++i) ra.a[i] = arrays[i];
-    hipLaunchKernelGGL(k_block_rot_cols, dim3(cdiv(n, 256), narr), dim3(256), 0, st, n, ra, ld, r, p,
-                       U);
+    switch (r) {
+#define RLS_ROT_CASE(R) case R: launch_rot_cols_r<R>(st, n, ra, narr, ld, p, U); break;
+        RLS_ROT_CASE(1) RLS_ROT_CASE(2) RLS_ROT_CASE(3) RLS_ROT_CASE(4) RLS_ROT_CASE(5) RLS_ROT_CASE(6)
+        RLS_ROT_CASE(7) RLS_ROT_CASE(8) RLS_ROT_CASE(9) RLS_ROT_CASE(10) RLS_ROT_CASE(11) RLS_ROT_CASE(12)
+        RLS_ROT_CASE(13) RLS_ROT_CASE(14) RLS_ROT_CASE(15) RLS_ROT_CASE(16)
+#undef RLS_ROT_CASE
+        default: throw RlsError{-1, "block rotation: bad block width"};
+    }
     LAUNCH_CHECK();
 }
 
