@@ -49,29 +49,34 @@ k_tall_gemm(int64_t n, TallOperands ops, const zd* __restrict__ B, int64_t ldb, 
         off += Kq;
     }
     __shared__ zd red[TG_WAVES - 1][TG_ROWS][KC];
-    const bool two = (Y2 != nullptr);
-    for (int pass = 0; pass < (two ? 2 : 1); ++pass) {
-        zd* ac = pass == 0 ? acc : acc2;
+    // cross-wave sum of one accumulator set and its store (the set is passed by reference
+    // and the helper inlined, so the accumulators stay in registers)
+    auto finish = [&](const zd (&ac)[KC], bool first) {
         if (w > 0) {
 #pragma unroll
             for (int j = 0; j < KC; ++j) red[w - 1][lane][j] = ac[j];
         }
         __syncthreads();
         if (w == 0 && valid) {
-            for (int j = 0; j < kc; ++j) {
-                zd s = ac[j];
-                for (int v = 0; v < TG_WAVES - 1; ++v) s = zadd(s, red[v][lane][j]);
-                if (pass == 0) {
-                    zd* y = Y + (int64_t)b * sY + row + ldy * (c0 + j);
-                    s = zscale(s, alpha);
-                    *y = (beta != 0.0) ? zadd(zscale(*y, beta), s) : s;
-                } else {
-                    Y2[(int64_t)b * sY2 + row + ldy2 * (c0 + j)] = s;
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                if (j < kc) {
+                    zd s = ac[j];
+                    for (int v = 0; v < TG_WAVES - 1; ++v) s = zadd(s, red[v][lane][j]);
+                    if (first) {
+                        zd* y = Y + (int64_t)b * sY + row + ldy * (c0 + j);
+                        s = zscale(s, alpha);
+                        *y = (beta != 0.0) ? zadd(zscale(*y, beta), s) : s;
+                    } else {
+                        Y2[(int64_t)b * sY2 + row + ldy2 * (c0 + j)] = s;
+                    }
                 }
             }
         }
         __syncthreads();
-    }
+    };
+    finish(acc, true);
+    if (Y2 != nullptr) finish(acc2, false);
 }
 
 // Gram partials: part[chunk][a + ka*b] = sum_{rows in chunk} conj(A[i,a]) B[i,b]
