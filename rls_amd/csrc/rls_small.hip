@@ -1234,33 +1234,30 @@ void launch_recy_coef2(hipStream_t st, const zd* G, int64_t ldg, int64_t sG, con
     LAUNCH_CHECK();
 }
 
-// Full Householder QR of the 2r x r matrix Y in LDS; writes U = Q^H (2r x 2r) into Uo.
-__device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo, double* red) {
+// Full Householder QR of the 2r x r matrix Y in LDS (2r <= 32); writes U = Q^H (2r x 2r)
+// into Uo.  Run by one wave (blockDim.x == 64): lane i holds row i of the reflector's column.
+__device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo) {
     const int m = 2 * r;
     const int tid = threadIdx.x;
     for (int idx = tid; idx < m * m; idx += blockDim.x) Q[idx] = (idx % m == idx / m) ? zmk(1.0, 0.0) : zzero();
     __shared__ zd sv[32];
-    __shared__ zd s_tau;
     __syncthreads();
     for (int j = 0; j < r; ++j) {
-        // reflector for Y[j:m, j]
-        if (tid == 0) {
-            zd alpha = Y[j + m * j];
-            double xn2 = 0.0;
-            for (int i = j + 1; i < m; ++i) xn2 += zabs2(Y[i + m * j]);
-            if (xn2 == 0.0 && alpha.y == 0.0) {
-                s_tau = zzero();
-                for (int i = 0; i < m; ++i) sv[i] = (i == j) ? zmk(1.0, 0.0) : zzero();
-            } else {
-                double nrm = sqrt(zabs2(alpha) + xn2);
-                double beta = alpha.x >= 0 ? -nrm : nrm;
-                s_tau = zmk((beta - alpha.x) / beta, -alpha.y / beta);
-                zd sc = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
-                for (int i = 0; i < m; ++i) sv[i] = (i < j) ? zzero() : (i == j ? zmk(1.0, 0.0) : zmul(Y[i + m * j], sc));
-            }
+        // reflector for Y[j:m, j]: every lane forms alpha, beta and tau from one wave sum
+        const zd alpha = Y[j + m * j];
+        const zd yi = (tid > j && tid < m) ? Y[tid + m * j] : zzero();
+        const double xn2 = wave_sum(zabs2(yi));
+        zd t = zzero();
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            if (tid < m) sv[tid] = (tid == j) ? zmk(1.0, 0.0) : zzero();
+        } else {
+            const double nrm = sqrt(zabs2(alpha) + xn2);
+            const double beta = alpha.x >= 0 ? -nrm : nrm;
+            t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+            const zd sc = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+            if (tid < m) sv[tid] = (tid < j) ? zzero() : (tid == j ? zmk(1.0, 0.0) : zmul(yi, sc));
         }
         __syncthreads();
-        const zd t = s_tau;
         // Y <- H^H Y for columns >= j  (H = I - t v v^H)
         for (int c = j + tid; c < r; c += blockDim.x) {
             zd s = zzero();
@@ -1282,7 +1279,6 @@ __device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo, double* red) {
         Uo[a + m * c] = zconj(Q[c + m * a]);
     }
     __syncthreads();
-    (void)red;
 }
 
 // Small side of _block_unitary_update! (recySpace.jl:226-286).  Rotation i only touches
@@ -1290,16 +1286,27 @@ __device__ void lds_block_unitary(zd* Y, int r, zd* Q, zd* Uo, double* red) {
 // rotations before i, so the U_i sequence depends only on the evolving r x r block
 // L[pb, pb]: U_i = Q^H from the QR of [L[pb,pb]; L[i,pb]] (rls_block_unitary_rot!,
 // RLSLinearAlgebra.jl:164-191) and L[pb,pb] <- R.  One workgroup computes the sequence;
-// applying it to L, H1, H2 and the tall arrays is done by the row/column passes.
-__global__ void __launch_bounds__(256)
+// applying it to L, H1, H2 and the tall arrays is done by the row/column passes.  One
+// wave; the newest block column L[0 : pb r, pb r : p r] is staged in LDS up front when it
+// fits (ROTSEQ_STAGE complex values), so the sequence itself reads no global memory.
+constexpr int ROTSEQ_STAGE = 4096;
+__global__ void __launch_bounds__(64)
 k_block_rot_seq(const zd* __restrict__ L, int64_t ld, int p, int r, zd* __restrict__ Uout) {
     __shared__ zd Y[32 * 16];
     __shared__ zd Q[32 * 32];
     __shared__ zd rend[16 * 16];
-    __shared__ double red[4];
+    __shared__ zd Lc[ROTSEQ_STAGE];
     const int tid = threadIdx.x;
     const int pb = p - 1;
     const int m = 2 * r;
+    const int rows = pb * r;
+    const bool staged = (int64_t)rows * r <= ROTSEQ_STAGE;
+    if (staged) {
+        for (int idx = tid; idx < rows * r; idx += blockDim.x) {
+            const int a = idx % rows, c = idx / rows;
+            Lc[idx] = L[a + ld * (pb * r + c)];
+        }
+    }
     for (int idx = tid; idx < r * r; idx += blockDim.x) {
         const int a = idx % r, c = idx / r;
         rend[a + r * c] = L[(pb * r + a) + ld * (pb * r + c)];
@@ -1308,11 +1315,12 @@ k_block_rot_seq(const zd* __restrict__ L, int64_t ld, int p, int r, zd* __restri
     for (int i = 0; i < pb; ++i) {
         for (int idx = tid; idx < m * r; idx += blockDim.x) {
             const int a = idx % m, c = idx / m;
-            Y[a + m * c] = (a < r) ? rend[a + r * c] : L[(i * r + a - r) + ld * (pb * r + c)];
+            Y[a + m * c] = (a < r) ? rend[a + r * c]
+                                   : (staged ? Lc[(i * r + a - r) + rows * c] : L[(i * r + a - r) + ld * (pb * r + c)]);
         }
         __syncthreads();
         zd* U = Uout + (int64_t)i * m * m;
-        lds_block_unitary(Y, r, Q, U, red);
+        lds_block_unitary(Y, r, Q, U);
         // Y now holds [R; ~0] (Householder applied in place): rend <- R
         for (int idx = tid; idx < r * r; idx += blockDim.x) {
             const int a = idx % r, c = idx / r;
@@ -1369,7 +1377,7 @@ void launch_rot_rows_r(hipStream_t st, int ncols, const SmallArrays& sa, int64_t
 void launch_block_rot_small(hipStream_t st, zd* L, zd* H1, zd* H2, int64_t ld, int p, int r, zd* U) {
     if (p <= 1) return;
     if (r > 16) throw RlsError{-1, "block rotation: r > 16 unsupported"};
-    hipLaunchKernelGGL(k_block_rot_seq, dim3(1), dim3(256), 0, st, L, ld, p, r, U);
+    hipLaunchKernelGGL(k_block_rot_seq, dim3(1), dim3(64), 0, st, L, ld, p, r, U);
     LAUNCH_CHECK();
     const int pr = p * r;
     SmallArrays sa{};
