@@ -1639,11 +1639,13 @@ extern "C" int rls_test_trsm_time(int64_t n, const double* T, int k, int upper, 
     return rc;
 }
 
-// Sparse-path test hooks on a context with a CSR operator: Y = C X or C^H X (spmm), and the
-// factor solves X <- (C - zI)^{-1} X or (C - zI)^{-H} X with the factor last uploaded.
+// Operator test hooks: Y = M X or M^H X (which 0 / 1) for the dense or CSR operator, and,
+// with a CSR operator, the factor solves X <- (C - zI)^{-1} X or (C - zI)^{-H} X with the
+// factor last uploaded (which 2 / 3).
 extern "C" int rls_test_sparse_op(rls_ctx* ctx, const double* X, int k, int which, double* Y) {
     return guarded(ctx, [&]() -> int {
-        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "no sparse operator"};
+        if (ctx->n <= 0) throw RlsError{RLS_E_STATE, "no operator set"};
+        if (!ctx->sparse && which > 1) throw RlsError{RLS_E_STATE, "no sparse operator"};
         const int64_t n = ctx->n;
         DArr<zd> dX, dY;
         dX.ensure((size_t)n * k);

@@ -423,12 +423,16 @@ size_t trmm_part_elems(int64_t n) { return (size_t)cdiv(n, TRMM_SLICE) * 8 * n; 
 
 void launch_trmm_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,
                        int64_t ldx, int k, zd* Y, int64_t ldy, zd* part) {
-    if (k > 8) throw RlsError{-1, "trmm: k > 8 unsupported"};
     const int nsl = (int)cdiv(n, TRMM_SLICE);
-    hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), nsl), dim3(256), 0, st, n, T, ldt, X, ldx, k, part);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_trmm_reduce, dim3(cdiv(n, 256), k), dim3(256), 0, st, n, nsl, 8, k, part, Y, ldy);
-    LAUNCH_CHECK();
+    for (int c0 = 0; c0 < k; c0 += 8) {       // 8 columns per launch pair
+        const int kc = min(8, k - c0);
+        hipLaunchKernelGGL(k_trmm_upper<8>, dim3(cdiv(n, 64), nsl), dim3(256), 0, st, n, T, ldt, X + ldx * c0,
+                           ldx, kc, part);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_trmm_reduce, dim3(cdiv(n, 256), kc), dim3(256), 0, st, n, nsl, 8, kc, part,
+                           Y + ldy * c0, ldy);
+        LAUNCH_CHECK();
+    }
 }
 
 void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* X,

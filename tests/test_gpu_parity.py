@@ -192,6 +192,26 @@ def test_csr_products_match_scipy():
         np.testing.assert_allclose(s.apply_operator(X, adjoint=True), C.conj().T @ X, rtol=0, atol=1e-12)
 
 
+def test_dense_products_match_numpy():
+    """T X and T^H X of the dense operator (slice-partitioned triangular products; T^H from
+    the adjoint copy made at upload), with n not a multiple of the 512-column slice and
+    blocks wider than one 8-column launch."""
+    import ctypes as C
+    n = 1100
+    rng = np.random.default_rng(11)
+    T = np.triu(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    _say(f"dense products n={n}")
+    s = R.TriSolver(T)
+    f = s.ctx.lib.rls_test_sparse_op
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    for k in (1, 6, 8, 13):
+        X = np.asfortranarray(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+        for which, want in ((0, T @ X), (1, T.conj().T @ X)):
+            Y = np.zeros((n, k), dtype=np.complex128, order="F")
+            s.ctx.check(f(s.ctx.h, X.ctypes.data, k, which, Y.ctypes.data))
+            np.testing.assert_allclose(Y, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
 def test_resumed_solves_follow_the_same_trajectory():
     """A traversal split into calls (resume=True) gives the one-call trajectory, also when
     the second part runs in the other execution mode."""
