@@ -215,6 +215,11 @@ int rls_set_stream(rls_ctx* ctx, void* stream);
  *   RLS_OP_KINV       P L^{-H} L^{-1} P^T X ldiv!(K, B)          RLSPreconditioner.jl:79-110
  *   RLS_OP_LINV       L^{-1} X              InvMat(L)            psPrecRecy.jl:388-402
  *   RLS_OP_LINV_H     L^{-H} X              InvMat(L)'
+ * and, for a context holding a dense operator (rls_set_tri_dense):
+ *   RLS_OP_TRI_INV    (T - zI)^{-1} X       InvMat(R) of get_R_factor! psInvLanc.jl:158-181,
+ *                                            RLSMatrices.jl:390-436
+ *   RLS_OP_TRI_INV_H  (T - zI)^{-H} X       InvMat(R)'
+ * (the block factors of T - zI are kept while z is unchanged).
  * Bounded waits of the triangular solves report through rls_dev_sync. */
 #define RLS_OP_SHIFTED 0
 #define RLS_OP_SHIFTED_H 1
@@ -223,6 +228,8 @@ int rls_set_stream(rls_ctx* ctx, void* stream);
 #define RLS_OP_KINV 4
 #define RLS_OP_LINV 5
 #define RLS_OP_LINV_H 6
+#define RLS_OP_TRI_INV 7
+#define RLS_OP_TRI_INV_H 8
 int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* X, int64_t ldx, int k,
                   double* Y, int64_t ldy);
 /* Wait for the context's stream and report a triangular-solve wait that exceeded its bound. */

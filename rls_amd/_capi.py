@@ -45,6 +45,7 @@ EXPORTED = [
     "rls_dev_gram",
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
+RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 
 

@@ -372,9 +372,14 @@ struct rls_ctx {
     }
     // the refine operator's factor for shift z: dense -> diagonal-block inverses of T - zI;
     // sparse -> the host's factor callback, which uploads L, U (rls_set_sparse_factor)
+    // block factors of T - zI for the shift in tri_fz (valid while have_tri_fz)
+    bool have_tri_fz = false;
+    cplx tri_fz = cplx(0, 0);
     void refine_factor(cplx z) {
         if (!sparse) {
             launch_tri_factors(st, n, T.p, n, zmk(z.real(), z.imag()), tri_F.p);
+            have_tri_fz = true;
+            tri_fz = z;
             return;
         }
         if (!factor_cb) throw RlsError{RLS_E_STATE, "sparse operator: no factor callback set (rls_set_factor_callback)"};
@@ -1117,6 +1122,7 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         ctx->nkeep = 0;
         ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
         ctx->tri_ctr.release();
+        ctx->have_tri_fz = false;
         ctx->T.ensure((size_t)n * n);
         HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
                                  hipMemcpyHostToDevice, ctx->st));
@@ -1259,12 +1265,34 @@ int rls_set_stream(rls_ctx* ctx, void* stream) {
 int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* Xd, int64_t ldx, int k,
                   double* Yd, int64_t ldy) {
     return guarded(ctx, [&]() -> int {
-        if (!ctx->sparse) throw RlsError{RLS_E_STATE, "rls_dev_apply: no sparse operator set (rls_set_csr)"};
+        const bool tri_op = (op == RLS_OP_TRI_INV || op == RLS_OP_TRI_INV_H);
+        if (tri_op && (ctx->sparse || ctx->n <= 0 || !ctx->T.p))
+            throw RlsError{RLS_E_STATE, "rls_dev_apply: no dense operator set (rls_set_tri_dense)"};
+        if (!tri_op && !ctx->sparse) throw RlsError{RLS_E_STATE, "rls_dev_apply: no sparse operator set (rls_set_csr)"};
         const int64_t n = ctx->n;
         if (k < 0 || ldx < n || ldy < n || (k > 0 && (!Xd || !Yd)))
             throw RlsError{RLS_E_ARG, "rls_dev_apply: bad block shape"};
         if (Xd == Yd && k > 0) throw RlsError{RLS_E_ARG, "rls_dev_apply: Y must not alias X"};
         if (k == 0) return RLS_OK;
+        if (tri_op) {
+            // (T - zI)^{-1} X or (T - zI)^{-H} X: block factors of the shift (kept while the
+            // shift is unchanged), then the persistent solve on a copy of X
+            HIP_TRY(hipSetDevice(ctx->device));
+            const cplx z(z_re, z_im);
+            if (!ctx->have_tri_fz || ctx->tri_fz != z) {
+                ctx->tri_F.ensure(tri_factor_elems(n));
+                ctx->refine_factor(z);
+            }
+            zd* W = reinterpret_cast<zd*>(Yd);
+            if (ldy != n) {
+                ctx->trsm_tmp.ensure((size_t)n * k);
+                W = ctx->trsm_tmp.p;
+            }
+            launch_copy2d(ctx->st, n, k, reinterpret_cast<const zd*>(Xd), ldx, W, n);
+            ctx->tri_solve(W, k, op == RLS_OP_TRI_INV);
+            if (W != reinterpret_cast<zd*>(Yd)) launch_copy2d(ctx->st, n, k, W, n, reinterpret_cast<zd*>(Yd), ldy);
+            return RLS_OK;
+        }
         if (op >= RLS_OP_INVF && !ctx->have_ic) throw RlsError{RLS_E_STATE, "rls_dev_apply: no IC factor set"};
         HIP_TRY(hipSetDevice(ctx->device));
         const zd* X = reinterpret_cast<const zd*>(Xd);
