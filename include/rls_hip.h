@@ -215,11 +215,13 @@ int rls_set_stream(rls_ctx* ctx, void* stream);
  *   RLS_OP_KINV       P L^{-H} L^{-1} P^T X ldiv!(K, B)          RLSPreconditioner.jl:79-110
  *   RLS_OP_LINV       L^{-1} X              InvMat(L)            psPrecRecy.jl:388-402
  *   RLS_OP_LINV_H     L^{-H} X              InvMat(L)'
- * and, for a context holding a dense operator (rls_set_tri_dense):
- *   RLS_OP_TRI_INV    (T - zI)^{-1} X       InvMat(R) of get_R_factor! psInvLanc.jl:158-181,
- *                                            RLSMatrices.jl:390-436
- *   RLS_OP_TRI_INV_H  (T - zI)^{-H} X       InvMat(R)'
- * (the block factors of T - zI are kept while z is unchanged).
+ * and, for a context holding a dense operator T (rls_set_tri_dense) or a CSR operator C with
+ * a factor callback (rls_set_csr, rls_set_factor_callback), M = T or C:
+ *   RLS_OP_TRI_INV    (M - zI)^{-1} X       InvMat(R) of psInvLanc.jl:140-185 (dense R from
+ *                                            get_R_factor!, sparse R from qr(Mz); RLSMatrices.jl:390-436)
+ *   RLS_OP_TRI_INV_H  (M - zI)^{-H} X       InvMat(R)'
+ * (dense: the block factors of T - zI; sparse: the callback's L U of C - zI, whose
+ * triangular solves run on the device; kept while z is unchanged).
  * Bounded waits of the triangular solves report through rls_dev_sync. */
 #define RLS_OP_SHIFTED 0
 #define RLS_OP_SHIFTED_H 1

@@ -372,7 +372,8 @@ struct rls_ctx {
     }
     // the refine operator's factor for shift z: dense -> diagonal-block inverses of T - zI;
     // sparse -> the host's factor callback, which uploads L, U (rls_set_sparse_factor)
-    // block factors of T - zI for the shift in tri_fz (valid while have_tri_fz)
+    // the factor of the shift tri_fz is in place (dense: block factors of T - zI; sparse: the
+    // uploaded L U of C - zI), valid while have_tri_fz
     bool have_tri_fz = false;
     cplx tri_fz = cplx(0, 0);
     void refine_factor(cplx z) {
@@ -384,9 +385,12 @@ struct rls_ctx {
         }
         if (!factor_cb) throw RlsError{RLS_E_STATE, "sparse operator: no factor callback set (rls_set_factor_callback)"};
         factor_ready = false;
+        have_tri_fz = false;
         const int rc = factor_cb(factor_user, z.real(), z.imag());
         if (rc != 0 || !factor_ready)
             throw RlsError{RLS_E_NUMERIC, "factor callback failed for a refine shift"};
+        have_tri_fz = true;
+        tri_fz = z;
     }
     // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
@@ -1213,6 +1217,7 @@ int rls_set_sparse_factor(rls_ctx* ctx, const rls_csr* L, const rls_csr* U, cons
         upload(ctx, ctx->perm_r, perm_r, (size_t)n);
         upload(ctx, ctx->perm_c, perm_c, (size_t)n);
         ctx->sync();
+        ctx->have_tri_fz = false;   // set again by refine_factor when it made this upload
         ctx->factor_ready = true;
         return RLS_OK;
     });
@@ -1266,8 +1271,10 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
                   double* Yd, int64_t ldy) {
     return guarded(ctx, [&]() -> int {
         const bool tri_op = (op == RLS_OP_TRI_INV || op == RLS_OP_TRI_INV_H);
-        if (tri_op && (ctx->sparse || ctx->n <= 0 || !ctx->T.p))
-            throw RlsError{RLS_E_STATE, "rls_dev_apply: no dense operator set (rls_set_tri_dense)"};
+        if (tri_op && (ctx->n <= 0 || (!ctx->sparse && !ctx->T.p)))
+            throw RlsError{RLS_E_STATE, "rls_dev_apply: no operator set"};
+        if (tri_op && ctx->sparse && !ctx->factor_cb)
+            throw RlsError{RLS_E_STATE, "rls_dev_apply: sparse operator without a factor callback (rls_set_factor_callback)"};
         if (!tri_op && !ctx->sparse) throw RlsError{RLS_E_STATE, "rls_dev_apply: no sparse operator set (rls_set_csr)"};
         const int64_t n = ctx->n;
         if (k < 0 || ldx < n || ldy < n || (k > 0 && (!Xd || !Yd)))
@@ -1275,12 +1282,13 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
         if (Xd == Yd && k > 0) throw RlsError{RLS_E_ARG, "rls_dev_apply: Y must not alias X"};
         if (k == 0) return RLS_OK;
         if (tri_op) {
-            // (T - zI)^{-1} X or (T - zI)^{-H} X: block factors of the shift (kept while the
-            // shift is unchanged), then the persistent solve on a copy of X
+            // (M - zI)^{-1} X or (M - zI)^{-H} X with the factor of the shift (kept while the
+            // shift is unchanged): dense T, the block factors and the persistent solve; CSR
+            // C, the host LU from the factor callback and the device triangular solves
             HIP_TRY(hipSetDevice(ctx->device));
             const cplx z(z_re, z_im);
             if (!ctx->have_tri_fz || ctx->tri_fz != z) {
-                ctx->tri_F.ensure(tri_factor_elems(n));
+                if (!ctx->sparse) ctx->tri_F.ensure(tri_factor_elems(n));
                 ctx->refine_factor(z);
             }
             zd* W = reinterpret_cast<zd*>(Yd);
@@ -1289,7 +1297,8 @@ int rls_dev_apply(rls_ctx* ctx, int op, double z_re, double z_im, const double* 
                 W = ctx->trsm_tmp.p;
             }
             launch_copy2d(ctx->st, n, k, reinterpret_cast<const zd*>(Xd), ldx, W, n);
-            ctx->tri_solve(W, k, op == RLS_OP_TRI_INV);
+            if (ctx->sparse) ctx->sp_apply(W, k, op == RLS_OP_TRI_INV_H);
+            else ctx->tri_solve(W, k, op == RLS_OP_TRI_INV);
             if (W != reinterpret_cast<zd*>(Yd)) launch_copy2d(ctx->st, n, k, W, n, reinterpret_cast<zd*>(Yd), ldy);
             return RLS_OK;
         }

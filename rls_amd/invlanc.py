@@ -13,8 +13,10 @@ Lanczos bidiagonalization (the augmented form of Baglama and Reichel, SIAM J. Sc
 of R^{-1} and R^{-H} are the library's persistent triangular solves (rls_dev_apply with
 RLS_OP_TRI_INV / RLS_OP_TRI_INV_H, the block factors of T - zI kept per shift); the
 orthogonalization runs on the library's Gram and block-product kernels (rls_dev_gram,
-rls_dev_gemm).  Sparse input is densified through the Schur form, which leaves
-sigma_min(C - zI) unchanged.
+rls_dev_gemm).  Sparse input stays sparse (psInvLanc.jl:122-156): the reference's R from the
+sparse QR of C - zI becomes the LU Pr (C - zI) Pc = L U of SparseTriSolver's factor callback
+(host SuperLU), whose triangular solves run on the device; (C - zI)^{-1} = Pc U^{-1} L^{-1} Pr
+has the singular values of R^{-1}.
 """
 from __future__ import annotations
 
@@ -56,18 +58,25 @@ class PsInvLancStats:
 # ---- the vector operations the Lanczos process uses (largest_sv_inv takes any object with
 # n, block, put, cols, solve, gram, comb, from_host, to_host) ----
 class _DeviceBackend:
-    """Device blocks of a context holding T (rls_set_tri_dense): solves through rls_dev_apply,
-    Grams and combinations through rls_dev_gram / rls_dev_gemm, on one stream."""
+    """Device blocks of a context holding T (rls_set_tri_dense) or a sparse C with its factor
+    callback (SparseTriSolver): solves through rls_dev_apply, Grams and combinations through
+    rls_dev_gram / rls_dev_gemm, on one stream."""
 
-    def __init__(self, T: np.ndarray, device: int = 0):
+    def __init__(self, T: Optional[np.ndarray] = None, Csp=None, device: int = 0):
         import torch
         from .prec import _Blas
+        from . import SparseTriSolver
         self.torch = torch
         self.dev = torch.device("cuda", device)
-        self.ctx = _capi.Context(device)
-        T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
-        self.n = T.shape[0]
-        self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, self.n, self.n))
+        if Csp is not None:
+            self.sparse = SparseTriSolver(Csp, device)      # keeps the callback alive
+            self.ctx = self.sparse.ctx
+            self.n = self.sparse.n
+        else:
+            self.ctx = _capi.Context(device)
+            T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
+            self.n = T.shape[0]
+            self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, self.n, self.n))
         self.stream = torch.cuda.Stream(self.dev)
         self.ctx.check(self.ctx.lib.rls_set_stream(self.ctx.h, C.c_void_p(self.stream.cuda_stream)))
         self.la = _Blas(self)
@@ -217,14 +226,21 @@ def _rand_vec(n: int, seed: int, i: int) -> np.ndarray:
 def psInvLanc(C, rec: Recursion, control: Optional[PsInvLancControl] = None, *, T: Optional[np.ndarray] = None,
               device: int = 0):
     """sigma_min(C - zI) over rec.Z by inverse Lanczos on the device (psInvLanc.jl:87-261).
-    Returns (sigma_min, stats).  T: the Schur factor when already known (skips pre_process)."""
+    Returns (sigma_min, stats).  Dense C goes through its Schur factor (T when already known
+    skips pre_process); sparse C is factored per shift (stats.qr_time_ns holds the host
+    factorization time)."""
+    import scipy.sparse as sp
     ctl = control or PsInvLancControl()
     t0 = time.perf_counter_ns()
-    if T is None:
+    sparse_in = T is None and sp.issparse(C)
+    if sparse_in:
+        if C.shape[0] != C.shape[1]:
+            raise ValueError(f"psInvLanc only supports square sparse matrices, got size {C.shape}")
+    elif T is None:
         T = schur_preprocess(C)
     pre_ns = time.perf_counter_ns() - t0
-    n = T.shape[0]
-    be = _DeviceBackend(T, device)
+    be = _DeviceBackend(Csp=C, device=device) if sparse_in else _DeviceBackend(T=T, device=device)
+    n = be.n
     Z = np.asarray(rec.Z, dtype=np.complex128)
     nZ = len(Z)
     sig = np.zeros(nZ)
@@ -252,6 +268,7 @@ def psInvLanc(C, rec: Recursion, control: Optional[PsInvLancControl] = None, *, 
     with torch.cuda.device(be.dev), torch.cuda.stream(be.stream):
         run()
     be.ctx.check(be.ctx.lib.rls_dev_sync(be.ctx.h))
-    st = PsInvLancStats(mv=mv, n_pts=nZ, pts=Z.copy(), preprocess_time_ns=pre_ns,
-                        invlan_time_ns=time.perf_counter_ns() - t1, converged=conv)
+    qr_ns = int(be.sparse.factor_time_s * 1e9) if sparse_in else 0
+    st = PsInvLancStats(mv=mv, n_pts=nZ, pts=Z.copy(), preprocess_time_ns=pre_ns, qr_time_ns=qr_ns,
+                        invlan_time_ns=time.perf_counter_ns() - t1 - qr_ns, converged=conv)
     return sig, st

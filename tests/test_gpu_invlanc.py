@@ -23,3 +23,18 @@ def test_ps_inv_lanc_matches_svd():
         assert st.converged.all()
         assert np.max(np.abs(sig - tr) / tr) < 1e-8
         assert st.n_pts == len(rec.Z) and np.all(st.mv >= 2)
+
+
+def test_ps_inv_lanc_sparse_matches_svd():
+    """Sparse input: per-shift host LU (the factor callback) with device triangular solves."""
+    from rls_amd import workloads as WL
+    C = WL.sprand(n=700, density=8e-3, norm2=3.0)
+    n = C.shape[0]
+    rec = R.recursion_grid(-0.4, 0.4, -0.2, 0.2, 3, 2, 2)
+    print(f"[invlanc sparse] n={n} points={len(rec.Z)}", flush=True)
+    sig, st = IL.psInvLanc(C, rec, IL.PsInvLancControl(tol=1e-10))
+    Cd = C.toarray()
+    tr = np.array([np.linalg.svd(Cd - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
+    assert st.converged.all()
+    assert np.max(np.abs(sig - tr) / tr) < 1e-8
+    assert st.qr_time_ns > 0
