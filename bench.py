@@ -65,6 +65,8 @@ def main():
                     help="leading traversal points timed with the plain FP64 solver")
     ap.add_argument("--mid-points", type=int, default=2000,
                     help="points of a mid-grid segment timed with both solvers (0 = skip)")
+    ap.add_argument("--invlanc-points", type=int, default=40,
+                    help="leading traversal points also solved by psInvLanc on the device (0 = skip)")
     ap.add_argument("--truth-samples", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -223,6 +225,29 @@ def main():
         log(f"mid-grid points {P}..{P + nmid}: plain {nmid / t_mp:.0f} pts/s, batched {nmid / t_mb:.0f} pts/s "
             f"({t_mp / t_mb:.1f}x)")
 
+    # the reference's baseline, psInvLanc (inverse Lanczos per point, psInvLanc.jl:87-261), on
+    # the device over the leading points, with the reference's default control
+    inv = None
+    ninv = min(args.invlanc_points, nZ)
+    if ninv > 0:
+        from rls_amd import invlanc as IL
+
+        class _Points:
+            Z = rec.Z[:ninv]
+        ictl = IL.PsInvLancControl(tol=1e-4)
+        t0 = time.perf_counter()
+        if sparse:
+            sig_i, st_i = IL.psInvLanc(C, _Points, ictl)
+        else:
+            sig_i, st_i = IL.psInvLanc(None, _Points, ictl, T=T)
+        t_i = time.perf_counter() - t0
+        inv = {"points": int(ninv), "pts_per_s": ninv / t_i, "mv_mean": float(st_i.mv.mean()),
+               "converged": bool(st_i.converged.all()),
+               "max_rel_diff_vs_batched": float(np.max(np.abs(sig_i - sig[:ninv]) / sig[:ninv])),
+               "control": "tol=1e-4 krylovdim=20 maxiter=800, a new random start per point"}
+        log(f"psInvLanc {ninv} points: {ninv / t_i:.1f} pts/s, {st_i.mv.mean():.1f} operator "
+            f"applications per point")
+
     # accuracy against FP64 ground truth on sampled grid points: the full singular spectrum
     # (dense SVD) for n <= 2000, else block inverse iteration on the Schur factor.  Besides the
     # error against sigma_min, report the distance to the nearest singular value: the
@@ -299,6 +324,7 @@ def main():
         "plain_fp64": {"value": plain_pts, "points": npl, "batched_same_points": npl / t_b,
                        "speedup": (npl / t_b) / plain_pts, "max_rel_diff_vs_batched": plain_agree,
                        "mid_grid": mid},
+        "psInvLanc": inv,
         "refined_points": int((st.lobpcg_it > 0).sum()),
         "windows": int(st.n_windows),
         "preprocess_s": t_pre,
