@@ -29,15 +29,16 @@ constexpr double kShiftFactor = 2.0;   // hsl_preconditioner.jl:2
 constexpr double kLowAlpha = 1.0e-3;   // ALPHAM, hsl_preconditioner.jl:4
 constexpr double kSmall = 1e-20;       // mi35_control.small
 
-// Partition ix[1..n] so entries ncut..n reference moduli >= those of 1..ncut-1.
-void split2(int64_t n, int64_t ncut, int64_t* ix, const cplx* x) {
+// Partition ix[1..n] so entries ncut..n reference moduli >= those of 1..ncut-1 (mod[k] is
+// the modulus |x_k| of entry k, formed once per column).
+void split2(int64_t n, int64_t ncut, int64_t* ix, const double* mod) {
     if (n <= 1 || ncut <= 0 || ncut > n) return;
     int64_t first = 1, last = n;
     while (first < last) {
         int64_t mid = last;
-        const double midval = std::abs(x[ix[mid]]);
+        const double midval = mod[ix[mid]];
         for (int64_t j = last - 1; j >= first; --j) {
-            if (std::abs(x[ix[j]]) > midval) {
+            if (mod[ix[j]] > midval) {
                 --mid;
                 std::swap(ix[mid], ix[j]);
             }
@@ -191,6 +192,7 @@ int64_t ictkl3(const Lower& A, int64_t lsize, int64_t rsize, const std::vector<d
     cvec ar(nnzr + 2, cplx(0, 0));
     ivec startl(n + 2, 0), listl(n + 2, 0), startr(n + 2, 0), listr(n + 2, 0);
     cvec wr(n + 1, cplx(0, 0));
+    std::vector<double> wm(n + 1, 0.0);   // |wr[k]| of the column's touched rows
     ivec wn(n + 1, 0);
     int64_t saved = 0, savedr = 0;
     int64_t a_jstrt = A.ptr[1];
@@ -261,6 +263,7 @@ int64_t ictkl3(const Lower& A, int64_t lsize, int64_t rsize, const std::vector<d
         for (int64_t ii = 1; ii <= ind2; ++ii) {
             const int64_t k = wn[ii];
             wr[k] *= djtemp;
+            wm[k] = std::abs(wr[k]);
             startl[k] = 0;
         }
         const int64_t jsize = std::min(lsize + saved, ind2);
@@ -268,13 +271,13 @@ int64_t ictkl3(const Lower& A, int64_t lsize, int64_t rsize, const std::vector<d
         const int64_t largep = ind2 - jsize + 1;
         int64_t lind = il[j] + 1, rind = ir[j];
         if (ind2 >= 1) {
-            split2(ind2, largep, wn.data(), wr.data());
+            split2(ind2, largep, wn.data(), wm.data());
             srtshai2(jsize, wn.data() + (largep - 1));
             int64_t indr = largep;
             for (int64_t i = largep; i <= ind2; ++i) {
                 const int64_t k = wn[i];
                 const cplx temp = wr[k];
-                const double at = std::abs(temp);
+                const double at = wm[k];
                 if (at >= tau1) {
                     d[k] -= std::norm(temp);
                     if (d[k] <= kSmall && k > j) return -j;
@@ -287,13 +290,13 @@ int64_t ictkl3(const Lower& A, int64_t lsize, int64_t rsize, const std::vector<d
             }
             const int64_t radd = std::min(indr - 1, rsize + savedr);
             const int64_t larger = (indr - 1) - radd + 1;
-            split2(indr - 1, larger, wn.data(), wr.data());
+            split2(indr - 1, larger, wn.data(), wm.data());
             savedr = savedr + rsize - radd;
             for (int64_t ii = larger; ii <= indr - 1; ++ii) {
                 const int64_t k = wn[ii];
                 const cplx temp = wr[k];
                 wr[k] = cplx(0, 0);
-                if (tau2 == 0.0 || std::abs(temp) > tau2) {
+                if (tau2 == 0.0 || wm[k] > tau2) {
                     ar[rind] = temp;
                     jr[rind] = k;
                     ++rind;
