@@ -34,6 +34,18 @@ def test_library_exports_every_declared_symbol():
     assert set(_capi.EXPORTED) <= set(decl)
 
 
+def test_header_constants_match_python_mirror():
+    """Every RLS_* integer constant that rls_amd._capi mirrors has the header's value."""
+    src = open(os.path.join(ROOT, "include", "rls_hip.h")).read()
+    consts = {m.group(1): int(m.group(2))
+              for m in re.finditer(r"#define\s+(RLS_[A-Z0-9_]+)\s+(-?\d+)\b", src)}
+    mirrored = [k for k in dir(_capi) if k.startswith("RLS_") and isinstance(getattr(_capi, k), int)]
+    assert len(mirrored) >= 14
+    for name in mirrored:
+        assert name in consts, f"{name} is not defined in include/rls_hip.h"
+        assert getattr(_capi, name) == consts[name], name
+
+
 def test_control_defaults_match_reference():
     """rls_tri_control_default mirrors PsTriRecyControl's defaults (psTriRecy.jl:350-370)."""
     if not os.path.exists(_capi.LIB_PATH):
