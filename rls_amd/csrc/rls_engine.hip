@@ -68,7 +68,8 @@ struct rls_ctx {
     // operator: dense upper-triangular Schur factor T, or (sparse path) C in CSR with the CSR
     // of C^H, and the host-supplied factor Pr (C - zI) Pc = L U of the current refine shift
     int64_t n = 0;
-    DArr<zd> T, TH;   // dense upper-triangular operator and its adjoint (column-major, ld n)
+    DArr<zd> T, TT;   // dense upper-triangular operator (column-major, ld n) and its upper
+                      // triangle in 64 x 64 tiles for the shifted solves
     bool sparse = false;
     DArr<int64_t> crp, cci, hrp, hci;
     DArr<zd> cv, hv;
@@ -172,7 +173,7 @@ struct rls_ctx {
     }
 
     ~rls_ctx() {
-        DArr<zd>* zs[] = {&T, &TH, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &tri_F, &trsm_tmp, &trmm_part,
+        DArr<zd>* zs[] = {&T, &TT, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &tri_F, &trsm_tmp, &trmm_part,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
                           &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev,
                           &vkeep, &zkeep, &Qs, &gY, &gU, &gUq, &gVq};
@@ -343,7 +344,7 @@ struct rls_ctx {
             int kc = std::min(8, k - c0);
             ++tri_epoch;
             kt_begin(RLS_KF_TRSM);
-            launch_trsm_persistent(st, n, T.p, TH.p, n, tri_F.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
+            launch_trsm_persistent(st, n, TT.p, tri_F.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
                                    tri_flags.p, tri_epoch, tri_err.p);
             // algorithmic: the triangle of T once + the RHS block read and written
             kt_end(RLS_KF_TRSM, 8.0 * 0.5 * (double)n * n * kc,
@@ -1130,9 +1131,9 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         ctx->T.ensure((size_t)n * n);
         HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
                                  hipMemcpyHostToDevice, ctx->st));
-        // T^H for the lower solves, which then read column blocks like the upper ones
-        ctx->TH.ensure((size_t)n * n);
-        launch_conj_transpose(ctx->st, (int)n, (int)n, ctx->T.p, n, ctx->TH.p, n);
+        // the tiles of the shifted solves (contiguous 64 KB blocks for both directions)
+        ctx->TT.ensure(tri_tiles_elems(n));
+        launch_tile_upper(ctx->st, n, ctx->T.p, n, ctx->TT.p);
         ctx->sync();
         return RLS_OK;
     });
@@ -1170,7 +1171,7 @@ int rls_set_csr(rls_ctx* ctx, const rls_csr* C) {
             }
         ctx->n = n;
         ctx->T.release();
-        ctx->TH.release();
+        ctx->TT.release();
         ctx->sparse = true;
         ctx->resumable = false;
         ctx->have_q = false;

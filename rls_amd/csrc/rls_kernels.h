@@ -88,12 +88,14 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B);
 void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
                        int B);
-// Persistent single-launch shifted triangular solve (see rls_tall.hip) with T, its adjoint TH
-// (both n x n, ldt) and the block factors F of launch_tri_factors; counter/flags are device
+// Upper triangle of T (n x n, ldt) in 64 x 64 tiles: tri_tiles_elems(n) complex values.
+size_t tri_tiles_elems(int64_t n);
+void launch_tile_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd* TT);
+// Persistent single-launch shifted triangular solve (see rls_tall.hip) with the tiles TT of
+// launch_tile_upper and the block factors F of launch_tri_factors; counter/flags are device
 // words owned by the caller (flags: one per 64-row block), epoch increases per call.
-void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, const zd* TH, int64_t ldt, const zd* F,
-                            zd* X, int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
-                            unsigned int epoch, int* err);
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* TT, const zd* F, zd* X, int64_t ldx, int k,
+                            bool upper, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err);
 void read_heev_phases(long long* out);
 // Tiled FP64 complex GEMM Y = [A_q] * B for wide B (batched recycle products).
 void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,

@@ -547,8 +547,10 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 // j-1) and forms c_j = D_j^{-1} (b_j - sum T_jq x_q); the neighbour's solution then only
 // costs one 64 x 64 product with the precomputed chain block E_j (upper: D_j^{-1} T_{j,j+1};
 // lower: (T_{j-1,j} D_j^{-1})^H): x_j = c_j - E_j x_nb, which keeps the diagonal-block
-// solve off the dependency chain.  The lower solve reads the off-diagonal blocks from TH =
-// T^H (column-major), so both directions stream column blocks of their factor.  Block j publishes its solution by storing `epoch` into flag[j] after a device
+// solve off the dependency chain.  The off-diagonal blocks come from TT, the upper triangle
+// of T in 64 x 64 tiles (tile (bi, bj), bi <= bj, at tri_tile_index(bi, bj) * 4096, each
+// tile column-major and zero-padded past n): every block load is one contiguous 64 KB
+// tile, and both directions read the same tiles.  Block j publishes its solution by storing `epoch` into flag[j] after a device
 // fence; readers spin (bounded) on an agent-scope atomic load, then fence.  The counter is
 // never reset: tickets of solve e are [e*nblk, (e+1)*nblk) because launches on one stream
 // run in order.  A spin that exceeds its bound sets *err and the kernel still terminates.
@@ -580,8 +582,12 @@ __device__ __forceinline__ void pt_mfma_tile(pt_dv4& acc, const zd (*A)[65], con
     for (int v = 0; v < 4; ++v) acc[v] += acc2[v];
 }
 
+__device__ __forceinline__ int64_t tri_tile_index(int bi, int bj) {   // bi <= bj
+    return (int64_t)bj * (bj + 1) / 2 + bi;
+}
+
 __global__ void __launch_bounds__(PT_THREADS)
-k_trsm_persistent(int64_t n, const zd* __restrict__ Tsrc, int64_t ldt, const zd* __restrict__ Dinv,
+k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ Dinv,
                   const zd* __restrict__ Ech, zd* X, int64_t ldx, int k, int upper, unsigned int* counter,
                   unsigned int* flags, unsigned int epoch, int* err) {
     constexpr int PER = 4096 / PT_THREADS;      // T / Dinv block elements per thread
@@ -603,24 +609,25 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ Tsrc, int64_t ldt, const zd*
     const int j = upper ? nblk - 1 - t : t;
     const int64_t o = (int64_t)j * 64;
     const int m = (int)min((int64_t)64, n - o);
-    // block (j, kb) of the factor into registers, all loads issued before any is consumed:
-    // Ts[a][c] = Tsrc[o + a, ko + c] with Tsrc = T (upper) or T^H (lower); lanes over a
+    // tile of dependency kb into registers (one contiguous 64 KB tile; all loads issued before
+    // any is consumed): upper T_{j,kb} = tile (j, kb); lower T_{kb,j} = tile (kb, j), conjugated
+    // here and transposed by store_T so that Ts holds T_{kb,j}^H
     zd pre[PER];
     auto load_T = [&](int kb) {
-        const int64_t ko = (int64_t)kb * 64;
-        const int mk = (int)min((int64_t)64, n - ko);
+        const zd* tile = TT + (upper ? tri_tile_index(j, kb) : tri_tile_index(kb, j)) * 4096;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int idx = tid + PT_THREADS * u;
-            const int i0 = idx & 63, i1 = idx >> 6;
-            pre[u] = (i0 < m && i1 < mk) ? Tsrc[(o + i0) + ldt * (ko + i1)] : zzero();
+            const zd v = tile[tid + PT_THREADS * u];
+            pre[u] = upper ? v : zconj(v);
         }
     };
-    auto store_T = [&]() {
+    // is_e: pre holds a chain block E_j (already in Ts's orientation)
+    auto store_T = [&](bool is_e) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int idx = tid + PT_THREADS * u;
-            Ts[idx & 63][idx >> 6] = pre[u];
+            if (upper || is_e) Ts[idx & 63][idx >> 6] = pre[u];
+            else Ts[idx >> 6][idx & 63] = pre[u];
         }
     };
     // chain block E_j of block j (column-major) into the same registers
@@ -694,7 +701,7 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ Tsrc, int64_t ldt, const zd*
     pt_dv4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int q = 0; q < nfar; ++q) {
         __syncthreads();                       // the previous block's products are done with Ts
-        store_T();
+        store_T(false);
         if (q + 1 < nfar) load_T(dep(q + 1));  // in flight while this dependency is awaited
         else load_E();
         await_x(dep(q));
@@ -718,7 +725,7 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ Tsrc, int64_t ldt, const zd*
     pt_mfma_tile(xo, Ds, bj, w, lane);
     if (ndep > 0 && !s_bail) {
         __syncthreads();                       // the last far product is done with Ts and xs
-        store_T();
+        store_T(true);
         await_x(dep(ndep - 1));
         if (!s_bail) {
             pt_dv4 ex = {0.0, 0.0, 0.0, 0.0};
@@ -806,14 +813,41 @@ void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd 
     }
 }
 
-void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* T, const zd* TH, int64_t ldt, const zd* F,
-                            zd* X, int64_t ldx, int k, bool upper, unsigned int* counter, unsigned int* flags,
-                            unsigned int epoch, int* err) {
+namespace {
+// Upper triangle of T into 64 x 64 tiles (see k_trsm_persistent), one workgroup per tile.
+__global__ void __launch_bounds__(256)
+k_tile_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, zd* __restrict__ TT) {
+    const int t = blockIdx.x;
+    int bj = 0;
+    while ((int64_t)(bj + 1) * (bj + 2) / 2 <= t) ++bj;
+    const int bi = t - bj * (bj + 1) / 2;
+    const int64_t r0 = (int64_t)bi * 64, c0 = (int64_t)bj * 64;
+    zd* tile = TT + (int64_t)t * 4096;
+    for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        tile[idx] = (r0 + r < n && c0 + c < n) ? T[(r0 + r) + ldt * (c0 + c)] : zzero();
+    }
+}
+}  // namespace
+
+size_t tri_tiles_elems(int64_t n) {
+    const int64_t nb = cdiv(n, 64);
+    return (size_t)(nb * (nb + 1) / 2) * 4096;
+}
+
+void launch_tile_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd* TT) {
+    const int64_t nb = cdiv(n, 64);
+    hipLaunchKernelGGL(k_tile_upper, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, st, n, T, ldt, TT);
+    LAUNCH_CHECK();
+}
+
+void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* TT, const zd* F, zd* X, int64_t ldx, int k,
+                            bool upper, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
     if (k > PT_KC) throw RlsError{-1, "trsm: k > 8 unsupported"};
     const int nblk = cdiv(n, 64);
     const zd* Ech = F + (size_t)(upper ? 1 : 2) * nblk * 4096;
-    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, upper ? T : TH, ldt, F, Ech, X,
-                       ldx, k, upper ? 1 : 0, counter, flags, epoch, err);
+    hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, TT, F, Ech, X, ldx, k,
+                       upper ? 1 : 0, counter, flags, epoch, err);
     LAUNCH_CHECK();
 }
 
