@@ -193,9 +193,9 @@ def test_csr_products_match_scipy():
 
 
 def test_dense_products_match_numpy():
-    """T X and T^H X of the dense operator (slice-partitioned triangular products; T^H from
-    the adjoint copy made at upload), with n not a multiple of the 512-column slice and
-    blocks wider than one 8-column launch."""
+    """T X (slice-partitioned product with a fixed-order slice reduction) and T^H X (one
+    wave per output row) of the dense operator, with n not a multiple of the 512-column
+    slice and blocks wider than one 8-column launch."""
     import ctypes as C
     n = 1100
     rng = np.random.default_rng(11)
