@@ -33,7 +33,11 @@ from . import Recursion, schur_preprocess
 
 @dataclass
 class PsInvLancControl:
-    """PsInvLancControl (psInvLanc.jl:8-15)."""
+    """PsInvLancControl (psInvLanc.jl:8-15).  `ordering` is kept for the interface: the
+    reference's sparse branch passes ordering = 0 to its QR whatever the field holds
+    (psInvLanc.jl:155), and the sparse LU here uses a COLAMD column ordering; sigma does not
+    depend on the ordering.  `seed` selects the start vectors (the reference draws them from
+    Julia's global RNG)."""
     tol: float = 1e-4
     maxiter: int = 800
     krylovdim: int = 20
