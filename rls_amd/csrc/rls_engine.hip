@@ -81,9 +81,10 @@ struct rls_ctx {
     };
     SpTri fL, fU, fLH, fUH;
     DArr<int64_t> perm_r, perm_c;
-    DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block)]
+    DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block and 8-column chunk)]
     DArr<int> sp_err;
     unsigned int sp_epoch = 0;
+    unsigned int sp_tbase = 0;   // first ticket of the next sparse solve (sp_cnt[0] counts them)
     // psPrecRecy preconditioner K = P L L^H P^T: CSR of L and of L^H, p and invp
     SpTri icL, icLH;
     DArr<int64_t> ic_p, ic_invp;
@@ -396,16 +397,18 @@ struct rls_ctx {
     // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
         const int nb = (int)cdiv(n, sptrsv_block_rows());
-        if (!sp_cnt.p || sp_cnt.n < (size_t)nb + 1) {
-            sp_cnt.ensure((size_t)nb + 1);
+        const size_t nflags = (size_t)nb * (size_t)((k + 7) / 8);   // one per (row block, chunk)
+        if (!sp_cnt.p || sp_cnt.n < nflags + 1) {
+            sp_cnt.ensure(nflags + 1);
             sp_err.ensure(1);
             HIP_TRY(hipMemsetAsync(sp_cnt.p, 0, sizeof(unsigned int) * sp_cnt.n, st));
             HIP_TRY(hipMemsetAsync(sp_err.p, 0, sizeof(int), st));
             sp_epoch = 0;
+            sp_tbase = 0;
         }
         kt_begin(RLS_KF_TRSM);
-        launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, sp_cnt.p, sp_cnt.p + 1,
-                      ++sp_epoch, sp_err.p);
+        sp_tbase += launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, sp_cnt.p, sp_tbase,
+                                  sp_cnt.p + 1, ++sp_epoch, sp_err.p);
         // algorithmic: the factor once per 8-column chunk (values, column indices, row
         // pointers) + the block read and written
         const double chunks = (double)((k + 7) / 8);

@@ -63,8 +63,11 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 }
 
 // Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
-// ascending order, upper: descending).  One workgroup per block of SP_RB rows, started in
-// dependency order by a ticket counter.  Everything that does not depend on other blocks is
+// ascending order, upper: descending).  One workgroup per block of SP_RB rows and 8-column
+// chunk of the right-hand sides (ticket t = block step * nch + chunk, so a workgroup only
+// waits for tickets taken before its own; tickets of a launch start at tbase), with one
+// flag per (block, chunk): the chains of the chunks run side by side, each moving only its
+// own columns.  Everything that does not depend on other blocks is
 // done before the block's dependency wait, which a block started early spends idle: each
 // row's entries are staged in LDS (SP_TPR threads per row, SP_SLOTS entries each), the
 // entries inside the block go to a dense tile D and the reciprocal diagonal is formed.  Then
@@ -82,7 +85,8 @@ template <bool LOWER>
 __global__ void __launch_bounds__(SP_THREADS)
 k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
          const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t rs,
-         int64_t cs, int k, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
+         int64_t cs, int k, int nch, unsigned int* counter, unsigned int tbase, unsigned int* flags,
+         unsigned int epoch, int* err) {
     __shared__ unsigned int s_ticket;
     __shared__ int s_ok;
     __shared__ zd xs[SP_RB][8];
@@ -97,8 +101,9 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     if (tid == 0) { s_ticket = atomicAdd(counter, 1u); s_ok = 1; }
     for (int q = tid; q < SP_RB * (SP_RB + 1); q += SP_THREADS) (&D[0][0])[q] = zzero();
     __syncthreads();
-    const int t = (int)(s_ticket % (unsigned)nblk);
-    const int b = LOWER ? t : nblk - 1 - t;
+    const int t = (int)(s_ticket - tbase);      // 0 .. nblk * nch - 1
+    const int bt = t / nch, ch = t % nch;
+    const int b = LOWER ? bt : nblk - 1 - bt;
     const int64_t r0 = (int64_t)b * SP_RB;
     const int rows = (int)min((int64_t)SP_RB, n - r0);
     const int li = tid / SP_TPR, sub = tid % SP_TPR;   // row of the block, entry phase
@@ -147,7 +152,8 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             const int q = q0 + lane;
             if (q < hi && ok) {
                 unsigned int spins = 0;
-                while (__hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                while (__hip_atomic_load(&flags[(int64_t)q * nch + ch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                       epoch) {
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > (1u << 24)) { ok = false; atomicExch(err, 1); break; }
                 }
@@ -159,8 +165,8 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     __syncthreads();
     if (!s_ok) return;
-    // the right-hand sides in chunks of 8 columns
-    for (int c0 = 0; c0 < k; c0 += 8) {
+    // this workgroup's chunk of 8 right-hand-side columns
+    for (int c0 = 8 * ch; c0 < k && c0 < 8 * ch + 8; c0 += 8) {
         const int kc = min(8, k - c0);
         zd* Xc = X + cs * c0;
         if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
@@ -248,7 +254,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     if (tid == 0) {
         __threadfence();
-        __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&flags[(int64_t)b * nch + ch], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -283,16 +289,20 @@ void launch_permute_rows_strided(hipStream_t st, int64_t n, const int64_t* p, co
 
 int sptrsv_block_rows() { return SP_RB; }
 
-void launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v, bool lower,
-                   bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k, unsigned int* counter,
-                   unsigned int* flags, unsigned int epoch, int* err) {
-    if (k <= 0) return;
+unsigned int launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v,
+                           bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k,
+                           unsigned int* counter, unsigned int tbase, unsigned int* flags, unsigned int epoch,
+                           int* err) {
+    if (k <= 0) return 0;
     const int nblk = (int)cdiv(n, SP_RB);
+    const int nch = (k + 7) / 8;
+    const unsigned int nt = (unsigned int)nblk * (unsigned int)nch;
     if (lower)
-        hipLaunchKernelGGL(k_sptrsv<true>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
-                           rs, cs, k, counter, flags, epoch, err);
+        hipLaunchKernelGGL(k_sptrsv<true>, dim3(nt), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
+                           rs, cs, k, nch, counter, tbase, flags, epoch, err);
     else
-        hipLaunchKernelGGL(k_sptrsv<false>, dim3(nblk), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
-                           rs, cs, k, counter, flags, epoch, err);
+        hipLaunchKernelGGL(k_sptrsv<false>, dim3(nt), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep,
+                           X, rs, cs, k, nch, counter, tbase, flags, epoch, err);
     LAUNCH_CHECK();
+    return nt;
 }
