@@ -81,7 +81,7 @@ struct rls_ctx {
     };
     SpTri fL, fU, fLH, fUH;
     DArr<int64_t> perm_r, perm_c;
-    DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block and 8-column chunk)]
+    DArr<unsigned int> sp_cnt;   // [ticket | flags (one per row block and chunk of columns)]
     DArr<int> sp_err;
     unsigned int sp_epoch = 0;
     unsigned int sp_tbase = 0;   // first ticket of the next sparse solve (sp_cnt[0] counts them)
@@ -397,7 +397,8 @@ struct rls_ctx {
     // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
         const int nb = (int)cdiv(n, sptrsv_block_rows());
-        const size_t nflags = (size_t)nb * (size_t)((k + 7) / 8);   // one per (row block, chunk)
+        const int cw = sptrsv_chunk_width(k);
+        const size_t nflags = (size_t)nb * (size_t)((k + cw - 1) / cw);   // one per (row block, chunk)
         if (!sp_cnt.p || sp_cnt.n < nflags + 1) {
             sp_cnt.ensure(nflags + 1);
             sp_err.ensure(1);
@@ -409,9 +410,9 @@ struct rls_ctx {
         kt_begin(RLS_KF_TRSM);
         sp_tbase += launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, sp_cnt.p, sp_tbase,
                                   sp_cnt.p + 1, ++sp_epoch, sp_err.p);
-        // algorithmic: the factor once per 8-column chunk (values, column indices, row
+        // algorithmic: the factor once per chunk of columns (values, column indices, row
         // pointers) + the block read and written
-        const double chunks = (double)((k + 7) / 8);
+        const double chunks = (double)((k + cw - 1) / cw);
         kt_end(RLS_KF_TRSM, 8.0 * (double)F.nnz * k,
                chunks * (24.0 * (double)F.nnz + 8.0 * (double)(n + 1)) + 32.0 * (double)n * k);
     }

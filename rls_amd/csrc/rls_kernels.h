@@ -125,16 +125,17 @@ void launch_permute_rows(hipStream_t st, int64_t n, const int64_t* p, const zd* 
 // p = nullptr copies (a layout change)
 void launch_permute_rows_strided(hipStream_t st, int64_t n, const int64_t* p, const zd* X, int64_t xrs,
                                  int64_t xcs, zd* Y, int64_t yrs, int64_t ycs, int k, bool scatter);
-// X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), any k (8
-// columns per workgroup, all chunks in one launch); element (j, c) of X at j * rs + c * cs
+// X <- A^{-1} X for a triangular CSR A (lower or upper, optional unit diagonal), any k (a
+// chunk of columns per workgroup, all chunks in one launch); element (j, c) of X at j * rs + c * cs
 // (row-major blocks, rs = k, cs = 1, keep a row's right-hand sides in one cache line).
 // dep[b] (b = block of sptrsv_block_rows() rows): lower -> the first block any row of b
 // references, upper -> the last.  counter / flags start at zero; epoch grows by one per
 // solve; *err is raised if a wait overran its bound.
 int sptrsv_block_rows();
-// One workgroup per (row block, 8-column chunk); flags holds nblk * ceil(k / 8) words (block b,
-// chunk c at b * ceil(k / 8) + c), the launch's tickets are tbase .. tbase + nblk * ceil(k / 8) - 1;
-// returns that ticket count.
+// One workgroup per (row block, chunk of w = sptrsv_chunk_width(k) columns); flags holds
+// nblk * ceil(k / w) words (block b, chunk c at b * ceil(k / w) + c), the launch's tickets are
+// tbase .. tbase + nblk * ceil(k / w) - 1; returns that ticket count.
+int sptrsv_chunk_width(int k);
 unsigned int launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v,
                            bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k,
                            unsigned int* counter, unsigned int tbase, unsigned int* flags, unsigned int epoch,

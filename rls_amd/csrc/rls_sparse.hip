@@ -63,8 +63,8 @@ __global__ void k_permute_rows(int64_t n, const int64_t* __restrict__ p, const z
 }
 
 // Sync-free sparse triangular solve X <- A^{-1} X for a triangular CSR A (lower: rows in
-// ascending order, upper: descending).  One workgroup per block of SP_RB rows and 8-column
-// chunk of the right-hand sides (ticket t = block step * nch + chunk, so a workgroup only
+// ascending order, upper: descending).  One workgroup per block of SP_RB rows and chunk of
+// ccw <= 8 right-hand-side columns (ticket t = block step * nch + chunk, so a workgroup only
 // waits for tickets taken before its own; tickets of a launch start at tbase), with one
 // flag per (block, chunk): the chains of the chunks run side by side, each moving only its
 // own columns.  Everything that does not depend on other blocks is
@@ -85,7 +85,7 @@ template <bool LOWER>
 __global__ void __launch_bounds__(SP_THREADS)
 k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ ci,
          const zd* __restrict__ v, int unit_diag, const int* __restrict__ dep, zd* X, int64_t rs,
-         int64_t cs, int k, int nch, unsigned int* counter, unsigned int tbase, unsigned int* flags,
+         int64_t cs, int k, int ccw, int nch, unsigned int* counter, unsigned int tbase, unsigned int* flags,
          unsigned int epoch, int* err) {
     __shared__ unsigned int s_ticket;
     __shared__ int s_ok;
@@ -165,9 +165,9 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     }
     __syncthreads();
     if (!s_ok) return;
-    // this workgroup's chunk of 8 right-hand-side columns
-    for (int c0 = 8 * ch; c0 < k && c0 < 8 * ch + 8; c0 += 8) {
-        const int kc = min(8, k - c0);
+    // this workgroup's chunk of ccw right-hand-side columns
+    for (int c0 = ccw * ch; c0 < k && c0 < ccw * ch + ccw; c0 += ccw) {
+        const int kc = min(ccw, k - c0);
         zd* Xc = X + cs * c0;
         if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
         // all rows' sums over earlier blocks: staged entries two at a time (their X loads in
@@ -289,20 +289,25 @@ void launch_permute_rows_strided(hipStream_t st, int64_t n, const int64_t* p, co
 
 int sptrsv_block_rows() { return SP_RB; }
 
+// Right-hand-side columns per workgroup: narrow chunks for few columns (more chains side by
+// side, fewer bytes per block step), 8 otherwise.
+int sptrsv_chunk_width(int k) { return k <= 8 ? 2 : 8; }
+
 unsigned int launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v,
                            bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k,
                            unsigned int* counter, unsigned int tbase, unsigned int* flags, unsigned int epoch,
                            int* err) {
     if (k <= 0) return 0;
     const int nblk = (int)cdiv(n, SP_RB);
-    const int nch = (k + 7) / 8;
+    const int cw = sptrsv_chunk_width(k);
+    const int nch = (k + cw - 1) / cw;
     const unsigned int nt = (unsigned int)nblk * (unsigned int)nch;
     if (lower)
         hipLaunchKernelGGL(k_sptrsv<true>, dim3(nt), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep, X,
-                           rs, cs, k, nch, counter, tbase, flags, epoch, err);
+                           rs, cs, k, cw, nch, counter, tbase, flags, epoch, err);
     else
         hipLaunchKernelGGL(k_sptrsv<false>, dim3(nt), dim3(SP_THREADS), 0, st, n, rp, ci, v, unit_diag ? 1 : 0, dep,
-                           X, rs, cs, k, nch, counter, tbase, flags, epoch, err);
+                           X, rs, cs, k, cw, nch, counter, tbase, flags, epoch, err);
     LAUNCH_CHECK();
     return nt;
 }
