@@ -332,6 +332,7 @@ struct rls_ctx {
     void tri_solve(zd* X, int k, bool upper) {
         const int nblk = cdiv(n, 64);
         if (tri_flags.n < (size_t)nblk + 1 || !tri_ctr.p) {
+            check_tri_err();   // a pending wait error is reported, not cleared by the re-allocation
             tri_flags.ensure((size_t)nblk + 1);
             tri_ctr.ensure(1);
             tri_err.ensure(1);
@@ -353,12 +354,22 @@ struct rls_ctx {
         }
     }
     bool tri_ticket_base_ok = false;
+    // Reports (once) a bounded dependency wait that gave up: the flag is cleared after it has
+    // been read, so a later solve on the context starts clean.
     void check_tri_err() {
         int e = 0;
         if (tri_err.p) d2h(&e, tri_err.p, 1);
-        if (e) throw RlsError{RLS_E_HIP, "triangular solve: dependency wait exceeded its bound"};
+        if (e) {
+            HIP_TRY(hipMemsetAsync(tri_err.p, 0, sizeof(int), st));
+            sync();
+            throw RlsError{RLS_E_HIP, "triangular solve: dependency wait exceeded its bound"};
+        }
         if (sp_err.p) d2h(&e, sp_err.p, 1);
-        if (e) throw RlsError{RLS_E_HIP, "sparse triangular solve: dependency wait exceeded its bound"};
+        if (e) {
+            HIP_TRY(hipMemsetAsync(sp_err.p, 0, sizeof(int), st));
+            sync();
+            throw RlsError{RLS_E_HIP, "sparse triangular solve: dependency wait exceeded its bound"};
+        }
     }
     // Y = M X and Y = M^H X (n x k): dense triangular multiply or CSR products
     void apply_M(const zd* X, int k, zd* Y) {
@@ -400,6 +411,7 @@ struct rls_ctx {
         const int cw = sptrsv_chunk_width(k);
         const size_t nflags = (size_t)nb * (size_t)((k + cw - 1) / cw);   // one per (row block, chunk)
         if (!sp_cnt.p || sp_cnt.n < nflags + 1) {
+            check_tri_err();   // a pending wait error is reported, not cleared by the re-allocation
             sp_cnt.ensure(nflags + 1);
             sp_err.ensure(1);
             HIP_TRY(hipMemsetAsync(sp_cnt.p, 0, sizeof(unsigned int) * sp_cnt.n, st));
@@ -1602,84 +1614,6 @@ int rls_test_tall_qr(int64_t n, int k, double* X, double* R) {
 }
 
 }  // extern "C"
-
-extern "C" int rls_test_heev_batch_time(int n, const double* A, int batch, int reps, int r, int nvec,
-                                        int mode_fixed, double* ms_out) {
-    rls_ctx* ctx = nullptr;
-    int rc = rls_ctx_create(0, &ctx);
-    if (rc) return rc;
-    rc = guarded(ctx, [&]() -> int {
-        DArr<zd> dA, dW, dG;
-        DArr<double> dl, dt;
-        DArr<int> drr;
-        dA.ensure((size_t)n * n * batch);
-        dW.ensure(heev_workspace_elems(n, nvec) * batch);
-        dG.ensure((size_t)n * nvec * batch);
-        dl.ensure((size_t)n * batch);
-        dt.ensure(batch);
-        drr.ensure(batch);
-        for (int b = 0; b < batch; ++b) ctx->h2d(dA.p + (size_t)b * n * n, reinterpret_cast<const zd*>(A), (size_t)n * n);
-        std::vector<double> taus(batch, 1e-10);
-        ctx->h2d(dt.p, taus.data(), batch);
-        launch_heev_rr(ctx->st, dA.p, n, (int64_t)n * n, n, batch, dW.p, dl.p, n, dG.p, n, (int64_t)n * nvec, drr.p, dt.p, r, nvec, mode_fixed);
-        ctx->sync();
-        hipEvent_t e0, e1;
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, ctx->st));
-        for (int it = 0; it < reps; ++it)
-            launch_heev_rr(ctx->st, dA.p, n, (int64_t)n * n, n, batch, dW.p, dl.p, n, dG.p, n, (int64_t)n * nvec, drr.p, dt.p, r, nvec, mode_fixed);
-        HIP_TRY(hipEventRecord(e1, ctx->st));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        *ms_out = ms / reps;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        dA.release(); dW.release(); dG.release(); dl.release(); dt.release(); drr.release();
-        return RLS_OK;
-    });
-    rls_ctx_destroy(ctx);
-    return rc;
-}
-
-extern "C" void rls_test_heev_phases(long long* out) { read_heev_phases(out); }
-
-// Average time of one shifted triangular solve (n x n upper T, k right-hand sides).
-extern "C" int rls_test_trsm_time(int64_t n, const double* T, int k, int upper, int reps, double* ms_out) {
-    rls_ctx* ctx = nullptr;
-    int rc = rls_ctx_create(0, &ctx);
-    if (rc) return rc;
-    rc = guarded(ctx, [&]() -> int {
-        if (rls_set_tri_dense(ctx, T, n, n) != RLS_OK) throw RlsError{RLS_E_ARG, ctx->err};
-        DArr<zd> dX;
-        dX.ensure((size_t)n * k);
-        std::vector<cplx> X0((size_t)n * k);
-        for (size_t e = 0; e < X0.size(); ++e) X0[e] = cplx(u01_splitmix(7, 2 * e), u01_splitmix(7, 2 * e + 1));
-        ctx->h2d(dX.p, reinterpret_cast<zd*>(X0.data()), X0.size());
-        ctx->tri_F.ensure(tri_factor_elems(n));
-        launch_tri_factors(ctx->st, n, ctx->T.p, n, zmk(0.3, -0.2), ctx->tri_F.p);
-        ctx->tri_solve(dX.p, k, upper != 0);
-        ctx->sync();
-        hipEvent_t e0, e1;
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, ctx->st));
-        for (int i = 0; i < reps; ++i) ctx->tri_solve(dX.p, k, upper != 0);
-        HIP_TRY(hipEventRecord(e1, ctx->st));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        *ms_out = ms / reps;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        ctx->check_tri_err();
-        dX.release();
-        return RLS_OK;
-    });
-    rls_ctx_destroy(ctx);
-    return rc;
-}
 
 // Operator test hooks: Y = M X or M^H X (which 0 / 1) for the dense or CSR operator, and,
 // with a CSR operator, the factor solves X <- (C - zI)^{-1} X or (C - zI)^{-H} X with the

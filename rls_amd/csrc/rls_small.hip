@@ -159,10 +159,6 @@ __device__ void multisect(const double* d, const double* e2, int n, const int* i
 
 }  // namespace
 
-// Phase timestamps of problem 0 of the last k_heev_rr launch (s_memrealtime ticks, 100 MHz):
-// [0] start, [1] tridiagonal done, [2] eigenvalues done, [3] inverse iteration done, [4] end.
-__device__ long long g_heev_phase[8];
-
 // A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
 //    batch stride sA (elements).  Wk: per-problem workspace (heev_workspace_elems).
 // Outputs: lam (n per problem, ascending; entries not needed by the rule stay NaN),
@@ -205,7 +201,6 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __shared__ int s_rr;
     __shared__ double s_part[HEEV_THREADS];
 
-    if (b == 0 && tid == 0) g_heev_phase[0] = wall_clock64();
     // 1. copy with Hermitian completion from the upper triangle
     for (int64_t idx = tid; idx < (int64_t)n * n; idx += nthr) {
         int i = (int)(idx % n), j = (int)(idx / n);
@@ -432,7 +427,6 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __shared__ double s_d[HEEV_THREADS], s_e[HEEV_THREADS], s_e2[HEEV_THREADS];
     for (int i = tid; i < n; i += nthr) { s_d[i] = d[i]; s_e[i] = e[i]; s_e2[i] = e2[i]; }
     __syncthreads();
-    if (b == 0 && tid == 0) g_heev_phase[1] = wall_clock64();
 
     // 3. Gershgorin interval
     double gl = 1e300, gu = -1e300;
@@ -525,7 +519,6 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
     __syncthreads();
     for (int i = tid; i < n; i += nthr) lo_[i] = lam[i];
     const int nv = s_rr < nvec_max ? s_rr : nvec_max;
-    if (b == 0 && tid == 0) g_heev_phase[2] = wall_clock64();
 
     // 5. inverse iteration for eigenvectors 0..nv-1 (thread j owns vector j); vectors whose
     //    eigenvalues are closer than ortol are re-orthogonalised (one wave per cluster).
@@ -576,9 +569,7 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         __syncthreads();
     }
 
-    if (b == 0 && tid == 0) g_heev_phase[3] = wall_clock64();
     // 6. the back-transformation runs in k_heev_backtr (many workgroups per problem)
-    if (b == 0 && tid == 0) g_heev_phase[4] = wall_clock64();
 }
 
 // Back-transformation of k_heev_rr's eigenvectors, G = H(0) H(1) ... H(n-2) Y (y <- H(k) y for
@@ -644,9 +635,6 @@ k_heev_backtr(int n, const zd* __restrict__ Wk, int64_t sW, zd* __restrict__ G, 
     }
 }
 
-void read_heev_phases(long long* out) {
-    (void)hipMemcpyFromSymbol(out, HIP_SYMBOL(g_heev_phase), sizeof(long long) * 5);
-}
 
 size_t heev_wave_workspace_elems(int n, int nvec_max);
 
@@ -1584,7 +1572,6 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     zd* Hh = Wk + (int64_t)b * sW;
     double* Y = reinterpret_cast<double*>(Hh + (int64_t)n * (n + 1) / 2);
 
-    if (b == 0 && tid0 == 0) g_heev_phase[0] = wall_clock64();
     // load the lower triangle from the upper one (heevr uplo='U'): column i of A holds
     // A[j, i], j <= i, contiguous; L[i, j] = conj(A[j, i])
     for (int i = 0; i < n; ++i) {
@@ -1711,7 +1698,6 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     __syncthreads();
     for (int i = tid0; i < n; i += 256) { e2[i] = e[i] * e[i]; lam[i] = nan(""); }
     __syncthreads();
-    if (b == 0 && tid0 == 0) g_heev_phase[1] = wall_clock64();
 
     // eigenvalues on demand + Ritz-dimension rule (as in k_heev_rr)
     double gl = 1e300, gu = -1e300;
@@ -1785,7 +1771,6 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     double* lo_ = lam_out + (int64_t)b * sL;
     for (int i = tid0; i < n; i += 256) lo_[i] = lam[i];
     const int nv = rr < nvec_max ? rr : nvec_max;
-    if (b == 0 && tid0 == 0) g_heev_phase[2] = wall_clock64();
     const bool wave0 = (tid0 < 64);
 
     // Eigenvectors in chunks of 64 (lane = vector), with the LDS of the packed matrix reused:
@@ -1853,7 +1838,6 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             }
             __syncthreads();
         }
-        if (b == 0 && tid0 == 0) g_heev_phase[3] = wall_clock64();
         // Y chunk -> global (coalesced), then G chunk in LDS (ld n+1)
         for (int idx = tid0; idx < n * nc; idx += 256) Y[idx] = Ys[idx];
         __syncthreads();
@@ -1903,7 +1887,6 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             Gb[i + ldg * (c0 + jj)] = Gs[i + ldl * jj];
         }
     }
-    if (b == 0 && tid0 == 0) g_heev_phase[4] = wall_clock64();
 }
 
 size_t heev_wave_workspace_elems(int n, int nvec_max) {

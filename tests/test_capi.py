@@ -62,10 +62,85 @@ def test_control_defaults_match_reference():
     assert c.maxit == 2000 and c.implict_recycle == 1 and c.implict_update == 0
 
 
-def test_struct_sizes_match_header():
-    # offsets computed by the C compiler for the header layout (x86-64 SysV)
-    assert ctypes.sizeof(_capi.TriControl) == 144
-    assert ctypes.sizeof(_capi.TriStats) == 128
+def _header_fields(struct):
+    """Field names of `typedef struct <struct> {...}` in include/rls_hip.h, in order."""
+    src = open(os.path.join(ROOT, "include", "rls_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\}\s*%s;" % (struct, struct), src, flags=re.S).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        parts = [p.strip() for p in decl.split(",")]
+        names.append(re.findall(r"\*?\s*(\w+)$", parts[0])[0])
+        names += [p.lstrip("*").strip() for p in parts[1:]]
+    return names
+
+
+def _c_layout(struct, fields):
+    """(sizeof, {field: offsetof}) of a header struct, computed by the C compiler."""
+    import shutil
+    import subprocess
+    import tempfile
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = [f'printf("%zu\\n", sizeof({struct}));']
+    lines += [f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    prog = ("#include <stddef.h>\n#include <stdio.h>\n#include \"rls_hip.h\"\nint main(void){"
+            + "".join(lines) + "return 0;}\n")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "l")
+        subprocess.run([cc, "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    return out[0], dict(zip(fields, out[1:]))
+
+
+@pytest.mark.parametrize("struct,mirror", [("rls_tri_control", "TriControl"), ("rls_tri_stats", "TriStats")])
+def test_ctypes_mirror_matches_header_layout(struct, mirror):
+    fields = _header_fields(struct)
+    size, offs = _c_layout(struct, fields)
+    cls = getattr(_capi, mirror)
+    assert [f[0] for f in cls._fields_] == fields
+    assert ctypes.sizeof(cls) == size
+    for f in fields:
+        assert getattr(cls, f).offset == offs[f], f
+
+
+_JULIA_SIZES = {"Float64": 8, "Int64": 8, "UInt64": 8, "Int32": 4, "UInt8": 1, "Cchar": 1}
+
+
+def _julia_struct(name):
+    """(field names, offsets, size) of a Julia `struct` block in INTEGRATION.md, laid out with
+    the C rules Julia applies to isbits structs (each field at its natural alignment)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = re.search(r"(?:mutable\s+)?struct %s\n(.*?)\nend" % name, text, flags=re.S).group(1)
+    body = re.sub(r"#.*", "", body)
+    fields = re.findall(r"(\w+)::([\w{}]+)", body)
+    offs, off, align = [], 0, 1
+    for _, ty in fields:
+        sz = 8 if ty.startswith("Ptr") else _JULIA_SIZES[ty]
+        off = (off + sz - 1) // sz * sz
+        offs.append(off)
+        off += sz
+        align = max(align, sz)
+    size = (off + align - 1) // align * align
+    return [f for f, _ in fields], offs, size
+
+
+@pytest.mark.parametrize("struct,jl", [("rls_tri_control", "CTriControl"), ("rls_tri_stats", "CTriStats")])
+def test_julia_shim_structs_match_header(struct, jl):
+    """The Julia binding in INTEGRATION.md passes every header field at the C offset, so no
+    field (e.g. resume) is read from undefined padding."""
+    fields = _header_fields(struct)
+    size, offs = _c_layout(struct, fields)
+    names, joffs, jsize = _julia_struct(jl)
+    assert names == fields
+    assert jsize == size
+    assert joffs == [offs[f] for f in fields]
 
 
 def test_grid_mirror_matches_oracle():
