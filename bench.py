@@ -67,8 +67,12 @@ def main():
                     help="points of a mid-grid segment timed with both solvers (0 = skip)")
     ap.add_argument("--invlanc-points", type=int, default=40,
                     help="leading traversal points also solved by psInvLanc on the device (0 = skip)")
-    ap.add_argument("--truth-samples", type=int, default=24)
-    ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    ap.add_argument("--truth-samples", type=int, default=200,
+                    help="grid points checked against FP64 ground truth: half uniformly at random, "
+                         "half among the smallest computed sigma")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="CPU-restatement time, split over --cpu-segments segments of the traversal")
+    ap.add_argument("--cpu-segments", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", action="store_true",
                     help="keep the certified singular vectors and time d sigma/dz (and the "
@@ -254,8 +258,7 @@ def main():
     # psTriRecy acceptance test certifies a singular triplet, which in a tight cluster of
     # singular values need not be the smallest one.
     from oracle import truth
-    rng = np.random.default_rng(1)
-    idx = rng.choice(nZ, size=min(args.truth_samples, nZ), replace=False)
+    idx = truth_sample(sig, args.truth_samples)
     t0 = time.perf_counter()
     rel_any = None
     if sparse:
@@ -274,31 +277,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import rls_oracle as O
-        import scipy.linalg as sla  # noqa
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        orec = O.recursion_grid(x_st, x_ed, y_st, y_ed, nx, ny, k)
-        octl = O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6)
-        done = [0]
-        t_start = time.perf_counter()
-
-        class Stop(Exception):
-            pass
-
-        def cb(i, *a):
-            done[0] = i + 1
-            if time.perf_counter() - t_start > args.cpu_seconds:
-                raise Stop()
-        try:
-            O.ps_tri_recy(O.sparse_tri_problem(C) if sparse else O.dense_tri_problem(T), orec, octl,
-                          on_point=cb)
-        except Stop:
-            pass
-        t_cpu = time.perf_counter() - t_start
-        cpu = {"value": done[0] / t_cpu, "unit": "grid pts/s", "cores": threads, "kind": "port",
-               "sample": f"first {done[0]} points of the same zigzag traversal, numpy/OpenBLAS "
-                         f"FP64 restatement, {t_cpu:.1f}s"}
-        log(f"cpu baseline {cpu['value']:.1f} pts/s over {done[0]} points")
+        cpu = cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC,
+                                    (x_st, x_ed, y_st, y_ed, nx, ny, k))
 
     out = {
         "metric": "sigma_min grid pts/sec at rel err<=1e-4: batched mixed vs plain FP64 on GPU",
@@ -327,6 +307,9 @@ def main():
         "psInvLanc": inv,
         "refined_points": int((st.lobpcg_it > 0).sum()),
         "windows": int(st.n_windows),
+        "window_shifts": int(st.n_window_shifts),
+        "window_shifts_per_point": st.n_window_shifts / nZ,
+        "host": host_info(),
         "preprocess_s": t_pre,
         "refine_factor_s": getattr(solver, "factor_time_s", None),
         "cpu_baseline": cpu,
@@ -337,6 +320,120 @@ def main():
         out["adjoint"] = adj
     print(json.dumps(out), flush=True)
     return 0
+
+
+def truth_sample(sig, count, seed=1):
+    """Grid points checked against ground truth: half uniformly at random, half among the
+    smallest computed sigma (where a non-minimal acceptance or an FP error would show)."""
+    nZ = len(sig)
+    count = min(count, nZ)
+    rng = np.random.default_rng(seed)
+    n_small = count // 2
+    order = np.argsort(sig)
+    small = rng.choice(order[: max(n_small, nZ // 20)], size=n_small, replace=False)
+    rest = np.setdiff1d(np.arange(nZ), small)
+    uni = rng.choice(rest, size=count - n_small, replace=False)
+    return np.sort(np.concatenate([small, uni]))
+
+
+def host_info():
+    """CPU model, logical CPUs this process may use, and the BLAS thread setting."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+        blas = [{"api": d.get("internal_api"), "threads": d.get("num_threads")} for d in threadpool_info()]
+    except Exception:
+        pass
+    return {"cpu_model": model, "cpus_available": ncpu, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "blas": blas}
+
+
+def cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC, grid):
+    """The numpy/OpenBLAS FP64 restatement (the oracle) timed on segments spread across the
+    traversal.  A segment starting at point P > 0 continues from the state the GPU solver
+    reaches there (rls_tri_state_get: recycle space, block, acceptance state), so the CPU
+    runs the same warm trajectory; the first segment starts cold, as the grid does.  Each
+    segment runs for cpu_seconds / segments; the sigma it computes is compared with the
+    GPU's on the same points."""
+    import rls_amd as R
+    from oracle import rls_oracle as O
+    nZ = len(rec.Z)
+    nseg = max(1, args.cpu_segments)
+    starts = sorted(set(int(f * nZ) for f in np.arange(nseg) / nseg))
+    per = args.cpu_seconds / len(starts)
+    octl = O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6)
+    problem = O.sparse_tri_problem(C) if sparse else O.dense_tri_problem(T)
+
+    class Stop(Exception):
+        pass
+
+    segs, tot_pts, tot_s, max_rel, mism = [], 0, 0.0, 0.0, 0
+    for P in starts:
+        state = None
+        if P > 0:
+            if sparse:
+                continue
+            solver.solve(rec.Z[:P], ctl, rec)        # the GPU state at point P
+            state = solver.state()
+        orec = O.recursion_grid(*grid)
+        orec.Z = orec.Z[P:]
+        done = [0]
+        t_start = time.perf_counter()
+
+        def cb(i, *a):
+            done[0] = i + 1
+            if time.perf_counter() - t_start > per:
+                raise Stop()
+        out = {}
+
+        def keep(i, z, what, *a):
+            out[i] = (a[-1], what.startswith("refine"))
+            cb(i)
+        x0 = None if state is not None else R.initial_block(T.shape[0] if T is not None else C.shape[0],
+                                                             ctl.r, ctl.seed)
+        try:
+            O.ps_tri_recy(problem, orec, octl, on_point=keep, state=state, X_init=x0)
+        except Stop:
+            pass
+        dt = time.perf_counter() - t_start
+        m = done[0]
+        tot_pts += m
+        tot_s += dt
+        if m:
+            csig = np.array([out[i][0] for i in range(m)])
+            cref = np.array([out[i][1] for i in range(m)])
+            rel = float(np.max(np.abs(csig - sig[P:P + m]) / sig[P:P + m]))
+            mm = int((cref != (st.lobpcg_it[P:P + m] > 0)).sum())
+            max_rel = max(max_rel, rel)
+            mism += mm
+            segs.append({"first_point": P, "points": m, "seconds": dt, "pts_per_s": m / dt,
+                         "refined": int(cref.sum()), "max_rel_diff_vs_gpu": rel, "decisions_differ": mm})
+        log(f"cpu segment at {P}: {m} points in {dt:.1f}s ({m / max(dt, 1e-9):.1f} pts/s)")
+    hi = host_info()
+    threads = None
+    for b in hi.get("blas") or []:
+        threads = max(threads or 0, b.get("threads") or 0)
+    return {"value": tot_pts / tot_s, "unit": "grid pts/s", "cores": threads or hi["cpus_available"],
+            "kind": "port",
+            "sample": f"{tot_pts} points in {len(segs)} segments starting at "
+                      f"{[sg['first_point'] for sg in segs]} of the same zigzag traversal (mid-grid "
+                      f"segments continue from the GPU solver's state there), numpy/OpenBLAS FP64 "
+                      f"restatement, {tot_s:.1f}s",
+            "segments": segs, "max_rel_diff_vs_gpu": max_rel, "decisions_differ": mism,
+            "cpu_model": hi["cpu_model"], "cpus_available": hi["cpus_available"]}
 
 
 def bench_prec(args):

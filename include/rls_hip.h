@@ -124,6 +124,28 @@ int rls_set_sparse_factor(rls_ctx* ctx, const rls_csr* L, const rls_csr* U, cons
 int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_control* control,
                   double* sigma_out, rls_tri_stats* stats);
 
+/* State a resumed solve continues from (the context after its last rls_tri_solve): the
+ * recycle space of RecycledSpace (src/core/recySpace.jl:40-120) with S = I, i.e. V,
+ * J1 = M^H M V, J2 = M^H V, J5 = M V (n x (p r) each, column-major, ld n; J3 = J5,
+ * J4 = J6 = V, H3 = I), L, H1 = J5^H J5, H2 = J5^H V ((p r) x (p r), ld p r), the p refine
+ * shifts of the blocks, the LOBPCG start block X (n x r) and the acceptance state of
+ * _psTriRecy_impl (gap_tau scaling countdown, last refined shift; psTriRecy.jl:735-856).
+ * No counterpart in the reference, whose state lives in one Julia call: a checkpoint of a
+ * traversal, and the start state of a CPU restatement timed on a mid-grid segment.  Every
+ * array pointer may be NULL; complex arrays are interleaved doubles. */
+typedef struct rls_tri_state {
+    int64_t n;
+    int32_t r;
+    int32_t p;                        /* blocks held (<= p_max) */
+    int32_t p_max;
+    int32_t scaled_remaining;         /* points left with gap_tau * gap_tau_scale */
+    int32_t have_failed;              /* 1 once a shift was refined */
+    int32_t pad_;
+    double last_failed[2];            /* the last refined shift (re, im) */
+} rls_tri_state;
+int rls_tri_state_get(rls_ctx* ctx, rls_tri_state* info, double* V, double* J1, double* J2, double* J5,
+                      double* L, double* H1, double* H2, double* pts, double* X);
+
 /* Kernel timing (no counterpart in the reference; the reference's timers are host-side
  * time_ns() phase totals, psTriRecy.jl:264-268).  When enabled, every launch of a kernel
  * family is bracketed with HIP events on the solver's stream and its algorithmic work

@@ -802,16 +802,39 @@ def sparse_tri_problem(C: sp.spmatrix, perm: Optional[np.ndarray] = None):
     return M, S, refine_op
 
 
+def space_from_state(state: dict, p_min: int = 1, implict_update: bool = False) -> RecycledSpace:
+    """RecycledSpace (S = I, orth_S) holding a state exported by the library after a solve:
+    V, J1 = M^H M V, J2 = M^H V, J5 = M V, L, H1, H2 and the blocks' shifts; with S = I,
+    J3 = J5, J4 = J6 = V and H3 = I (recySpace.jl:40-120, 392-417)."""
+    n, r, p, p_max = state["n"], state["r"], state["p"], state["p_max"]
+    rs = RecycledSpace(n, p_max, r, p_min=p_min, implict_update=implict_update)
+    pr = p * r
+    rs.p = p
+    rs.pts[:p] = state["pts"]
+    for name in ("V", "J1", "J2", "J5"):
+        getattr(rs, name)[:, :pr] = state[name]
+    rs.J3[:, :pr] = state["J5"]
+    rs.J6[:, :pr] = state["V"]
+    rs.L[:pr, :pr] = state["L"]
+    rs.H1[:pr, :pr] = state["H1"]
+    rs.H2[:pr, :pr] = state["H2"]
+    rs.H3[:pr, :pr] = np.eye(pr)
+    return rs
+
+
 def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed: int = 0,
                 X_init: Optional[np.ndarray] = None, verbose: int = 0,
-                on_point: Optional[Callable] = None, keep_vectors: bool = False):
+                on_point: Optional[Callable] = None, keep_vectors: bool = False,
+                state: Optional[dict] = None):
     """Sequential psTriRecy over rec.Z (src/core/psTriRecy.jl:708-881).
 
     With keep_vectors, stats.vectors[:, i] is column 1 of the block that passed the
     acceptance test at point i (the recycled Ritz block or the LOBPCG block): the certified
     right singular vector the adjoint is built from.
 
-    ``problem`` = (M, S, refine_op) from dense_tri_problem / sparse_tri_problem.
+    ``problem`` = (M, S, refine_op) from dense_tri_problem / sparse_tri_problem.  ``state``
+    (space_from_state's input) continues a traversal from a saved recycle space, block and
+    acceptance state instead of an empty space.
     Returns (sigma_min, stats, recycle_space)."""
     control = control or PsTriRecyControl()
     M, S, refine_op = problem
@@ -826,12 +849,17 @@ def ps_tri_recy(problem, rec: Recursion, control: PsTriRecyControl = None, seed:
     shift = control.shift
     thresh = resolve_recycle_fail_dist_thresh(control.recycle_fail_dist_thresh, rec)
     scaled_remaining = 0
+    last_failed = None
+    if state is not None:
+        rs = space_from_state(state, control.p_min, control.implict_update)
+        X = np.array(state["X"], dtype=np.complex128)
+        scaled_remaining = int(state["scaled_remaining"])
+        last_failed = complex(state["last_failed"]) if state["have_failed"] else None
     nZ = len(rec.Z)
     sigma_min = np.full(nZ, np.nan)
     stats = PsTriRecyStats.alloc(nZ)
     if keep_vectors:
         stats.vectors = np.zeros((n, nZ), dtype=np.complex128)
-    last_failed = None
     ctrl_main = LobpcgSvdControl(m=n, n=n, r=r, conv_range=(1, 1),
                                  tol=float(control.lobpcg_tol_scale * control.tol),
                                  side=control.lobpcg_side, normC=control.normC)

@@ -35,31 +35,32 @@ def svd_triplet(C: np.ndarray, z: complex):
 
 def tri_sigma_min(T: np.ndarray, Z, block: int = 4, tol: float = 1e-13, maxit: int = 400,
                   seed: int = 7) -> np.ndarray:
-    """sigma_min(T - zI) for upper-triangular T by block inverse subspace iteration."""
-    T = np.asarray(T, dtype=np.complex128)
+    """sigma_min(T - zI) for upper-triangular T by block inverse subspace iteration on
+    (T - zI)^H (T - zI): with Y orthonormal and (T - zI)^H (T - zI) W = Y, W = Q R gives
+    1 / sqrt(sigma_max(R)) -> sigma_min (the dominant eigenvalue of the inverse).  Stops when
+    the estimate changes by less than tol relative; the last estimate is then confirmed by
+    the smallest singular value of (T - zI) Q."""
+    T = np.array(T, dtype=np.complex128, order="F")       # working copy, diagonal shifted in place
     n = T.shape[0]
     rng = np.random.default_rng(seed)
     out = np.empty(len(Z))
     diag = np.diag(T).copy()
-    Y = None
     for i, z in enumerate(Z):
-        Rz = T.copy()
-        Rz[np.diag_indices(n)] = diag - z
-        if Y is None:
-            Y = rng.standard_normal((n, block)) + 1j * rng.standard_normal((n, block))
+        T[np.diag_indices(n)] = diag - z
+        Y = rng.standard_normal((n, block)) + 1j * rng.standard_normal((n, block))
         Y, _ = np.linalg.qr(Y)
         prev = np.inf
         for _ in range(maxit):
-            # Y <- (Rz^H Rz)^{-1} Y
-            Y = sla.solve_triangular(Rz, Y, trans="C", lower=False)
-            Y = sla.solve_triangular(Rz, Y, lower=False)
-            Y, _ = np.linalg.qr(Y)
-            s = sla.svdvals(Rz @ Y)
-            cur = s[-1]
+            W = sla.solve_triangular(T, Y, trans="C", lower=False, check_finite=False)
+            W = sla.solve_triangular(T, W, lower=False, check_finite=False)
+            Y, Rf = np.linalg.qr(W)
+            cur = 1.0 / np.sqrt(sla.svdvals(Rf)[0])
             if abs(cur - prev) <= tol * cur:
                 break
             prev = cur
-        out[i] = cur
+        # Ritz value on the converged subspace (T is upper triangular: zeros below)
+        out[i] = sla.svdvals(T @ Y)[-1]
+    T[np.diag_indices(n)] = diag
     return out
 
 

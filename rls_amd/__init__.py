@@ -276,6 +276,29 @@ class TriSolver:
         self._last_nZ = 0
         self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, n, n))
 
+    def state(self) -> dict:
+        """The state a resumed solve would continue from (rls_tri_state_get): the recycle
+        space (V, J1, J2, J5 n x (p r); L, H1, H2 (p r) x (p r)), the blocks' refine shifts,
+        the start block X and the acceptance state, as host arrays."""
+        info = _capi.TriState()
+        self.ctx.check(self.ctx.lib.rls_tri_state_get(self.ctx.h, C.byref(info), *([None] * 9)))
+        n, r, p = int(info.n), int(info.r), int(info.p)
+        pr = p * r
+        tall = {k: np.zeros((n, pr), dtype=np.complex128, order="F") for k in ("V", "J1", "J2", "J5")}
+        sq = {k: np.zeros((pr, pr), dtype=np.complex128, order="F") for k in ("L", "H1", "H2")}
+        pts = np.zeros(p, dtype=np.complex128)
+        X = np.zeros((n, r), dtype=np.complex128, order="F")
+        ptr = lambda a: a.ctypes.data if a.size else None  # noqa: E731
+        self.ctx.check(self.ctx.lib.rls_tri_state_get(
+            self.ctx.h, C.byref(info), ptr(tall["V"]), ptr(tall["J1"]), ptr(tall["J2"]), ptr(tall["J5"]),
+            ptr(sq["L"]), ptr(sq["H1"]), ptr(sq["H2"]), ptr(pts), X.ctypes.data))
+        out = {"n": n, "r": r, "p": p, "p_max": int(info.p_max), "pts": pts, "X": X,
+               "scaled_remaining": int(info.scaled_remaining), "have_failed": bool(info.have_failed),
+               "last_failed": complex(info.last_failed[0], info.last_failed[1])}
+        out.update(tall)
+        out.update(sq)
+        return out
+
     # ---- adjoint (no counterpart in the reference) ----
     def keep_vectors(self, on: bool = True):
         """Keep every point's certified right singular vector on the device during solve()."""

@@ -21,6 +21,7 @@ EXPORTED = [
     "rls_tri_control_default",
     "rls_set_tri_dense",
     "rls_tri_solve",
+    "rls_tri_state_get",
     "rls_kernel_timing_enable",
     "rls_kernel_timing",
     "rls_tri_keep_vectors",
@@ -101,6 +102,20 @@ class TriStats(C.Structure):
     ]
 
 
+class TriState(C.Structure):
+    """Mirror of ``rls_tri_state`` (the resumable state after a solve; no reference counterpart)."""
+    _fields_ = [
+        ("n", C.c_int64),
+        ("r", C.c_int32),
+        ("p", C.c_int32),
+        ("p_max", C.c_int32),
+        ("scaled_remaining", C.c_int32),
+        ("have_failed", C.c_int32),
+        ("pad_", C.c_int32),
+        ("last_failed", C.c_double * 2),
+    ]
+
+
 class Csr(C.Structure):
     """rls_csr: 0-based CSR with interleaved complex values."""
     _fields_ = [("n", C.c_int64), ("rowptr", C.c_void_p), ("colind", C.c_void_p), ("values", C.c_void_p)]
@@ -153,6 +168,7 @@ def load() -> C.CDLL:
     i64 = C.c_int64
     for name, args in (("rls_tri_keep_vectors", [vp, C.c_int]),
                        ("rls_set_schur_vectors", [vp, vp, i64]),
+                       ("rls_tri_state_get", [vp, C.POINTER(TriState), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
                        ("rls_sigma_grad", [vp, i64, i64, vp]),
                        ("rls_tri_singular_vectors", [vp, i64, i64, vp, vp]),
                        ("rls_sigma_grad_pattern", [vp, i64, i64, i64, vp, vp, vp]),

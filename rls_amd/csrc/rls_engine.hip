@@ -1421,6 +1421,48 @@ int rls_dev_sync(rls_ctx* ctx) {
     });
 }
 
+int rls_tri_state_get(rls_ctx* ctx, rls_tri_state* info, double* V, double* J1, double* J2, double* J5,
+                      double* L, double* H1, double* H2, double* pts, double* X) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->resumable) throw RlsError{RLS_E_STATE, "rls_tri_state_get: no solve state on this context"};
+        if (ctx->sparse) throw RlsError{RLS_E_STATE, "rls_tri_state_get: dense (Schur) operator only"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = ctx->n;
+        const int pr = ctx->p * ctx->r;
+        if (info) {
+            info->n = n;
+            info->r = ctx->r;
+            info->p = ctx->p;
+            info->p_max = ctx->pmax;
+            info->scaled_remaining = ctx->res_scaled;
+            info->have_failed = ctx->res_have_failed ? 1 : 0;
+            info->pad_ = 0;
+            info->last_failed[0] = ctx->res_last_failed.real();
+            info->last_failed[1] = ctx->res_last_failed.imag();
+        }
+        const zd* tall[4] = {ctx->V.p, ctx->J1.p, ctx->J2.p, ctx->J5.p};
+        double* tall_out[4] = {V, J1, J2, J5};
+        for (int a = 0; a < 4; ++a)
+            if (tall_out[a] && pr > 0)
+                ctx->d2h(reinterpret_cast<zd*>(tall_out[a]), tall[a], (size_t)n * pr);
+        const zd* sq[3] = {ctx->L.p, ctx->H1.p, ctx->H2.p};
+        double* sq_out[3] = {L, H1, H2};
+        for (int a = 0; a < 3; ++a) {
+            if (!sq_out[a] || pr == 0) continue;
+            HIP_TRY(hipMemcpy2DAsync(sq_out[a], sizeof(zd) * pr, sq[a], sizeof(zd) * ctx->prm, sizeof(zd) * pr, pr,
+                                     hipMemcpyDeviceToHost, ctx->st));
+            ctx->sync();
+        }
+        if (pts)
+            for (int b = 0; b < ctx->p; ++b) {
+                pts[2 * b] = ctx->pts[b].real();
+                pts[2 * b + 1] = ctx->pts[b].imag();
+            }
+        if (X) std::memcpy(X, ctx->lastX.data(), sizeof(cplx) * ctx->lastX.size());
+        return RLS_OK;
+    });
+}
+
 int rls_tri_keep_vectors(rls_ctx* ctx, int on) {
     return guarded(ctx, [&]() -> int {
         ctx->keep_vec = on != 0;

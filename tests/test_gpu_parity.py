@@ -244,3 +244,30 @@ def test_solves_are_repeatable():
     s2, st2 = solver.solve(rec.Z, ctl, rec)
     assert np.array_equal(s1, s2)
     assert np.array_equal(st1.lobpcg_it, st2.lobpcg_it)
+
+
+def test_state_export_continues_in_the_oracle():
+    """rls_tri_state_get: the oracle continued from the exported state (recycle space, block,
+    acceptance state) follows the GPU's resumed trajectory."""
+    C, T, normC, win, p = _setup("laser300")
+    _say("state export laser300")
+    rec = R.recursion_grid(*win)
+    solver = R.TriSolver(T)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="batched")
+    k1 = len(rec.Z) // 2
+    solver.solve(rec.Z[:k1], ctl, rec)
+    st0 = solver.state()
+    assert st0["p"] > 0 and st0["V"].shape == (T.shape[0], st0["p"] * ctl.r)
+    V = st0["V"]
+    assert np.linalg.norm(V.conj().T @ V - np.eye(V.shape[1])) < 1e-10
+    np.testing.assert_allclose(st0["J5"], T @ V, atol=1e-10 * np.linalg.norm(T))
+    np.testing.assert_allclose(st0["H1"], st0["J5"].conj().T @ st0["J5"], atol=1e-9 * np.linalg.norm(T) ** 2)
+    ctl_r = R.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6, mode="batched", resume=True)
+    s2, st2 = solver.solve(rec.Z[k1:], ctl_r, rec)
+    orec = O.recursion_grid(*win)
+    orec.Z = orec.Z[k1:]
+    octl = O.PsTriRecyControl(tol=1e-4, normC=normC, p=p, recy_tau=1.6)
+    osig, ost, _ = O.ps_tri_recy(O.dense_tri_problem(T), orec, octl, state=st0)
+    assert np.max(np.abs(s2 - osig) / osig) <= 1e-4
+    mism = int(((st2.lobpcg_it > 0) != ost.refined).sum())
+    assert mism <= max(2, len(s2) // 50)
