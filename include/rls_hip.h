@@ -270,6 +270,18 @@ int rls_dev_gemm(rls_ctx* ctx, int nops, const double* const* A, const int64_t* 
 int rls_dev_gram(rls_ctx* ctx, const double* A, int64_t lda, int ka, const double* B, int64_t ldb, int kb,
                  double* C, int64_t ldc);
 
+/* rls_dev_gram without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, result and kernels as rls_dev_gram;
+ * the kernels' partial sums go to a fixed device array of RLS_NOALLOC_GRAM_MAX_ELEMS complex
+ * values (16 MiB) that every GPU the library runs on holds until the process ends, so
+ * min(256, ceil(n / 64)) * ka * kb must not exceed it (RLS_E_ARG otherwise; for n >= 16384,
+ * ka * kb <= 4096).  All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (other streams, other host threads) race and must be serialised by the caller. */
+#define RLS_NOALLOC_GRAM_MAX_ELEMS 1048576
+int rls_dev_gram_noalloc(rls_ctx* ctx, const double* A, int64_t lda, int ka, const double* B, int64_t ldb,
+                         int kb, double* C, int64_t ldc);
+
 #ifdef __cplusplus
 }
 #endif

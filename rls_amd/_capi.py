@@ -44,10 +44,12 @@ EXPORTED = [
     "rls_dev_sync",
     "rls_dev_gemm",
     "rls_dev_gram",
+    "rls_dev_gram_noalloc",
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
+RLS_NOALLOC_GRAM_MAX_ELEMS = 1048576
 
 
 class TriControl(C.Structure):
@@ -187,7 +189,8 @@ def load() -> C.CDLL:
                        ("rls_dev_apply", [vp, C.c_int, C.c_double, C.c_double, vp, i64, C.c_int, vp, i64]),
                        ("rls_dev_sync", [vp]),
                        ("rls_dev_gemm", [vp, C.c_int, vp, vp, vp, vp, i64, C.c_int, C.c_double, vp, i64]),
-                       ("rls_dev_gram", [vp, vp, i64, C.c_int, vp, i64, C.c_int, vp, i64])):
+                       ("rls_dev_gram", [vp, vp, i64, C.c_int, vp, i64, C.c_int, vp, i64]),
+                       ("rls_dev_gram_noalloc", [vp, vp, i64, C.c_int, vp, i64, C.c_int, vp, i64])):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_int

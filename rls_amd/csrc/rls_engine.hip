@@ -26,6 +26,7 @@
 #include <complex>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -1036,6 +1037,34 @@ void upload_tri(rls_ctx* ctx, rls_ctx::SpTri& F, const rls_csr* A, bool lower, c
     ctx->sync();   // the copies read the local vector dep
 }
 
+// Device scratch of the allocation-free Gram entry point (rls_dev_gram_noalloc): the
+// per-row-chunk partial sums of k_gram_partial, at most RLS_NOALLOC_GRAM_MAX_ELEMS complex
+// values (16 MiB), held by every GPU the library runs on until the process ends.  Host code
+// never passes the array's name to a kernel: its per-device address comes from
+// hipGetSymbolAddress, looked up once per device when the first context on it is created.
+__device__ __align__(256) double scratch[2 * (size_t)RLS_NOALLOC_GRAM_MAX_ELEMS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// With `device` current: the address of this device's copy of `scratch` (once per device).
+hipError_t init_noalloc_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
 template <typename F>
 int guarded(rls_ctx* ctx, F&& f) {
     if (!ctx) return RLS_E_ARG;
@@ -1062,6 +1091,7 @@ int rls_ctx_create(int device, rls_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RLS_E_HIP;
     if (device < 0 || device >= ndev) return RLS_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return RLS_E_HIP;
+    if (init_noalloc_scratch(device) != hipSuccess) return RLS_E_HIP;
     rls_ctx* c = new rls_ctx();
     c->device = device;
     if (hipStreamCreate(&c->own_st) != hipSuccess) {
@@ -1408,6 +1438,26 @@ int rls_dev_gram(rls_ctx* ctx, const double* Ad, int64_t lda, int ka, const doub
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->gram_op(reinterpret_cast<const zd*>(Ad), lda, ka, reinterpret_cast<const zd*>(Bd), ldb, kb,
                      reinterpret_cast<zd*>(Cd), ldc);
+        return RLS_OK;
+    });
+}
+
+int rls_dev_gram_noalloc(rls_ctx* ctx, const double* Ad, int64_t lda, int ka, const double* Bd, int64_t ldb,
+                         int kb, double* Cd, int64_t ldc) {
+    return guarded(ctx, [&]() -> int {
+        const int64_t n = ctx->n;
+        if (n <= 0) throw RlsError{RLS_E_STATE, "rls_dev_gram_noalloc: no operator set"};
+        if (ka < 0 || kb < 0 || lda < n || ldb < n || ldc < std::max(ka, 1) || (ka && kb && (!Ad || !Bd || !Cd)))
+            throw RlsError{RLS_E_ARG, "rls_dev_gram_noalloc: bad arguments"};
+        if (ka == 0 || kb == 0) return RLS_OK;
+        if (gram_partial_elems(n, ka, kb) > (size_t)RLS_NOALLOC_GRAM_MAX_ELEMS)
+            throw RlsError{RLS_E_ARG, "rls_dev_gram_noalloc: min(256, ceil(n/64)) * ka * kb exceeds "
+                                      "RLS_NOALLOC_GRAM_MAX_ELEMS (1048576) partial sums"};
+        void* part = noalloc_scratch(ctx->device);
+        if (!part) throw RlsError{RLS_E_STATE, "rls_dev_gram_noalloc: device scratch not initialised"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        launch_gram(ctx->st, n, reinterpret_cast<const zd*>(Ad), lda, ka, reinterpret_cast<const zd*>(Bd), ldb,
+                    kb, reinterpret_cast<zd*>(Cd), ldc, reinterpret_cast<zd*>(part));
         return RLS_OK;
     });
 }
