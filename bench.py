@@ -193,17 +193,21 @@ def main():
     # (1) the leading points of the traversal (the recycle space still growing), and
     # (2) a mid-grid segment at steady state: both solvers resume from the state the batched
     #     solver reaches at the segment start (the trajectories of the two modes are identical)
-    npl = max(1, min(args.plain_points, nZ))
-    ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain", progress=1)
-    t0 = time.perf_counter()
-    sig_p, st_p = solver.solve(rec.Z[:npl], ctl_plain, rec)
-    t_plain = time.perf_counter() - t0
-    plain_pts = npl / t_plain
-    t0 = time.perf_counter()
-    sig_b, _ = solver.solve(rec.Z[:npl], ctl, rec)
-    t_b = time.perf_counter() - t0
-    log(f"plain FP64 {npl} pts {t_plain:.2f}s ({plain_pts:.0f} pts/s); batched same pts {t_b:.2f}s")
-    plain_agree = float(np.nanmax(np.abs(sig_p - sig[:npl]) / sig[:npl]))
+    npl = min(args.plain_points, nZ)
+    plain = None
+    if npl > 0:
+        ctl_plain = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="plain", progress=1)
+        t0 = time.perf_counter()
+        sig_p, st_p = solver.solve(rec.Z[:npl], ctl_plain, rec)
+        t_plain = time.perf_counter() - t0
+        plain_pts = npl / t_plain
+        t0 = time.perf_counter()
+        sig_b, _ = solver.solve(rec.Z[:npl], ctl, rec)
+        t_b = time.perf_counter() - t0
+        log(f"plain FP64 {npl} pts {t_plain:.2f}s ({plain_pts:.0f} pts/s); batched same pts {t_b:.2f}s")
+        plain_agree = float(np.nanmax(np.abs(sig_p - sig[:npl]) / sig[:npl]))
+        plain = {"value": plain_pts, "points": npl, "batched_same_points": npl / t_b,
+                 "speedup": (npl / t_b) / plain_pts, "max_rel_diff_vs_batched": plain_agree}
     mid = None
     nmid = min(args.mid_points, nZ // 2)
     if nmid > 0:
@@ -301,9 +305,7 @@ def main():
         "max_rel_err": float(rel.max()),
         "max_rel_err_nearest_singular_value": None if rel_any is None else float(rel_any.max()),
         "rel_err_samples": int(len(idx)),
-        "plain_fp64": {"value": plain_pts, "points": npl, "batched_same_points": npl / t_b,
-                       "speedup": (npl / t_b) / plain_pts, "max_rel_diff_vs_batched": plain_agree,
-                       "mid_grid": mid},
+        "plain_fp64": None if plain is None else {**plain, "mid_grid": mid},
         "psInvLanc": inv,
         "refined_points": int((st.lobpcg_it > 0).sum()),
         "windows": int(st.n_windows),
@@ -312,6 +314,7 @@ def main():
         "host": host_info(),
         "preprocess_s": t_pre,
         "refine_factor_s": getattr(solver, "factor_time_s", None),
+        "refine_factors": getattr(solver, "factor_count", None),
         "cpu_baseline": cpu,
         "roofline": roof,
         "kernel_families": kt,

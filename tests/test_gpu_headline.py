@@ -185,3 +185,32 @@ def test_sparse_triangular_solve_widths(k):
     got_h = s.apply_factor_solve(X, adjoint=True)
     want_h = spla.spsolve(A.conj().T.tocsc(), X).reshape(n, k)
     np.testing.assert_allclose(got_h, want_h, rtol=0, atol=1e-9 * np.abs(want_h).max())
+
+
+def test_banded_cfd_shaped_sparse_path():
+    """The CFD-Jacobian-shaped sparse workload (banded + 1 % random off-band entries,
+    ||C||_2 = 645.74) at n = 3000 on a window of its grid: sparse psTriRecy (CSR products
+    on the device, a sparse LU per refine shift whose triangular solves run on the device)
+    against the oracle's sparse problem (same LU) and sparse FP64 ground truth."""
+    from oracle import rls_oracle as O
+    from oracle import truth
+    n = 3000
+    C = WL.banded_cfd(n=n, half_band=60, norm2=645.74)
+    g = WL.GRIDS["cfd"]
+    dx, dy = (g[1] - g[0]) / 199, (g[3] - g[2]) / 199
+    win = (-0.4, -0.4 + 20 * dx, -0.2, -0.2 + 28 * dy, 6, 8, 4)
+    _say(f"banded cfd-shaped sparse n={n} nnz={C.nnz} grid 6x8")
+    rec = R.recursion_grid(*win)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=645.74, p=20, recy_tau=1.6, mode="batched",
+                             sparse_path="factor")
+    sig, st = R.psTriRecy(C, rec, control=ctl, verbose=0)
+    osig, ost, _ = O.ps_tri_recy(O.sparse_tri_problem(C), O.recursion_grid(*win),
+                                 O.PsTriRecyControl(tol=1e-4, normC=645.74, p=20, recy_tau=1.6),
+                                 X_init=R.initial_block(n, 6, 0))
+    tr = truth.sparse_sigma_min(C, rec.Z)
+    rel, orel = np.abs(sig - tr) / tr, np.abs(sig - osig) / osig
+    _say(f"max rel err vs truth {rel.max():.2e}, vs oracle {orel.max():.2e}, refined {st.n_pts}")
+    assert rel.max() <= 1e-4
+    assert orel.max() <= 1e-6
+    assert int(((st.lobpcg_it > 0) != ost.refined).sum()) <= 1
+    assert 0 < st.n_pts < len(sig)
