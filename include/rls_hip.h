@@ -270,6 +270,107 @@ int rls_dev_gemm(rls_ctx* ctx, int nops, const double* const* A, const int64_t* 
 int rls_dev_gram(rls_ctx* ctx, const double* A, int64_t lda, int ka, const double* B, int64_t ldb, int kb,
                  double* C, int64_t ldc);
 
+/* ---- psPrecRecy behind the C ABI (src/core/psPrecRecy.jl:617-1042) ----
+ *
+ * PsPrecRecyControl (psPrecRecy.jl:220-261), same names, meaning and defaults; Greek names
+ * spelled out (lambda_lower_min, lambda_upper_max, gamma_tol, lambda_n_tol).  Julia symbols
+ * map to integers (pred_orth_type: RLS_ORTH_LAPACK for :lapack, RLS_ORTH_SHIFTCHOL for
+ * :shiftchol); pred_conv_range = nothing is pred_conv_first = pred_conv_last = 0 (the
+ * reference overwrites it with 1:r_pred or 1:idx_pred before every prediction solve,
+ * psPrecRecy.jl:741-778, so it does not change a trajectory).  rec_dx / rec_dy are the grid
+ * steps of the default recycle_fail_dist_thresh (recySpace.jl:22-38). */
+#define RLS_ORTH_LAPACK 0
+#define RLS_ORTH_SHIFTCHOL 1
+typedef struct rls_prec_control {
+    double tol;                       /* 1e-3 */
+    double lobpcg_tol_scale;          /* 0.1 */
+    double main_normC;                /* 10.0 */
+    double main_tau;                  /* 1.666 */
+    double shift;                     /* main_normC^2 * eps * 100 */
+    double gap_tau;                   /* 1e-10 */
+    double recycle_fail_dist_thresh;  /* NaN -> 1.01 * max(rec_dx, rec_dy) */
+    double tol_pred;                  /* 1e-3 */
+    double lobpcg_pred_tol_scale;     /* 1.0 */
+    int32_t pred_conv_first;          /* 0 (nothing) */
+    int32_t pred_conv_last;           /* 0 */
+    double pred_tau;                  /* 1.666 */
+    double idx_cutoff_pred;           /* 1e-2 */
+    int32_t pred_orth_type;           /* RLS_ORTH_LAPACK */
+    int32_t pred_conv_thresh_type;    /* 0 */
+    double lambda_lower_min;          /* 0.1 */
+    double lambda_upper_max;          /* 30.0 */
+    int32_t r_main;                   /* 6 */
+    int32_t p_main;                   /* 60 */
+    int32_t p_min_main;               /* 1 */
+    int32_t main_implict_update;      /* 0 */
+    int32_t main_implict_recycle;     /* 0 */
+    int32_t r_pred;                   /* 20 */
+    int32_t p_pred;                   /* 20 */
+    int32_t p_min_pred;               /* 1 */
+    int32_t pred_implict_update;      /* 1 */
+    int32_t pred_implict_recycle;     /* 1 */
+    double hsl_tol;                   /* 1e-3 */
+    double hsl_cntl_1;                /* 1e-6 */
+    double hsl_cntl_2;                /* 1e-5 */
+    int32_t hsl_icntl_1;              /* 200 */
+    int32_t hsl_icntl_2;              /* 200 */
+    double gamma_tol;                 /* 1e-3 */
+    double lambda_n_tol;              /* 1e-3 */
+    int32_t collect_timing;           /* 1 */
+    int32_t maxit;                    /* LOBPCG maxit, 2000 (src/core/lobpcgSvd.jl:24) */
+    double rec_dx, rec_dy;
+    /* --- execution knobs (no counterpart in the reference) --- */
+    uint64_t seed;                    /* start blocks: the counter stream of rls_tri_solve's block
+                                         with seed + tag; tags 0 (main X), 1 (prediction X) and
+                                         1000 + 3k, 1001 + 3k, 1002 + 3k (lambda_n start, gamma
+                                         start, prediction reset of the k-th IC build) */
+    int32_t verbose;                  /* >= 1: a line per refined point on stderr */
+    int32_t pad_;
+} rls_prec_control;
+
+/* PsPrecRecyStats (psPrecRecy.jl:318-354).  Array pointers may be NULL; non-NULL arrays hold
+ * nZ entries (complex arrays 2 nZ doubles).  main_it / pred_it are extras: LOBPCG
+ * iterations of each point's main and prediction solves. */
+typedef struct rls_prec_stats {
+    int64_t* p_count;
+    int64_t* pred_p_count;
+    int64_t* r_recy;
+    double* pts;
+    double* pred_pts;
+    double* precond_rebuild_pts;
+    int64_t* main_M_vec_apps;
+    int64_t* main_M_vec_apps_no_recycle;
+    int64_t* main_L_vec_apps;
+    int64_t* pred_M_vec_apps;
+    int64_t* pred_L_vec_apps;
+    int64_t* main_it;
+    int64_t* pred_it;
+    uint64_t main_recycle_time_ns;
+    uint64_t main_refine_time_ns;
+    uint64_t main_update_time_ns;
+    uint64_t pred_recycle_time_ns;
+    uint64_t pred_refine_time_ns;
+    uint64_t pred_update_time_ns;
+    uint64_t precond_time_ns;
+    int64_t n_it;
+    int64_t n_pred_it;
+    int64_t n_pts;
+    int64_t n_pred_pts;
+    int64_t n_precond_rebuild_pts;
+} rls_prec_stats;
+
+/* Fill `c` with PsPrecRecyControl's defaults for a given main_normC (psPrecRecy.jl:220-261). */
+void rls_prec_control_default(rls_prec_control* c, double main_normC);
+
+/* Run psPrecRecy over the nZ shifts Z (complex, traversal order) on the context's sparse C
+ * (rls_set_csr) and write sigma_min to sigma_out (NaN where unset).  perm: the column
+ * ordering p of the preconditioner (0-based, p[k] = original index at position k; the
+ * reference's qr(C).cpiv, psPrecRecy.jl:625), NULL = natural order.  The threshold IC of
+ * (C - zI)^H (C - zI) is built on the host (rls_ic_build) at the first point and at every
+ * rebuild; everything else runs on the device.  `stats` may be NULL. */
+int rls_prec_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const int64_t* perm, const rls_prec_control* control,
+                   double* sigma_out, rls_prec_stats* stats);
+
 /* rls_dev_gram without any device allocation.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, result and kernels as rls_dev_gram;

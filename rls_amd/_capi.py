@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librls_hip.so")
 
 RLS_OK = 0
+RLS_E_ARG, RLS_E_HIP, RLS_E_STATE, RLS_E_NUMERIC = -1, -2, -3, -4
 RLS_MODE_PLAIN_FP64 = 0
 RLS_MODE_BATCHED = 1
 
@@ -45,11 +46,14 @@ EXPORTED = [
     "rls_dev_gemm",
     "rls_dev_gram",
     "rls_dev_gram_noalloc",
+    "rls_prec_control_default",
+    "rls_prec_solve",
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
 RLS_NOALLOC_GRAM_MAX_ELEMS = 1048576
+RLS_ORTH_LAPACK, RLS_ORTH_SHIFTCHOL = 0, 1
 
 
 class TriControl(C.Structure):
@@ -116,6 +120,37 @@ class TriState(C.Structure):
         ("pad_", C.c_int32),
         ("last_failed", C.c_double * 2),
     ]
+
+
+class PrecControl(C.Structure):
+    """Mirror of ``rls_prec_control`` (PsPrecRecyControl, src/core/psPrecRecy.jl:220-261)."""
+    _fields_ = [
+        ("tol", C.c_double), ("lobpcg_tol_scale", C.c_double), ("main_normC", C.c_double),
+        ("main_tau", C.c_double), ("shift", C.c_double), ("gap_tau", C.c_double),
+        ("recycle_fail_dist_thresh", C.c_double), ("tol_pred", C.c_double),
+        ("lobpcg_pred_tol_scale", C.c_double), ("pred_conv_first", C.c_int32), ("pred_conv_last", C.c_int32),
+        ("pred_tau", C.c_double), ("idx_cutoff_pred", C.c_double), ("pred_orth_type", C.c_int32),
+        ("pred_conv_thresh_type", C.c_int32), ("lambda_lower_min", C.c_double), ("lambda_upper_max", C.c_double),
+        ("r_main", C.c_int32), ("p_main", C.c_int32), ("p_min_main", C.c_int32), ("main_implict_update", C.c_int32),
+        ("main_implict_recycle", C.c_int32), ("r_pred", C.c_int32), ("p_pred", C.c_int32), ("p_min_pred", C.c_int32),
+        ("pred_implict_update", C.c_int32), ("pred_implict_recycle", C.c_int32), ("hsl_tol", C.c_double),
+        ("hsl_cntl_1", C.c_double), ("hsl_cntl_2", C.c_double), ("hsl_icntl_1", C.c_int32),
+        ("hsl_icntl_2", C.c_int32), ("gamma_tol", C.c_double), ("lambda_n_tol", C.c_double),
+        ("collect_timing", C.c_int32), ("maxit", C.c_int32), ("rec_dx", C.c_double), ("rec_dy", C.c_double),
+        ("seed", C.c_uint64), ("verbose", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class PrecStats(C.Structure):
+    """Mirror of ``rls_prec_stats`` (PsPrecRecyStats, src/core/psPrecRecy.jl:318-354)."""
+    _fields_ = [(k, C.POINTER(C.c_int64)) for k in ("p_count", "pred_p_count", "r_recy")] + \
+        [(k, C.POINTER(C.c_double)) for k in ("pts", "pred_pts", "precond_rebuild_pts")] + \
+        [(k, C.POINTER(C.c_int64)) for k in ("main_M_vec_apps", "main_M_vec_apps_no_recycle", "main_L_vec_apps",
+                                            "pred_M_vec_apps", "pred_L_vec_apps", "main_it", "pred_it")] + \
+        [(k, C.c_uint64) for k in ("main_recycle_time_ns", "main_refine_time_ns", "main_update_time_ns",
+                                   "pred_recycle_time_ns", "pred_refine_time_ns", "pred_update_time_ns",
+                                   "precond_time_ns")] + \
+        [(k, C.c_int64) for k in ("n_it", "n_pred_it", "n_pts", "n_pred_pts", "n_precond_rebuild_pts")]
 
 
 class Csr(C.Structure):
@@ -194,6 +229,10 @@ def load() -> C.CDLL:
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = C.c_int
+    lib.rls_prec_control_default.argtypes = [C.POINTER(PrecControl), C.c_double]
+    lib.rls_prec_control_default.restype = None
+    lib.rls_prec_solve.argtypes = [vp, vp, i64, vp, C.POINTER(PrecControl), vp, C.POINTER(PrecStats)]
+    lib.rls_prec_solve.restype = C.c_int
     lib.rls_ic_last_error.argtypes = []
     lib.rls_ic_last_error.restype = C.c_char_p
     lib.rls_ic_free.argtypes = [vp]

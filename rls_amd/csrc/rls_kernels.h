@@ -63,6 +63,10 @@ void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n,
 void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, int r, bool side_L,
                         double shift_scale, zd* G, int64_t sG, double* sig, double* lam,
                         int64_t sSig, int* info, int batch);
+// svdRaylRitz!'s small part from a given upper-triangular factor R (nS x nS, ld nS) of the
+// image block (a Cholesky-QR of it): same outputs as launch_svdrr_small for one problem.
+void launch_svdrr_from_r(hipStream_t st, const zd* R, int nS, int r, bool side_L, zd* G, double* sig,
+                         double* lam, int* info);
 void launch_chol_rinv(hipStream_t st, const zd* Gram, int64_t sGram, int k, double shift_scale,
                       zd* R, zd* Rinv, int64_t sR, double* offI, int* info, int batch,
                       double skip_tol = -1.0);

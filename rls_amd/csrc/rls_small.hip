@@ -891,7 +891,7 @@ __device__ void lds_jacobi_svd(zd* A, zd* V, double* sig, int* perm, int k, int*
 __global__ void __launch_bounds__(SMALL_THREADS)
 k_svdrr_small(const zd* __restrict__ Gram, int64_t sGram, int nS, int r, int side_L,
               double shift_scale, zd* __restrict__ Gout, int64_t sG, double* __restrict__ sig_out,
-              double* __restrict__ lam_out, int64_t sSig, int* __restrict__ info) {
+              double* __restrict__ lam_out, int64_t sSig, int* __restrict__ info, int input_R) {
     extern __shared__ zd lds[];
     const int b = blockIdx.x, tid = threadIdx.x;
     const int k = nS;
@@ -908,7 +908,15 @@ k_svdrr_small(const zd* __restrict__ Gram, int64_t sGram, int nS, int r, int sid
     const double eps = 2.220446049250313e-16;
     int nshift = 0;
     double shift = 0.0;
-    for (int attempt = 0; attempt < 60; ++attempt) {
+    if (input_R) {
+        // the triangular factor is given (a Cholesky-QR of the image block): its upper triangle
+        for (int idx = tid; idx < k * k; idx += blockDim.x) {
+            int a = idx % k, c = idx / k;
+            A[idx] = (a <= c) ? Gb[a + (int64_t)k * c] : zzero();
+        }
+        __syncthreads();
+    }
+    for (int attempt = 0; attempt < 60 && !input_R; ++attempt) {
         for (int idx = tid; idx < k * k; idx += blockDim.x) {
             int a = idx % k, c = idx / k;
             zd v = (a <= c) ? Gb[a + (int64_t)k * c] : zconj(Gb[c + (int64_t)k * a]);
@@ -1000,7 +1008,16 @@ void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, i
     if (nS > 64) throw RlsError{-1, "svdrr: nS > 64 unsupported"};
     size_t lds = (size_t)3 * nS * nS * sizeof(zd);
     hipLaunchKernelGGL(k_svdrr_small, dim3(batch), dim3(SMALL_THREADS), lds, st, Gram, sGram, nS,
-                       r, side_L ? 1 : 0, shift_scale, G, sG, sig, lam, sSig, info);
+                       r, side_L ? 1 : 0, shift_scale, G, sG, sig, lam, sSig, info, 0);
+    LAUNCH_CHECK();
+}
+
+void launch_svdrr_from_r(hipStream_t st, const zd* R, int nS, int r, bool side_L, zd* G, double* sig,
+                         double* lam, int* info) {
+    if (nS > 64) throw RlsError{-1, "svdrr: nS > 64 unsupported"};
+    size_t lds = (size_t)3 * nS * nS * sizeof(zd);
+    hipLaunchKernelGGL(k_svdrr_small, dim3(1), dim3(SMALL_THREADS), lds, st, R, (int64_t)nS * nS, nS, r,
+                       side_L ? 1 : 0, 0.0, G, (int64_t)nS * nS, sig, lam, (int64_t)nS, info, 1);
     LAUNCH_CHECK();
 }
 

@@ -69,7 +69,7 @@ def _header_fields(struct):
     body = re.search(r"typedef struct %s \{(.*?)\}\s*%s;" % (struct, struct), src, flags=re.S).group(1)
     names = []
     for decl in body.split(";"):
-        decl = decl.strip()
+        decl = re.sub(r"\[\d+\]", "", decl).strip()
         if not decl:
             continue
         parts = [p.strip() for p in decl.split(",")]
@@ -99,7 +99,9 @@ def _c_layout(struct, fields):
     return out[0], dict(zip(fields, out[1:]))
 
 
-@pytest.mark.parametrize("struct,mirror", [("rls_tri_control", "TriControl"), ("rls_tri_stats", "TriStats")])
+@pytest.mark.parametrize("struct,mirror", [("rls_tri_control", "TriControl"), ("rls_tri_stats", "TriStats"),
+                                           ("rls_tri_state", "TriState"), ("rls_prec_control", "PrecControl"),
+                                           ("rls_prec_stats", "PrecStats")])
 def test_ctypes_mirror_matches_header_layout(struct, mirror):
     fields = _header_fields(struct)
     size, offs = _c_layout(struct, fields)
@@ -162,3 +164,26 @@ def test_context_without_gpu_fails_loudly():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                        env={**os.environ, "HIP_VISIBLE_DEVICES": "-1"})
     assert r.returncode != 0 and "rls_ctx_create" in r.stderr
+
+
+def test_prec_control_defaults_match_reference():
+    """rls_prec_control_default mirrors PsPrecRecyControl's defaults (psPrecRecy.jl:220-261)
+    and the Python mirror's."""
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librls_hip.so not built")
+    from rls_amd import prec as RP
+    lib = _capi.load()
+    c = _capi.PrecControl()
+    lib.rls_prec_control_default(ctypes.byref(c), 7.5)
+    py = RP.PsPrecRecyControl(main_normC=7.5)
+    for f in ("tol", "lobpcg_tol_scale", "main_normC", "main_tau", "gap_tau", "tol_pred", "lobpcg_pred_tol_scale",
+              "pred_tau", "idx_cutoff_pred", "lambda_lower_min", "lambda_upper_max", "hsl_tol", "hsl_cntl_1",
+              "hsl_cntl_2", "gamma_tol", "lambda_n_tol", "r_main", "p_main", "p_min_main", "r_pred", "p_pred",
+              "p_min_pred", "hsl_icntl_1", "hsl_icntl_2", "pred_conv_thresh_type", "maxit"):
+        assert getattr(c, f) == getattr(py, f), f
+    assert c.shift == pytest.approx(py.shift)
+    assert (c.tol, c.main_tau, c.r_pred, c.p_pred, c.hsl_cntl_2, c.lambda_upper_max) == (1e-3, 1.666, 20, 20, 1e-5, 30.0)
+    assert c.main_implict_recycle == 0 and c.pred_implict_recycle == 1 and c.pred_implict_update == 1
+    assert c.pred_orth_type == _capi.RLS_ORTH_LAPACK and c.pred_conv_first == 0
+    import math
+    assert math.isnan(c.recycle_fail_dist_thresh)

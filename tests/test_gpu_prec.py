@@ -73,8 +73,9 @@ def test_ps_prec_recy_matches_oracle_and_svd():
     octl, gctl = _controls(normC)
     perm = RP.column_ordering(A)
     rb = lambda nn, rr, tag: R.initial_block(nn, rr, 77 + tag)
+    seed = 77
     print(f"[prec solve] skewlap n={n} grid 6x6", flush=True)
-    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, rand_block=rb)
+    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, block_seed=seed)
     osig, ost = OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb)
     Ad = A.toarray()
     tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
@@ -96,7 +97,8 @@ def test_ps_prec_recy_explicit_main_recycle():
                            pred_conv_thresh_type=0)
     perm = RP.column_ordering(A)
     rb = lambda nn, rr, tag: R.initial_block(nn, rr, 5 + tag)
-    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, rand_block=rb)
+    seed = 5
+    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, block_seed=seed)
     osig, ost = OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb)
     Ad = A.toarray()
     tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
@@ -144,10 +146,70 @@ def test_ps_prec_recy_wide_prediction_block():
     octl, gctl = _controls(normC, r_pred=12, p_pred=4)
     perm = RP.column_ordering(A)
     rb = lambda nn, rr, tag: R.initial_block(nn, rr, 31 + tag)
-    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, rand_block=rb)
+    seed = 31
+    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, block_seed=seed)
     osig, ost = OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb)
     Ad = A.toarray()
     tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
     assert np.max(np.abs(sig - tr) / tr) < 1e-4
     assert st.n_pts == ost.n_pts and st.n_pred_pts == ost.n_pred_pts
     assert np.max(np.abs(sig - osig) / osig) < 1e-6
+
+
+def test_ps_prec_recy_config_widths():
+    """The skew-Laplacian workload's block widths (r_main = 6, r_pred = 40): the prediction
+    LOBPCG's 80- and 120-column Rayleigh-Ritz problems take the Gram eigensolver route and
+    the prediction space's block rotations the general one (r > 16, S != I)."""
+    A = WL.skewlap3d(8)                          # n = 343
+    n = A.shape[0]
+    s = 64.0
+    rec = O.recursion_grid(-0.88 * s, 0.16 * s, -0.32 * s, 0.32 * s, 5, 6, 3)
+    normC = WL.norm2_estimate(A)
+    octl, gctl = _controls(normC, r_main=6, p_main=20, r_pred=40, p_pred=4, lambda_lower_min=0.01,
+                           hsl_icntl_1=40, hsl_icntl_2=80)
+    perm = RP.column_ordering(A)
+    rb = lambda nn, rr, tag: R.initial_block(nn, rr, 9 + tag)
+    print(f"[prec solve] skewlap n={n} r_pred=40 grid 5x6", flush=True)
+    sig, st = RP.psPrecRecy(A, rec, gctl, perm=perm, block_seed=9)
+    osig, ost = OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb)
+    Ad = A.toarray()
+    tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in rec.Z])
+    print(f"[prec solve] max rel vs SVD {np.max(np.abs(sig - tr) / tr):.2e}, vs oracle "
+          f"{np.max(np.abs(sig - osig) / osig):.2e}, main refines {st.n_pts}/{ost.n_pts}, "
+          f"pred refines {st.n_pred_pts}/{ost.n_pred_pts}", flush=True)
+    assert np.max(np.abs(sig - tr) / tr) < 1e-4
+    assert st.n_pred_pts >= 1 and st.n_pts >= 1
+    assert abs(st.n_pts - ost.n_pts) <= 1 and abs(st.n_pred_pts - ost.n_pred_pts) <= 1
+    assert np.max(np.abs(sig - osig) / osig) < 1e-5
+
+
+def test_prec_solve_errors_through_the_c_abi():
+    """rls_prec_solve's argument checks (RLS_E_ARG with a message) and the natural ordering
+    when no permutation is given."""
+    import ctypes as C
+    A, rec = _small_case(5)
+    ps = RP.PrecSolver(A)
+    lib = ps.ctx.lib
+    c = _capi.PrecControl()
+    lib.rls_prec_control_default(C.byref(c), 100.0)
+    Z = np.ascontiguousarray(rec.Z[:3], dtype=np.complex128)
+    sig = np.zeros(3)
+    c.r_pred = 50
+    rc = lib.rls_prec_solve(ps.ctx.h, Z.ctypes.data, 3, None, C.byref(c), sig.ctypes.data, None)
+    assert rc == _capi.RLS_E_ARG
+    assert "r_pred" in lib.rls_last_error(ps.ctx.h).decode()
+    c.r_pred = 4
+    c.p_pred = 4
+    bad = np.zeros(A.shape[0], dtype=np.int64)
+    rc = lib.rls_prec_solve(ps.ctx.h, Z.ctypes.data, 3, bad.ctypes.data, C.byref(c), sig.ctypes.data, None)
+    assert rc == _capi.RLS_E_ARG and "permutation" in lib.rls_last_error(ps.ctx.h).decode()
+    c.tol = 1e-4
+    c.hsl_icntl_1, c.hsl_icntl_2, c.hsl_cntl_2 = 20, 50, 1e-4
+    c.lambda_lower_min = 0.01
+    c.rec_dx = c.rec_dy = 1.0
+    rc = lib.rls_prec_solve(ps.ctx.h, Z.ctypes.data, 3, None, C.byref(c), sig.ctypes.data, None)
+    assert rc == 0, lib.rls_last_error(ps.ctx.h).decode()
+    n = A.shape[0]
+    Ad = A.toarray()
+    tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in Z])
+    assert np.max(np.abs(sig - tr) / tr) < 1e-3
