@@ -439,6 +439,57 @@ def cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC, 
             "cpu_model": hi["cpu_model"], "cpus_available": hi["cpus_available"]}
 
 
+def cpu_baseline_prec(args, A, rec, ctl, perm, sig, st):
+    """The oracle's psPrecRecy (numpy/scipy FP64 restatement) on the leading points of the same
+    traversal, from the same start blocks, with the threshold IC built by the library's host
+    C++ routine (the restated one takes minutes per build at this size), for cpu_seconds;
+    its sigma is compared with the GPU's on the same points."""
+    import scipy.sparse as sp
+    import rls_amd as R
+    from oracle import rls_ic as OI
+    from oracle import rls_prec as OP
+    from rls_amd import prec as RP
+
+    def ic_builder(C, z, p, c):
+        f = RP.ic_build(RP.normal_lower(C, z), p, c.hsl_tol, c.hsl_cntl_1, c.hsl_cntl_2, c.hsl_icntl_1,
+                        c.hsl_icntl_2)
+        L = sp.csc_matrix(f.to_scipy())
+        invp = np.empty(len(p), np.int64)
+        invp[p] = np.arange(len(p))
+        return OI.SparseLLTP(L, np.asarray(p), invp, f.alpha, f.n_shifts)
+
+    octl = OP.PsPrecRecyControl(**{k: getattr(ctl, k) for k in OP.PsPrecRecyControl.__dataclass_fields__
+                                   if hasattr(ctl, k)})
+    seed_base = ctl.seed * 1000003
+    rb = lambda nn, rr, tag: R.initial_block(nn, rr, seed_base + tag)  # noqa: E731
+    out = {}
+    t_start = time.perf_counter()
+
+    class Stop(Exception):
+        pass
+
+    def cb(i, z, what, s_):
+        out[i] = (s_, "main(" in what)
+        if time.perf_counter() - t_start > args.cpu_seconds:
+            raise Stop()
+    try:
+        OP.ps_prec_recy(A, rec, octl, perm=perm, rand_block=rb, on_point=cb, ic_builder=ic_builder)
+    except Stop:
+        pass
+    dt = time.perf_counter() - t_start
+    m = len(out)
+    csig = np.array([out[i][0] for i in range(m)])
+    rel = float(np.max(np.abs(csig - sig[:m]) / sig[:m])) if m else None
+    hi = host_info()
+    threads = max([b.get("threads") or 0 for b in (hi.get("blas") or [])] or [hi["cpus_available"]])
+    log(f"cpu baseline (oracle psPrecRecy) {m} points in {dt:.1f}s; max rel diff vs GPU {rel}")
+    return {"value": m / dt, "unit": "grid pts/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} points of the same zigzag traversal from the same start blocks, numpy/scipy "
+                      f"FP64 restatement of psPrecRecy with the threshold IC from the library's host C++ "
+                      f"build, {dt:.1f}s",
+            "max_rel_diff_vs_gpu": rel, "cpu_model": hi["cpu_model"], "cpus_available": hi["cpus_available"]}
+
+
 def bench_prec(args):
     """psPrecRecy on the C2 configuration (example/skewlap3d/skewlap3d_psPrecRecy.jl:16-45)
     over the leading points of the 200 x 200 traversal."""
@@ -451,7 +502,7 @@ def bench_prec(args):
     normC = WL.norm2_estimate(A)
     g = WL.GRIDS["skewlap3d"]
     rec = R.recursion_grid(*g)
-    npts = args.max_points if args.max_points > 0 else 200
+    npts = args.max_points if args.max_points > 0 else 2000
     rec.Z = rec.Z[:npts]
     ctl = RP.PsPrecRecyControl(
         tol=1e-4, main_normC=normC, main_tau=1.6, gap_tau=1e-10, tol_pred=1e-3, pred_tau=1.6,
@@ -492,6 +543,9 @@ def bench_prec(args):
     tr = truth.sparse_sigma_min(A, rec.Z[idx])
     rel = np.abs(sig[idx] - tr) / tr
     log(f"truth on {len(idx)} points: max rel {rel.max():.2e}; trsm {d['ms']:.0f} ms / {d['launches']} launches")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_prec(args, A, rec, ctl, perm, sig, st)
     out = {
         "metric": "sigma_min grid pts/sec at rel err<=1e-4: batched mixed vs plain FP64 on GPU",
         "value": value, "unit": "grid pts/s", "n_gpus": 1, "steps": len(times), "warmup": args.warmup,
@@ -508,7 +562,8 @@ def bench_prec(args):
                           "main_refine": st.main_refine_time_ns / 1e9, "main_update": st.main_update_time_ns / 1e9,
                           "pred_recycle": st.pred_recycle_time_ns / 1e9, "pred_refine": st.pred_refine_time_ns / 1e9,
                           "pred_update": st.pred_update_time_ns / 1e9},
-        "preprocess_s": t_pre, "cpu_baseline": None, "roofline": roof, "kernel_families": kt,
+        "preprocess_s": t_pre, "cpu_baseline": cpu, "roofline": roof, "kernel_families": kt,
+        "host": host_info(),
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -181,13 +181,18 @@ def pred_operators(C, CH, z, K: SparseLLTP):
     return M_pred, S_pred, Cz_pred
 
 
-def build_precond(C, CH, z, p, ctl: PsPrecRecyControl, rand_block, k: int):
+def build_precond(C, CH, z, p, ctl: PsPrecRecyControl, rand_block, k: int, ic_builder=None):
     """_get_pred_precond! (psPrecRecy.jl:436-462): K from get_hsl_LLTP, lambda_n =
     sigma_max((C - zI) invF)^2 (_compute_lambda_n 405-422) and gamma = sigma_max(L^{-1})
-    (_compute_gamma 388-402), both by one-column LOBPCG with side 'L'."""
+    (_compute_gamma 388-402), both by one-column LOBPCG with side 'L'.  ic_builder(C, z, p,
+    ctl) -> SparseLLTP replaces the restated get_hsl_LLTP (e.g. by a compiled build of the
+    same factor, for timing at full size)."""
     n = C.shape[0]
-    K = get_hsl_lltp(C, z, ctl.hsl_tol, ctl.hsl_cntl_1, ctl.hsl_cntl_2, ctl.hsl_icntl_1,
-                     ctl.hsl_icntl_2, p)
+    if ic_builder is not None:
+        K = ic_builder(C, z, p, ctl)
+    else:
+        K = get_hsl_lltp(C, z, ctl.hsl_tol, ctl.hsl_cntl_1, ctl.hsl_cntl_2, ctl.hsl_icntl_1,
+                         ctl.hsl_icntl_2, p)
     _, _, Cz_pred = pred_operators(C, CH, z, K)
     c1 = LobpcgSvdControl(m=n, n=n, r=1, tol=ctl.lambda_n_tol, side="L", normC=1.0,
                           conv_range=(1, 1))
@@ -205,7 +210,8 @@ def default_rand_block(seed: int = 0):
 
 
 def ps_prec_recy(C, rec, control: PsPrecRecyControl = None, perm: Optional[np.ndarray] = None,
-                 rand_block: Optional[Callable] = None, on_point: Optional[Callable] = None):
+                 rand_block: Optional[Callable] = None, on_point: Optional[Callable] = None,
+                 ic_builder: Optional[Callable] = None):
     """Sequential psPrecRecy over rec.Z (src/core/psPrecRecy.jl:617-1033).
 
     perm: the column ordering p (0-based) the reference takes from qr(C).cpiv
@@ -243,7 +249,7 @@ def ps_prec_recy(C, rec, control: PsPrecRecyControl = None, perm: Optional[np.nd
 
     n_builds = 0
     z_dag = complex(rec.Z[0])
-    K, lam_dag_n, gamma, _, _ = build_precond(C, CH, z_dag, p, ctl, rand_block, n_builds)
+    K, lam_dag_n, gamma, _, _ = build_precond(C, CH, z_dag, p, ctl, rand_block, n_builds, ic_builder)
     n_builds += 1
     record_build(z_dag)
     lam_upper = lam_lower = 0.0
@@ -326,7 +332,7 @@ def ps_prec_recy(C, rec, control: PsPrecRecyControl = None, perm: Optional[np.nd
         lam_upper = pred_normC ** 2
         if need_rebuild(main_recycled, lam_lower, lam_upper, ctl):
             record_build(z)
-            K, lam_dag_n, gamma, _, _ = build_precond(C, CH, z, p, ctl, rand_block, n_builds)
+            K, lam_dag_n, gamma, _, _ = build_precond(C, CH, z, p, ctl, rand_block, n_builds, ic_builder)
             M_pred, S_pred, Cz_pred = pred_operators(C, CH, z, K)
             rs_pred.p = 0
             X_pred = np.asarray(rand_block(n, r_pred, 1002 + 3 * n_builds), dtype=np.complex128)
