@@ -133,7 +133,8 @@ def _julia_struct(name):
     return [f for f, _ in fields], offs, size
 
 
-@pytest.mark.parametrize("struct,jl", [("rls_tri_control", "CTriControl"), ("rls_tri_stats", "CTriStats")])
+@pytest.mark.parametrize("struct,jl", [("rls_tri_control", "CTriControl"), ("rls_tri_stats", "CTriStats"),
+                                       ("rls_prec_control", "CPrecControl"), ("rls_prec_stats", "CPrecStats")])
 def test_julia_shim_structs_match_header(struct, jl):
     """The Julia binding in INTEGRATION.md passes every header field at the C offset, so no
     field (e.g. resume) is read from undefined padding."""
