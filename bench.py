@@ -12,6 +12,7 @@ Laplacian (n = 15625) over the leading --max-points points of its 200 x 200 trav
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -74,6 +75,10 @@ def main():
                     help="CPU-restatement time, split over --cpu-segments segments of the traversal")
     ap.add_argument("--cpu-segments", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--acceptance-checks", type=int, default=0, metavar="K",
+                    help="for up to K sampled points whose sigma differs from sigma_min by more than "
+                         "1e-4, continue the oracle from the GPU state a few points before and report "
+                         "its sigma there (dense n <= 2000)")
     ap.add_argument("--adjoint", action="store_true",
                     help="keep the certified singular vectors and time d sigma/dz (and the "
                          "Toeplitz-diagonal gradient for --workload toeplitz) after the steps")
@@ -279,6 +284,10 @@ def main():
     log(f"truth on {len(idx)} points {time.perf_counter() - t0:.1f}s: max rel {rel.max():.2e}"
         + (f", to nearest singular value {rel_any.max():.2e}" if rel_any is not None else ""))
 
+    acc_checks = None
+    if args.acceptance_checks > 0 and rel_any is not None:
+        acc_checks = acceptance_checks(args, solver, ctl, sig, rec, T, idx, rel, tr, (x_st, x_ed, y_st, y_ed, nx, ny, k),
+                                       normC)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC,
@@ -321,8 +330,41 @@ def main():
     }
     if adj is not None:
         out["adjoint"] = adj
+    if acc_checks is not None:
+        out["acceptance_checks"] = acc_checks
     print(json.dumps(out), flush=True)
     return 0
+
+
+def acceptance_checks(args, solver, ctl, sig, rec, T, idx, rel, tr, grid, normC, back=8):
+    """For up to args.acceptance_checks sampled points i whose sigma is more than 1e-4 from
+    sigma_min: the GPU state at i - back (the traversal resumed in pieces, one pass in total)
+    continued in the oracle over points i - back .. i; reports the oracle's sigma at i beside
+    the GPU's and sigma_min.  Equal values show the reference's acceptance rule certifying the
+    same non-minimal singular value."""
+    import rls_amd as R
+    from oracle import rls_oracle as O
+    miss = sorted(int(idx[j]) for j in np.argsort(-rel) if rel[j] > 1e-4)[: args.acceptance_checks]
+    out = []
+    pos = 0
+    for i in miss:
+        P = max(0, i - back)
+        if P > pos:
+            c = ctl if pos == 0 else dataclasses.replace(ctl, resume=True)
+            solver.solve(rec.Z[pos:P], c, rec)
+            pos = P
+        st0 = solver.state() if pos > 0 else None
+        orec = O.recursion_grid(*grid)
+        orec.Z = orec.Z[P:i + 1]
+        osig, ost, _ = O.ps_tri_recy(O.dense_tri_problem(T), orec,
+                                     O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6), state=st0,
+                                     X_init=None if st0 else R.initial_block(T.shape[0], ctl.r, ctl.seed))
+        j = list(idx).index(i)
+        out.append({"point": i, "gpu_sigma": float(sig[i]), "oracle_sigma": float(osig[-1]),
+                    "sigma_min": float(tr[j]), "rel_gpu_vs_oracle": float(abs(sig[i] - osig[-1]) / osig[-1]),
+                    "rel_vs_sigma_min": float(rel[j]), "oracle_refined": bool(ost.refined[-1])})
+        log(f"acceptance check at {i}: gpu {sig[i]:.8f} oracle {osig[-1]:.8f} sigma_min {tr[j]:.8f}")
+    return out
 
 
 def truth_sample(sig, count, seed=1):

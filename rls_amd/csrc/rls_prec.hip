@@ -972,8 +972,8 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
     };
 
     Space mainS, predS;
-    d.space_init(mainS, r_main, ctl.p_main, ctl.p_min_main, true, ctl.main_implict_update != 0,
-                 ctl.main_implict_recycle != 0, true);
+    if (!ctl.main_implict_recycle)   // explicit main recycling: the light-cache space of the driver
+        d.space_init(mainS, r_main, ctl.p_main, ctl.p_min_main, true, ctl.main_implict_update != 0, false, true);
     d.space_init(predS, r_pred, ctl.p_pred, ctl.p_min_pred, false, ctl.pred_implict_update != 0,
                  ctl.pred_implict_recycle != 0, false);
     DBuf<zd> Xm, Xp, Ep, CXp, Wtmp, Wp;
@@ -1005,24 +1005,69 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
         return idx >= 0 ? idx + 1 : r_pred;
     };
 
-    for (int64_t i = 0; i < nZ; ++i) {
+    // Main stage with implicit recycling: the context's own S = I recycle space (the layout of
+    // psTriRecy's) and its batched recycled Rayleigh-Ritz.  The main space changes only at
+    // main refines and a main-recycled point runs no other stage, so a window of consecutive
+    // shifts is evaluated against one snapshot; every shift before the first failing one is
+    // the sequential result (the window logic of rls_tri_solve's batched mode).
+    const bool engine_main = ctl.main_implict_recycle != 0;
+    if (engine_main) {
+        c->setup_space(r_main, ctl.p_main);
+        c->pmax_min = ctl.p_min_main;
+    }
+    auto main_p = [&]() { return engine_main ? c->p : mainS.p; };
+    const int window_max = 256;
+    double avg_run = 32.0;
+    int run_hint = 32;
+    int64_t i = 0;
+    while (i < nZ) {
+        if (engine_main && c->p > 0) {
+            const uint64_t t0 = sync_now();
+            const int B = (int)std::min<int64_t>({(int64_t)window_max, (int64_t)std::max(run_hint, 1), nZ - i});
+            std::vector<cplx> zs(Z + i, Z + i + B);
+            std::vector<double> taus(B, ctl.gap_tau), sg, res;
+            std::vector<int> rrv;
+            c->recycle_rr(zs, taus, ctl.shift, sg, res, rrv);
+            int f = B;
+            for (int b = 0; b < B; ++b) {
+                const double s0 = sg[(size_t)b * r_main];
+                const double th = std::max(ctl.main_normC * s0 * ctl.tol,
+                                           ctl.main_normC * ctl.main_normC * std::sqrt(2.0) * EPS64);
+                if (!(res[b] < th)) { f = b; break; }
+            }
+            for (int b = 0; b < f; ++b) {
+                sigma[i + b] = sg[(size_t)b * r_main];
+                st_p[i + b] = c->p;
+                st_pp[i + b] = predS.p;
+                st_rr[i + b] = rrv[b];
+            }
+            if (f == B) run_hint = std::min(window_max, 2 * B);
+            else {
+                avg_run = 0.7 * avg_run + 0.3 * (double)(f + 1);
+                run_hint = std::max(16, std::min(window_max, (int)(2.0 * avg_run) + 16));
+            }
+            t_mrec += sync_now() - t0;
+            if (f == B) { i += B; continue; }
+            i += f;
+            launch_copy2d(st, n, r_main, c->Xr.p + n * (int64_t)r_main * f, n, Xm.p, n);   // X of the failing shift
+            st_rr[i] = rrv[f];
+        }
         const cplx z = Z[i];
-        st_p[i] = mainS.p;
+        st_p[i] = main_p();
         st_pp[i] = predS.p;
         bool main_rec = false, pred_rec = false, pred_refine = false;
-        int r_svdrr = 0;
         int64_t mM = 0, mMnr = 0, mL = 0, pM = 0, pL = 0;
-        if (mainS.p > 0) {
+        if (!engine_main && mainS.p > 0) {
             const uint64_t t0 = sync_now();
             std::vector<double> sr;
-            r_svdrr = d.fast_svd_rr(mainS, z, ctl.shift, ctl.gap_tau, Xm.p, Wtmp.p, sr, &mM);
+            st_rr[i] = d.fast_svd_rr(mainS, z, ctl.shift, ctl.gap_tau, Xm.p, Wtmp.p, sr, &mM);
             main_rec = d.conv_ok(Wtmp.p, sr, ctl.tol, false, ctl.main_normC, 1, 1, 0);
             if (main_rec) sigma[i] = sr[0];
             t_mrec += sync_now() - t0;
         }
-        st_rr[i] = r_svdrr;
         if (main_rec) {
             st_mM[i] = mM;
+            ++i;
             continue;
         }
         // prediction stage (psPrecRecy.jl:735-820)
@@ -1144,7 +1189,8 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
         if (res.it > 0) {
             t0 = sync_now();
             const bool force = have_last_main && std::abs(z - last_main) <= thresh;
-            d.update_space(mainS, M, Sid, false, z, Xm.p, ctl.main_tau, 1, force);
+            if (engine_main) c->update_space(Xm.p, z, ctl.main_tau, force);
+            else d.update_space(mainS, M, Sid, false, z, Xm.p, ctl.main_tau, 1, force);
             const int64_t upd = ctl.main_implict_recycle ? 3 * r_main : r_main;
             mM += upd;
             mMnr += upd;
@@ -1160,10 +1206,12 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
         if (ctl.verbose > 0) {
             std::fprintf(stderr, "[psPrecRecy] %lld/%lld z=%.6f%+.6fi pred=%s main it=%d sigma=%.6e p=%d pred p=%d\n",
                          (long long)(i + 1), (long long)nZ, z.real(), z.imag(), pred_refine ? "refine" : "recycled",
-                         res.it, sigma[i], mainS.p, predS.p);
+                         res.it, sigma[i], main_p(), predS.p);
             std::fflush(stderr);
         }
+        ++i;
     }
+    c->resumable = false;
     c->sync();
     c->check_tri_err();
     if (stats) {

@@ -80,6 +80,7 @@ constexpr int SP_RB = 32;
 constexpr int SP_THREADS = 512;                // workgroup size
 constexpr int SP_TPR = SP_THREADS / SP_RB;      // threads per row (16)
 constexpr int SP_SLOTS = 8;                     // staged entries per thread (128 per row)
+constexpr int SP_NEAR = 4;                      // nearest blocks waited for last
 
 template <bool LOWER>
 __global__ void __launch_bounds__(SP_THREADS)
@@ -142,77 +143,100 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
         }
         if (sub == 0 && li < rows) sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));
     }
-    // wait for the earlier blocks this block reads: wave 0 polls 64 of their flags per step
-    // (one lane per block), the other waves wait at the barrier
-    if (w == 0) {
-        const int d = dep[b];
-        const int lo = LOWER ? d : b + 1, hi = LOWER ? b : d + 1;
-        bool ok = true;
-        for (int q0 = lo; q0 < hi; q0 += 64) {
-            const int q = q0 + lane;
-            if (q < hi && ok) {
-                unsigned int spins = 0;
-                while (__hip_atomic_load(&flags[(int64_t)q * nch + ch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-                       epoch) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1u << 24)) { ok = false; atomicExch(err, 1); break; }
+    // Dependencies: the blocks this one reads, [lo, hi) in block numbers.  They are waited
+    // for in two phases: first the blocks more than SP_NEAR steps behind this one in solve
+    // order (most of the entries: their contributions are gathered while the nearest blocks
+    // are still being solved), then the nearest SP_NEAR.  Wave 0 polls 64 flags per step (one
+    // lane per block, bounded), the other waves wait at the barrier.
+    const int d = dep[b];
+    const int lo = LOWER ? d : b + 1, hi = LOWER ? b : d + 1;
+    const int split = LOWER ? max(lo, b - SP_NEAR) : min(hi, b + 1 + SP_NEAR);
+    const int64_t jsplit = (int64_t)split * SP_RB;   // far: LOWER j < jsplit, UPPER j >= jsplit
+    auto wait_blocks = [&](int qa, int qb) {
+        if (w == 0) {
+            bool ok = true;
+            for (int q0 = qa; q0 < qb; q0 += 64) {
+                const int q = q0 + lane;
+                if (q < qb && ok) {
+                    unsigned int spins = 0;
+                    while (__hip_atomic_load(&flags[(int64_t)q * nch + ch], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > (1u << 24)) { ok = false; atomicExch(err, 1); break; }
+                    }
                 }
+                if (__any(!ok)) break;
             }
-            if (__any(!ok)) break;
+            if (!ok) s_ok = 0;
+            __threadfence();
         }
-        if (!ok) s_ok = 0;
-        __threadfence();
-    }
-    __syncthreads();
-    if (!s_ok) return;
+        __syncthreads();
+    };
     // this workgroup's chunk of ccw right-hand-side columns
-    for (int c0 = ccw * ch; c0 < k && c0 < ccw * ch + ccw; c0 += ccw) {
-        const int kc = min(ccw, k - c0);
-        zd* Xc = X + cs * c0;
-        if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
-        // all rows' sums over earlier blocks: staged entries two at a time (their X loads in
-        // flight together), then any entries past the staged slots
-        {
-            zd acc[8];
+    const int c0 = ccw * ch;
+    const int kc = min(ccw, k - c0);
+    zd* Xc = X + cs * c0;
+    if (tid < SP_RB * 8) xs[tid >> 3][tid & 7] = zzero();
+    zd acc[8];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) acc[c] = zzero();
-            for (int q = 0; q < SP_SLOTS; q += 2) {
-                const int j0 = s_j[q][tid], j1 = s_j[q + 1][tid];
-                if (j0 < 0) break;
-                const zd a0 = s_a[q][tid];
-                const zd a1 = (j1 >= 0) ? s_a[q + 1][tid] : zzero();
-                const int jj1 = (j1 >= 0) ? j1 : j0;
+    for (int c = 0; c < 8; ++c) acc[c] = zzero();
+    // acc += this thread's entries whose column is far (far = true) or near (far = false):
+    // staged entries two at a time (their X loads in flight together), then any entries past
+    // the staged slots, four at a time
+    auto gather = [&](bool far) {
+        for (int q = 0; q < SP_SLOTS; q += 2) {
+            int j0 = s_j[q][tid], j1 = s_j[q + 1][tid];
+            if (j0 < 0) break;
+            const bool f0 = LOWER ? (j0 < jsplit) : (j0 >= jsplit);
+            const bool f1 = j1 >= 0 && (LOWER ? (j1 < jsplit) : (j1 >= jsplit));
+            const bool u0 = (f0 == far), u1 = (j1 >= 0) && (f1 == far);
+            if (u0 || u1) {
+                const zd a0 = u0 ? s_a[q][tid] : zzero();
+                const zd a1 = u1 ? s_a[q + 1][tid] : zzero();
+                const int jj0 = u0 ? j0 : (u1 ? j1 : j0);
+                const int jj1 = u1 ? j1 : jj0;
                 zd x0[8], x1[8];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    x0[c] = (c < kc) ? Xc[j0 * rs + cs * c] : zzero();
+                    x0[c] = (c < kc) ? Xc[jj0 * rs + cs * c] : zzero();
                     x1[c] = (c < kc) ? Xc[jj1 * rs + cs * c] : zzero();
                 }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) acc[c] = zfma(a1, x1[c], zfma(a0, x0[c], acc[c]));
-                if (j1 < 0) break;
             }
-            // entries past the staged slots (long rows), four at a time: their index / value
-            // loads, then their X loads, issued back to back
-            for (int64_t e = e_over; e < e_end; e += 4 * SP_TPR) {
-                int64_t jj[4];
-                zd aa[4];
+            if (j1 < 0) break;
+        }
+        for (int64_t e = e_over; e < e_end; e += 4 * SP_TPR) {
+            int64_t jj[4];
+            zd aa[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t ee = e + (int64_t)SP_TPR * u;
-                    int64_t j = (ee < e_end) ? ci[ee] : -1;
-                    if (j == ri || (j >= r0 && j < r0 + rows)) j = -1;
-                    jj[u] = j;
-                    aa[u] = (j >= 0) ? v[ee] : zzero();
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (jj[u] < 0) continue;
-#pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        if (c < kc) acc[c] = zfma(aa[u], Xc[jj[u] * rs + cs * c], acc[c]);
-                }
+            for (int u = 0; u < 4; ++u) {
+                const int64_t ee = e + (int64_t)SP_TPR * u;
+                int64_t j = (ee < e_end) ? ci[ee] : -1;
+                if (j == ri || (j >= r0 && j < r0 + rows)) j = -1;
+                if (j >= 0 && ((LOWER ? (j < jsplit) : (j >= jsplit)) != far)) j = -1;
+                jj[u] = j;
+                aa[u] = (j >= 0) ? v[ee] : zzero();
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (jj[u] < 0) continue;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < kc) acc[c] = zfma(aa[u], Xc[jj[u] * rs + cs * c], acc[c]);
+            }
+        }
+    };
+    if (LOWER) wait_blocks(lo, split);
+    else wait_blocks(split, hi);
+    if (!s_ok) return;
+    gather(true);
+    if (LOWER) wait_blocks(split, hi);
+    else wait_blocks(lo, split);
+    if (!s_ok) return;
+    gather(false);
+    {
+        {
             for (int o = SP_TPR / 2; o > 0; o >>= 1) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
