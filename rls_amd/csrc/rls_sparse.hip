@@ -92,6 +92,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
     __shared__ int s_ok;
     __shared__ zd xs[SP_RB][8];
     __shared__ zd D[SP_RB][SP_RB + 1];
+    __shared__ zd Dinv[SP_RB][SP_RB + 1];     // inverse of the in-block triangle
     __shared__ zd ext[SP_RB][8];
     __shared__ zd sdg[SP_RB];                 // reciprocal diagonal
     __shared__ int s_j[SP_SLOTS][SP_THREADS];   // staged out-of-block column (-1: none)
@@ -142,6 +143,29 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             di += __shfl_xor(di, o);
         }
         if (sub == 0 && li < rows) sdg[li] = unit_diag ? zmk(1.0, 0.0) : zdiv(zmk(1.0, 0.0), zmk(dr, di));
+    }
+    // Inverse of the in-block triangle (its strict part in D, the reciprocal diagonal in sdg),
+    // formed before the dependency wait, off the chain: thread c < rows solves for column c
+    // of the inverse by substitution.  After the wait the block's unknowns are one parallel
+    // product Dinv * (rhs - sums over earlier blocks).
+    __syncthreads();
+    if (tid < rows) {
+        const int c = tid;
+        for (int r = 0; r < rows; ++r) Dinv[r][c] = zzero();
+        Dinv[c][c] = sdg[c];
+        if (LOWER) {
+            for (int r = c + 1; r < rows; ++r) {
+                zd acc = zzero();
+                for (int l = c; l < r; ++l) acc = zfma(D[r][l], Dinv[l][c], acc);
+                Dinv[r][c] = zneg(zmul(sdg[r], acc));
+            }
+        } else {
+            for (int r = c - 1; r >= 0; --r) {
+                zd acc = zzero();
+                for (int l = r + 1; l <= c; ++l) acc = zfma(D[r][l], Dinv[l][c], acc);
+                Dinv[r][c] = zneg(zmul(sdg[r], acc));
+            }
+        }
     }
     // Dependencies: the blocks this one reads, [lo, hi) in block numbers.  They are waited
     // for in two phases: first the blocks more than SP_NEAR steps behind this one in solve
@@ -251,23 +275,14 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
             }
         }
         __syncthreads();
-        // the in-block triangle, column by column: wave w < 4 solves columns 2w (lanes 0..31)
-        // and 2w+1 (lanes 32..63); lane r keeps row r's running right-hand side in a register, the
-        // newly solved unknown is broadcast with one shuffle and every later row subtracts
-        // its tile entry times it (entries outside the triangle are zero)
-        if (w < 4) {
-            const int cw = 2 * w + (lane >> 5), r = lane & 31;
-            zd er = (r < rows && cw < kc) ? ext[r][cw] : zzero();
-            for (int q = 0; q < rows; ++q) {
-                const int l = LOWER ? q : rows - 1 - q;
-                zd xl;
-                xl.x = __shfl(er.x, l, 32);
-                xl.y = __shfl(er.y, l, 32);
-                xl = zmul(xl, sdg[l]);
-                if (r == l) er = xl;
-                else if (LOWER ? (r > l) : (r < l)) er = zsub(er, zmul(D[r][l], xl));
+        // the in-block unknowns: x = Dinv * ext, thread (row r, column cw) one 32-term product
+        if (tid < SP_RB * 8) {
+            const int r = tid & (SP_RB - 1), cw = tid >> 5;
+            if (r < rows && cw < kc) {
+                zd acc = zzero();
+                for (int l = 0; l < rows; ++l) acc = zfma(Dinv[r][l], ext[l][cw], acc);
+                xs[r][cw] = acc;
             }
-            if (r < rows && cw < kc) xs[r][cw] = er;
         }
         __syncthreads();
         for (int idx = tid; idx < rows * 8; idx += SP_THREADS) {
