@@ -818,8 +818,10 @@ int rls_sigma_grad_toeplitz(rls_ctx* ctx, int64_t first, int64_t count, double* 
 // --------------------------------------------------------------------------------------
 extern "C" {
 
-int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_fixed, double* lam,
-                  double* G, int* rr_out) {
+// variant: 0 = the size-based choice of the library, 1 = the full-square tridiagonalisation
+// pass, 2 = the lower-triangle tiled pass
+int rls_test_heev_variant(int n, const double* A, double tau, int r, int nvec, int mode_fixed,
+                          int variant, double* lam, double* G, int* rr_out) {
     rls_ctx* ctx = nullptr;
     int rc = rls_ctx_create(0, &ctx);
     if (rc) return rc;
@@ -836,7 +838,8 @@ int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_
         drr.ensure(1);
         ctx->h2d(dA.p, reinterpret_cast<const zd*>(A), (size_t)n * n);
         ctx->h2d(dt.p, &tau, 1);
-        launch_heev_rr(ctx->st, dA.p, n, 0, n, 1, dW.p, dl.p, n, dG.p, n, 0, drr.p, dt.p, r, nvec, mode_fixed);
+        launch_heev_rr(ctx->st, dA.p, n, 0, n, 1, dW.p, dl.p, n, dG.p, n, 0, drr.p, dt.p, r, nvec, mode_fixed,
+                       variant);
         ctx->d2h(lam, dl.p, n);
         ctx->d2h(reinterpret_cast<zd*>(G), dG.p, (size_t)n * nvec);
         ctx->d2h(rr_out, drr.p, 1);
@@ -846,6 +849,11 @@ int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_
     if (rc) fprintf(stderr, "rls_test_heev: %s\n", ctx->err.c_str());
     rls_ctx_destroy(ctx);
     return rc;
+}
+
+int rls_test_heev(int n, const double* A, double tau, int r, int nvec, int mode_fixed, double* lam,
+                  double* G, int* rr_out) {
+    return rls_test_heev_variant(n, A, tau, r, nvec, mode_fixed, 0, lam, G, rr_out);
 }
 
 int rls_test_trsm(int64_t n, const double* T, double zr, double zi, int k, int upper, double* X) {

@@ -59,7 +59,7 @@ void launch_block_rot_cols(hipStream_t st, int64_t n, zd* const* arrays, int nar
 size_t heev_workspace_elems(int n, int nvec_max);
 void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
                     zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
-                    const double* taus, int r, int nvec_max, int mode_fixed);
+                    const double* taus, int r, int nvec_max, int mode_fixed, int variant = 0);
 void launch_svdrr_small(hipStream_t st, const zd* Gram, int64_t sGram, int nS, int r, bool side_L,
                         double shift_scale, zd* G, int64_t sG, double* sig, double* lam,
                         int64_t sSig, int* info, int batch);

@@ -157,274 +157,21 @@ __device__ void multisect(const double* d, const double* e2, int n, const int* i
     __syncthreads();
 }
 
-}  // namespace
 
-// A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
-//    batch stride sA (elements).  Wk: per-problem workspace (heev_workspace_elems).
-// Outputs: lam (n per problem, ascending; entries not needed by the rule stay NaN),
-//          G (ldg x nvec_max complex per problem, stride sG), rr (Ritz dimension).
-// Gap rule (find_rr_dim!): gap = lam_max / tau * eps; windows n_chk = n_init, n_init+2r, ...
-// (<= n); r_ = first i with |lam_i - lam_{i+1}| > gap inside the largest window, then
-// max(r_, r); if none, r_ = n.  Eigenvectors 1..min(r_, nvec_max) are returned.
-// mode_fixed > 0: skip the gap rule and return eigenvectors 1..mode_fixed (rr = mode_fixed).
-__global__ void __launch_bounds__(HEEV_THREADS)
-k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restrict__ Wk,
-          int64_t sW, double* __restrict__ lam_out, int64_t sL, zd* __restrict__ G, int64_t ldg,
-          int64_t sG, int* __restrict__ rr_out, const double* __restrict__ taus, int r,
-          int nvec_max, int mode_fixed) {
-    const int b = blockIdx.x;
+// Phases 3-5 of the large-n eigensolver, shared by the tridiagonalisation kernels: the
+// tridiagonal (d, e) of the per-problem workspace is copied to s_d / s_e / s_e2 (>= n doubles
+// each, in LDS), eigenvalues come from Sturm multisection with the Ritz-dimension rule of
+// find_rr_dim!, eigenvectors 0..nv-1 of the tridiagonal from inverse iteration into Y.
+__device__ void heev_tail(int b, int n, const double* d, const double* e, double* e2, double* lam,
+                          double* Y, double* scratch, double* __restrict__ lam_out, int64_t sL,
+                          int* __restrict__ rr_out, const double* __restrict__ taus, int r,
+                          int nvec_max, int mode_fixed, double* s_d, double* s_e, double* s_e2) {
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const zd* Ab = A + (int64_t)b * sA;
-    zd* M = Wk + (int64_t)b * sW;                  // n x n full Hermitian copy, ld = n
-    double* dws = reinterpret_cast<double*>(M + (int64_t)n * n);
-    double* d = dws;            // n
-    double* e = d + n;          // n
-    double* e2 = e + n;         // n
-    double* lam = e2 + n;       // n
-    zd* tau = reinterpret_cast<zd*>(lam + n);  // n complex
-    double* Y = reinterpret_cast<double*>(tau + n);   // n x nvec_max real eigenvectors of T
-    double* scratch = Y + (int64_t)n * nvec_max;     // per-vector tri-solve workspace 4n x nvec
-    __shared__ double red[HEEV_THREADS / 64];
-    __shared__ double red2s[2 * (HEEV_THREADS / 64)];
-    __shared__ zd s_alpha;
-    __shared__ zd sv[1024];    // current Householder vector v_k
-    __shared__ zd sw[1024];    // w_k
-    // pending rank-2 updates, deferred up to npmax steps: slot q holds (v, w) of an earlier
-    // step k_q at absolute row indices (offset q * n); on rows / columns after k_q the true
-    // matrix is M - sum_q (v_q w_q^H + w_q v_q^H)
-    __shared__ zd PVs[2 * HEEV_THREADS];
-    __shared__ zd PWs[2 * HEEV_THREADS];
-    __shared__ double s_red[HEEV_THREADS / 64][1 + 4 * HEEV_PMAX];
-    __shared__ double s_tot[1 + 4 * HEEV_PMAX];
-    __shared__ zd s_cw[HEEV_PMAX], s_cv[HEEV_PMAX];
-    __shared__ int s_rr;
-    __shared__ double s_part[HEEV_THREADS];
-
-    // 1. copy with Hermitian completion from the upper triangle
-    for (int64_t idx = tid; idx < (int64_t)n * n; idx += nthr) {
-        int i = (int)(idx % n), j = (int)(idx / n);
-        zd v = (i <= j) ? Ab[i + lda * j] : zconj(Ab[j + lda * i]);
-        if (i == j) v.y = 0.0;
-        M[i + (int64_t)n * j] = v;
-    }
-    __syncthreads();
-
-    // 2. tridiagonalisation (zhetd2 lower form).  The rank-2 updates are deferred: a step
-    //    whose pending list is not full only reads the trailing matrix for A22 v and corrects
-    //    the product with the pending vectors, sum_q (v_q (w_q^H v) + w_q (v_q^H v)); a step
-    //    with a full list applies every pending update inside its pass (one read + one write
-    //    per element) and empties the list.  Per element and step that is one read plus one
-    //    write every npmax steps instead of a read and a write every step.  The column of
-    //    step k and its diagonal entry get the pending updates on the fly.
     const int lane = tid & 63, wv = tid >> 6;
-    const int npmax = min(HEEV_PMAX, (2 * HEEV_THREADS) / n);    // >= 2 for n <= 1023
-    int np = 0;
-    for (int k = 0; k < n - 1; ++k) {
-        const int m = n - k - 1;                 // x = A[k+1:n, k]
-        zd* x = M + (k + 1) + (int64_t)n * k;
-        const bool flush = (np == npmax);
-        zd xi = zzero();
-        if (tid < m) {
-            xi = x[tid];
-            const int g = k + 1 + tid;
-            for (int q = 0; q < np; ++q) {
-                const zd* V = PVs + q * n;
-                const zd* W = PWs + q * n;
-                xi = zsub(xi, zadd(zmul(V[g], zconj(W[k])), zmul(W[g], zconj(V[k]))));
-            }
-        }
-        if (tid == 0) {
-            double dk = M[k + (int64_t)n * k].x;
-            for (int q = 0; q < np; ++q) {
-                const zd v = PVs[q * n + k], w = PWs[q * n + k];
-                dk -= 2.0 * (v.x * w.x + v.y * w.y);
-            }
-            d[k] = dk;
-            s_alpha = xi;
-        }
-        // one block reduction: |x'|^2 over rows k+2.., and (unless this step flushes) the
-        // dots w_q^H x', v_q^H x' over the same rows
-        {
-            const bool in = (tid >= 1 && tid < m);
-            const int nred = flush ? 1 : 1 + 4 * np;
-            const double x2 = wave_sum(in ? zabs2(xi) : 0.0);
-            if (lane == 0) s_red[wv][0] = x2;
-            if (!flush) {
-                for (int q = 0; q < np; ++q) {
-                    zd aw = zzero(), av = zzero();
-                    if (in) {
-                        const int g = k + 1 + tid;
-                        aw = zcmul(PWs[q * n + g], xi);
-                        av = zcmul(PVs[q * n + g], xi);
-                    }
-                    aw.x = wave_sum(aw.x); aw.y = wave_sum(aw.y);
-                    av.x = wave_sum(av.x); av.y = wave_sum(av.y);
-                    if (lane == 0) {
-                        s_red[wv][1 + 4 * q] = aw.x; s_red[wv][2 + 4 * q] = aw.y;
-                        s_red[wv][3 + 4 * q] = av.x; s_red[wv][4 + 4 * q] = av.y;
-                    }
-                }
-            }
-            __syncthreads();
-            if (tid < nred) {
-                double acc = 0.0;
-                for (int w2 = 0; w2 < nthr / 64; ++w2) acc += s_red[w2][tid];
-                s_tot[tid] = acc;
-            }
-            __syncthreads();
-        }
-        const double xn2 = s_tot[0];
-        const zd alpha = s_alpha;
-        zd t = zzero(), scal = zzero();
-        double beta;
-        if (xn2 == 0.0 && alpha.y == 0.0) {
-            beta = alpha.x;
-            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zzero();
-        } else {
-            double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-            beta = (alpha.x >= 0.0) ? -nrm : nrm;
-            t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
-            scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
-            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zmul(xi, scal);
-        }
-        // w_q^H v and v_q^H v over rows k+1.. (v = e_0 + scal x' on rows k+2..)
-        if (!flush && tid < np) {
-            const zd w0 = PWs[tid * n + k + 1], v0 = PVs[tid * n + k + 1];
-            s_cw[tid] = zadd(zconj(w0), zmul(scal, zmk(s_tot[1 + 4 * tid], s_tot[2 + 4 * tid])));
-            s_cv[tid] = zadd(zconj(v0), zmul(scal, zmk(s_tot[3 + 4 * tid], s_tot[4 + 4 * tid])));
-        }
-        if (tid >= 1 && tid < m) x[tid] = sv[tid];   // keep v for the back-transform
-        if (tid == 0) { e[k] = beta; tau[k] = t; }
-        __syncthreads();
-        // pass over A22 = M[k+1:, k+1:]: (flush) apply the pending updates, and A22 v
-        zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
-        const bool has_t = !(t.x == 0.0 && t.y == 0.0);
-        // each row of A22 belongs to a pair of adjacent lanes (even / odd columns), combined
-        // with one shuffle: no LDS reduction and no barrier inside the pass.  Columns are
-        // taken HB at a time with all HB loads issued before any is used.
-        {
-            constexpr int HB = 4;
-            const int h = tid & 1;
-            const int rows_per_pass = nthr >> 1;
-            const zd* PVc = PVs + (k + 1);   // column-side pending values at local index
-            const zd* PWc = PWs + (k + 1);
-            for (int i0 = 0; i0 < m; i0 += rows_per_pass) {
-                const int i = i0 + (tid >> 1);
-                zd acc0 = zzero(), acc1 = zzero();
-                if (i < m) {
-                    if (flush) {
-                        zd nrv[HEEV_PMAX], nrw[HEEV_PMAX];   // row side, negated
-#pragma unroll
-                        for (int q = 0; q < HEEV_PMAX; ++q) {
-                            nrv[q] = zzero(); nrw[q] = zzero();
-                            if (q < np) {
-                                const zd v = PVc[q * n + i], w = PWc[q * n + i];
-                                nrv[q] = zmk(-v.x, -v.y);
-                                nrw[q] = zmk(-w.x, -w.y);
-                            }
-                        }
-                        constexpr int HF = 2;   // narrower chunks: the row side of the updates stays in registers
-                        int jb = h;
-                        for (; jb + 2 * (HF - 1) < m; jb += 2 * HF) {
-                            zd a[HF], sj[HF];
-#pragma unroll
-                            for (int u = 0; u < HF; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
-#pragma unroll
-                            for (int u = 0; u < HF; ++u) sj[u] = sv[jb + 2 * u];
-#pragma unroll
-                            for (int q = 0; q < HEEV_PMAX; ++q) {
-                                if (q < np) {
-                                    zd cw[HF], cv[HF];
-#pragma unroll
-                                    for (int u = 0; u < HF; ++u) {
-                                        cw[u] = PWc[q * n + jb + 2 * u];
-                                        cv[u] = PVc[q * n + jb + 2 * u];
-                                    }
-#pragma unroll
-                                    for (int u = 0; u < HF; ++u)
-                                        a[u] = zfma(nrw[q], zconj(cv[u]), zfma(nrv[q], zconj(cw[u]), a[u]));
-                                }
-                            }
-#pragma unroll
-                            for (int u = 0; u < HF; ++u) A22[i + (int64_t)n * (jb + 2 * u)] = a[u];
-#pragma unroll
-                            for (int u = 0; u < HF; u += 2) {
-                                acc0 = zfma(a[u], sj[u], acc0);
-                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
-                            }
-                        }
-                        for (; jb < m; jb += 2) {
-                            zd a = A22[i + (int64_t)n * jb];
-#pragma unroll
-                            for (int q = 0; q < HEEV_PMAX; ++q)
-                                if (q < np)
-                                    a = zfma(nrw[q], zconj(PVc[q * n + jb]), zfma(nrv[q], zconj(PWc[q * n + jb]), a));
-                            A22[i + (int64_t)n * jb] = a;
-                            acc0 = zfma(a, sv[jb], acc0);
-                        }
-                    } else {
-                        int jb = h;
-                        for (; jb + 2 * (HB - 1) < m; jb += 2 * HB) {
-                            zd a[HB], sj[HB];
-#pragma unroll
-                            for (int u = 0; u < HB; ++u) a[u] = A22[i + (int64_t)n * (jb + 2 * u)];
-#pragma unroll
-                            for (int u = 0; u < HB; ++u) sj[u] = sv[jb + 2 * u];
-#pragma unroll
-                            for (int u = 0; u < HB; u += 2) {
-                                acc0 = zfma(a[u], sj[u], acc0);
-                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
-                            }
-                        }
-                        for (; jb < m; jb += 2) acc0 = zfma(A22[i + (int64_t)n * jb], sv[jb], acc0);
-                    }
-                }
-                zd acc = zadd(acc0, acc1);
-                acc.x += __shfl_xor(acc.x, 1);
-                acc.y += __shfl_xor(acc.y, 1);
-                if (h == 0 && i < m) {
-                    if (!flush) {
-                        for (int q = 0; q < np; ++q)
-                            acc = zsub(acc, zadd(zmul(PVc[q * n + i], s_cw[q]), zmul(PWc[q * n + i], s_cv[q])));
-                    }
-                    sw[i] = zmul(t, acc);
-                }
-            }
-        }
-        __syncthreads();
-        if (flush) np = 0;
-        if (!has_t) continue;
-        // alpha2 = -1/2 tau (w^H v); w += alpha2 v; (v, w) joins the pending list
-        double whv_r = 0.0, whv_i = 0.0;
-        if (tid < m) {
-            const zd c = zcmul(sw[tid], sv[tid]);
-            whv_r = c.x; whv_i = c.y;
-        }
-        block_sum2(whv_r, whv_i, red2s);
-        zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
-        for (int i = tid; i < m; i += nthr) {
-            PWs[np * n + k + 1 + i] = zfma(a2, sv[i], sw[i]);
-            PVs[np * n + k + 1 + i] = sv[i];
-        }
-        ++np;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        double dl = M[(n - 1) + (int64_t)n * (n - 1)].x;
-        for (int q = 0; q < np; ++q) {
-            const zd v = PVs[q * n + n - 1], w = PWs[q * n + n - 1];
-            dl -= 2.0 * (v.x * w.x + v.y * w.y);
-        }
-        d[n - 1] = dl;
-        e[n - 1] = 0.0;
-        tau[n - 1] = zzero();
-    }
-    __syncthreads();
+    __shared__ int s_rr;
     for (int i = tid; i < n; i += nthr) e2[i] = e[i] * e[i];
     __syncthreads();
-    // the tridiagonal in LDS for the Sturm counts and the inverse iterations
-    __shared__ double s_d[HEEV_THREADS], s_e[HEEV_THREADS], s_e2[HEEV_THREADS];
     for (int i = tid; i < n; i += nthr) { s_d[i] = d[i]; s_e[i] = e[i]; s_e2[i] = e2[i]; }
     __syncthreads();
 
@@ -568,7 +315,350 @@ k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restri
         }
         __syncthreads();
     }
+}
 
+}  // namespace
+
+namespace {
+// The value of lane (lane ^ X) for a double: quad permutations (DPP) for X = 1, 2, the
+// bit-mask swizzle within 32 lanes for X = 4, 8, 16 (immediate patterns, no address
+// registers), a lane permute for X = 32.
+template <int X>
+__device__ __forceinline__ double lane_xor(double v) {
+    const long long bits = __double_as_longlong(v);
+    int lo = (int)bits, hi = (int)(bits >> 32);
+    if constexpr (X == 1 || X == 2) {
+        constexpr int ctl = (X == 1) ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+        lo = __builtin_amdgcn_update_dpp(0, lo, ctl, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp(0, hi, ctl, 0xF, 0xF, false);
+    } else if constexpr (X < 32) {
+        constexpr int pat = 0x1F | (X << 10);          // and_mask 31, or_mask 0, xor_mask X
+        lo = __builtin_amdgcn_ds_swizzle(lo, pat);
+        hi = __builtin_amdgcn_ds_swizzle(hi, pat);
+    } else {
+        lo = __shfl_xor(lo, 32);
+        hi = __shfl_xor(hi, 32);
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Wave sum of a double with immediate-pattern exchanges inside each half-wave and the two
+// half sums read from lanes 0 and 32 (same value in every lane, fixed order).
+__device__ __forceinline__ double wave_sum_imm(double v) {
+    v += lane_xor<1>(v);
+    v += lane_xor<2>(v);
+    v += lane_xor<4>(v);
+    v += lane_xor<8>(v);
+    v += lane_xor<16>(v);
+    const long long bits = __double_as_longlong(v);
+    const int lo = (int)bits, hi = (int)(bits >> 32);
+    const long long b0 = ((long long)__builtin_amdgcn_readlane(hi, 0) << 32) |
+                         (unsigned int)__builtin_amdgcn_readlane(lo, 0);
+    const long long b1 = ((long long)__builtin_amdgcn_readlane(hi, 32) << 32) |
+                         (unsigned int)__builtin_amdgcn_readlane(lo, 32);
+    return __longlong_as_double(b0) + __longlong_as_double(b1);
+}
+
+// block_sum2 with wave_sum_imm (red: >= 2 * waves doubles)
+__device__ __forceinline__ void block_sum2_imm(double& a, double& b, double* red) {
+    a = wave_sum_imm(a);
+    b = wave_sum_imm(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < nw; ++i) { sa += red[2 * i]; sb += red[2 * i + 1]; }
+    __syncthreads();
+    a = sa;
+    b = sb;
+}
+
+}  // namespace
+
+// A: batch of n x n Hermitian matrices (upper triangle referenced), column-major, lda,
+//    batch stride sA (elements).  Wk: per-problem workspace (heev_workspace_elems).
+// Outputs: lam (n per problem, ascending; entries not needed by the rule stay NaN),
+//          G (ldg x nvec_max complex per problem, stride sG), rr (Ritz dimension).
+// Gap rule (find_rr_dim!): gap = lam_max / tau * eps; windows n_chk = n_init, n_init+2r, ...
+// (<= n); r_ = first i with |lam_i - lam_{i+1}| > gap inside the largest window, then
+// max(r_, r); if none, r_ = n.  Eigenvectors 1..min(r_, nvec_max) are returned.
+// mode_fixed > 0: skip the gap rule and return eigenvectors 1..mode_fixed (rr = mode_fixed).
+__global__ void __launch_bounds__(HEEV_THREADS)
+k_heev_rr(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __restrict__ Wk,
+          int64_t sW, double* __restrict__ lam_out, int64_t sL, zd* __restrict__ G, int64_t ldg,
+          int64_t sG, int* __restrict__ rr_out, const double* __restrict__ taus, int r,
+          int nvec_max, int mode_fixed) {
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const zd* Ab = A + (int64_t)b * sA;
+    zd* M = Wk + (int64_t)b * sW;                  // n x n full Hermitian copy, ld = n
+    double* dws = reinterpret_cast<double*>(M + (int64_t)n * n);
+    double* d = dws;            // n
+    double* e = d + n;          // n
+    double* e2 = e + n;         // n
+    double* lam = e2 + n;       // n
+    zd* tau = reinterpret_cast<zd*>(lam + n);  // n complex
+    double* Y = reinterpret_cast<double*>(tau + n);   // n x nvec_max real eigenvectors of T
+    double* scratch = Y + (int64_t)n * nvec_max;     // per-vector tri-solve workspace 4n x nvec
+    __shared__ double red2s[2 * (HEEV_THREADS / 64)];
+    __shared__ zd s_alpha;
+    __shared__ zd sv[1024];    // current Householder vector v_k
+    __shared__ zd sw[1024];    // w_k
+    // pending rank-2 updates, deferred up to npmax steps: slot q holds (v, w) of an earlier
+    // step k_q at absolute row indices (offset q * n); on rows / columns after k_q the true
+    // matrix is M - sum_q (v_q w_q^H + w_q v_q^H)
+    __shared__ zd PVs[2 * HEEV_THREADS];
+    __shared__ zd PWs[2 * HEEV_THREADS];
+    __shared__ double s_red[HEEV_THREADS / 64][1 + 4 * HEEV_PMAX];
+    __shared__ double s_tot[1 + 4 * HEEV_PMAX];
+    __shared__ zd s_cw[HEEV_PMAX], s_cv[HEEV_PMAX];
+
+    // 1. copy with Hermitian completion from the upper triangle
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += nthr) {
+        int i = (int)(idx % n), j = (int)(idx / n);
+        zd v = (i <= j) ? Ab[i + lda * j] : zconj(Ab[j + lda * i]);
+        if (i == j) v.y = 0.0;
+        M[i + (int64_t)n * j] = v;
+    }
+    __syncthreads();
+
+    // 2. tridiagonalisation (zhetd2 lower form).  The rank-2 updates are deferred: a step
+    //    whose pending list is not full only reads the trailing matrix for A22 v and corrects
+    //    the product with the pending vectors, sum_q (v_q (w_q^H v) + w_q (v_q^H v)); a step
+    //    with a full list applies every pending update inside its pass (one read + one write
+    //    per element) and empties the list.  Per element and step that is one read plus one
+    //    write every npmax steps instead of a read and a write every step.  The column of
+    //    step k and its diagonal entry get the pending updates on the fly.
+    const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
+    const int npmax = min(HEEV_PMAX, (2 * HEEV_THREADS) / n);    // >= 2 for n <= 1023
+    int np = 0;
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;                 // x = A[k+1:n, k]
+        zd* x = M + (k + 1) + (int64_t)n * k;
+        const bool flush = (np == npmax);
+        zd xi = zzero();
+        if (tid < m) {
+            xi = x[tid];
+            const int g = k + 1 + tid;
+            for (int q = 0; q < np; ++q) {
+                const zd* V = PVs + q * n;
+                const zd* W = PWs + q * n;
+                xi = zsub(xi, zadd(zmul(V[g], zconj(W[k])), zmul(W[g], zconj(V[k]))));
+            }
+        }
+        if (tid == 0) {
+            double dk = M[k + (int64_t)n * k].x;
+            for (int q = 0; q < np; ++q) {
+                const zd v = PVs[q * n + k], w = PWs[q * n + k];
+                dk -= 2.0 * (v.x * w.x + v.y * w.y);
+            }
+            d[k] = dk;
+            s_alpha = xi;
+        }
+        // one block reduction of |x'|^2 over rows k+2.. and (unless this step flushes) the
+        // dots w_q^H x', v_q^H x' over the same rows: every value's wave sum is formed in the
+        // same unrolled sequence, so the exchanges of different values overlap
+        {
+            const bool in = (tid >= 1 && tid < m);
+            const int nred = flush ? 1 : 1 + 4 * np;
+            double vals[1 + 4 * HEEV_PMAX];
+            vals[0] = in ? zabs2(xi) : 0.0;
+#pragma unroll
+            for (int q = 0; q < HEEV_PMAX; ++q) {
+                zd aw = zzero(), av = zzero();
+                if (!flush && q < np && in) {
+                    const int g = k + 1 + tid;
+                    aw = zcmul(PWs[q * n + g], xi);
+                    av = zcmul(PVs[q * n + g], xi);
+                }
+                vals[1 + 4 * q] = aw.x; vals[2 + 4 * q] = aw.y;
+                vals[3 + 4 * q] = av.x; vals[4 + 4 * q] = av.y;
+            }
+#pragma unroll
+            for (int v = 0; v < 1 + 4 * HEEV_PMAX; ++v)
+                if (v < nred) vals[v] = wave_sum_imm(vals[v]);
+            if (lane == 0) {
+#pragma unroll
+                for (int v = 0; v < 1 + 4 * HEEV_PMAX; ++v)
+                    if (v < nred) s_red[wv][v] = vals[v];
+            }
+            __syncthreads();
+            if (tid < nred) {
+                double acc = 0.0;
+                for (int w2 = 0; w2 < nw; ++w2) acc += s_red[w2][tid];
+                s_tot[tid] = acc;
+            }
+            __syncthreads();
+        }
+        const double xn2 = s_tot[0];
+        const zd alpha = s_alpha;
+        zd t = zzero(), scal = zzero();
+        double beta;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            beta = alpha.x;
+            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zzero();
+        } else {
+            double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = (alpha.x >= 0.0) ? -nrm : nrm;
+            t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+            scal = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+            if (tid < m) sv[tid] = (tid == 0) ? zmk(1.0, 0.0) : zmul(xi, scal);
+        }
+        // w_q^H v and v_q^H v over rows k+1.. (v = e_0 + scal x' on rows k+2..)
+        if (!flush && tid < np) {
+            const zd w0 = PWs[tid * n + k + 1], v0 = PVs[tid * n + k + 1];
+            s_cw[tid] = zadd(zconj(w0), zmul(scal, zmk(s_tot[1 + 4 * tid], s_tot[2 + 4 * tid])));
+            s_cv[tid] = zadd(zconj(v0), zmul(scal, zmk(s_tot[3 + 4 * tid], s_tot[4 + 4 * tid])));
+        }
+        if (tid >= 1 && tid < m) x[tid] = sv[tid];   // keep v for the back-transform
+        if (tid == 0) { e[k] = beta; tau[k] = t; }
+        __syncthreads();
+        // pass over A22 = M[k+1:, k+1:]: (flush) apply the pending updates, and A22 v
+        zd* A22 = M + (k + 1) + (int64_t)n * (k + 1);
+        const bool has_t = !(t.x == 0.0 && t.y == 0.0);
+        // each row of A22 belongs to L lanes of one wave (L = 2 while 2 m rows exceed the
+        // block, up to 16 for short trailing matrices, so that every thread has columns to
+        // stream), which split its columns and combine with lane exchanges: no LDS reduction
+        // and no barrier inside the pass.  The lanes sharing a column index h form a run of
+        // R = 64 / L consecutive lanes on R consecutive rows, so a load instruction reads L
+        // contiguous segments (lanes interleaved over columns halve the load throughput at
+        // L = 2).  Columns are taken HB at a time with all HB loads issued before any is
+        // used.  The row's w entry and its share of w^H v are formed here.
+        double whv_r = 0.0, whv_i = 0.0;
+        {
+            constexpr int HB = 4;
+            int lg = 1;                           // log2 L
+            while (lg < 4 && (m << (lg + 1)) <= nthr) ++lg;
+            const int L = 1 << lg;
+            const int R = 64 >> lg;                          // rows per wave
+            const int h = lane >> (6 - lg);                  // column offset of this lane
+            const int rows_per_pass = nthr >> lg;
+            const zd* PVc = PVs + (k + 1);   // column-side pending values at local index
+            const zd* PWc = PWs + (k + 1);
+            for (int i0 = 0; i0 < m; i0 += rows_per_pass) {
+                const int i = i0 + wv * R + (lane & (R - 1));
+                zd acc0 = zzero(), acc1 = zzero();
+                if (i < m) {
+                    if (flush) {
+                        zd nrv[HEEV_PMAX], nrw[HEEV_PMAX];   // row side, negated
+#pragma unroll
+                        for (int q = 0; q < HEEV_PMAX; ++q) {
+                            nrv[q] = zzero(); nrw[q] = zzero();
+                            if (q < np) {
+                                const zd v = PVc[q * n + i], w = PWc[q * n + i];
+                                nrv[q] = zmk(-v.x, -v.y);
+                                nrw[q] = zmk(-w.x, -w.y);
+                            }
+                        }
+                        constexpr int HF = 2;   // narrower chunks: the row side of the updates stays in registers
+                        int jb = h;
+                        for (; jb + L * (HF - 1) < m; jb += L * HF) {
+                            zd a[HF], sj[HF];
+#pragma unroll
+                            for (int u = 0; u < HF; ++u) a[u] = A22[i + (int64_t)n * (jb + L * u)];
+#pragma unroll
+                            for (int u = 0; u < HF; ++u) sj[u] = sv[jb + L * u];
+#pragma unroll
+                            for (int q = 0; q < HEEV_PMAX; ++q) {
+                                if (q < np) {
+                                    zd cw[HF], cv[HF];
+#pragma unroll
+                                    for (int u = 0; u < HF; ++u) {
+                                        cw[u] = PWc[q * n + jb + L * u];
+                                        cv[u] = PVc[q * n + jb + L * u];
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < HF; ++u)
+                                        a[u] = zfma(nrw[q], zconj(cv[u]), zfma(nrv[q], zconj(cw[u]), a[u]));
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < HF; ++u) A22[i + (int64_t)n * (jb + L * u)] = a[u];
+#pragma unroll
+                            for (int u = 0; u < HF; u += 2) {
+                                acc0 = zfma(a[u], sj[u], acc0);
+                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
+                            }
+                        }
+                        for (; jb < m; jb += L) {
+                            zd a = A22[i + (int64_t)n * jb];
+#pragma unroll
+                            for (int q = 0; q < HEEV_PMAX; ++q)
+                                if (q < np)
+                                    a = zfma(nrw[q], zconj(PVc[q * n + jb]), zfma(nrv[q], zconj(PWc[q * n + jb]), a));
+                            A22[i + (int64_t)n * jb] = a;
+                            acc0 = zfma(a, sv[jb], acc0);
+                        }
+                    } else {
+                        int jb = h;
+                        for (; jb + L * (HB - 1) < m; jb += L * HB) {
+                            zd a[HB], sj[HB];
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) a[u] = A22[i + (int64_t)n * (jb + L * u)];
+#pragma unroll
+                            for (int u = 0; u < HB; ++u) sj[u] = sv[jb + L * u];
+#pragma unroll
+                            for (int u = 0; u < HB; u += 2) {
+                                acc0 = zfma(a[u], sj[u], acc0);
+                                acc1 = zfma(a[u + 1], sj[u + 1], acc1);
+                            }
+                        }
+                        for (; jb < m; jb += L) acc0 = zfma(A22[i + (int64_t)n * jb], sv[jb], acc0);
+                    }
+                }
+                zd acc = zadd(acc0, acc1);
+                if (L >= 16) { acc.x += lane_xor<4>(acc.x); acc.y += lane_xor<4>(acc.y); }
+                if (L >= 8) { acc.x += lane_xor<8>(acc.x); acc.y += lane_xor<8>(acc.y); }
+                if (L >= 4) { acc.x += lane_xor<16>(acc.x); acc.y += lane_xor<16>(acc.y); }
+                acc.x += lane_xor<32>(acc.x);
+                acc.y += lane_xor<32>(acc.y);
+                if (h == 0 && i < m) {
+                    if (!flush) {
+                        for (int q = 0; q < np; ++q)
+                            acc = zsub(acc, zadd(zmul(PVc[q * n + i], s_cw[q]), zmul(PWc[q * n + i], s_cv[q])));
+                    }
+                    const zd wi = zmul(t, acc);
+                    sw[i] = wi;
+                    const zd c2 = zcmul(wi, sv[i]);
+                    whv_r += c2.x;
+                    whv_i += c2.y;
+                }
+            }
+        }
+        // alpha2 = -1/2 tau (w^H v); w += alpha2 v; (v, w) joins the pending list
+        whv_r = wave_sum_imm(whv_r);
+        whv_i = wave_sum_imm(whv_i);
+        if (lane == 0) { red2s[2 * wv] = whv_r; red2s[2 * wv + 1] = whv_i; }
+        __syncthreads();
+        if (flush) np = 0;
+        if (!has_t) continue;
+        whv_r = 0.0; whv_i = 0.0;
+        for (int w2 = 0; w2 < nw; ++w2) { whv_r += red2s[2 * w2]; whv_i += red2s[2 * w2 + 1]; }
+        zd a2 = zscale(zmul(t, zmk(whv_r, whv_i)), -0.5);
+        for (int i = tid; i < m; i += nthr) {
+            PWs[np * n + k + 1 + i] = zfma(a2, sv[i], sw[i]);
+            PVs[np * n + k + 1 + i] = sv[i];
+        }
+        ++np;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double dl = M[(n - 1) + (int64_t)n * (n - 1)].x;
+        for (int q = 0; q < np; ++q) {
+            const zd v = PVs[q * n + n - 1], w = PWs[q * n + n - 1];
+            dl -= 2.0 * (v.x * w.x + v.y * w.y);
+        }
+        d[n - 1] = dl;
+        e[n - 1] = 0.0;
+        tau[n - 1] = zzero();
+    }
+    __syncthreads();
+    __shared__ double s_d[HEEV_THREADS], s_e[HEEV_THREADS], s_e2[HEEV_THREADS];
+    heev_tail(b, n, d, e, e2, lam, Y, scratch, lam_out, sL, rr_out, taus, r, nvec_max, mode_fixed,
+              s_d, s_e, s_e2);
     // 6. the back-transformation runs in k_heev_backtr (many workgroups per problem)
 }
 
@@ -655,9 +745,9 @@ size_t heev_wave_workspace_elems(int n, int nvec_max);
 
 void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n, int batch,
                     zd* Wk, double* lam, int64_t sL, zd* G, int64_t ldg, int64_t sG, int* rr,
-                    const double* taus, int r, int nvec_max, int mode_fixed) {
+                    const double* taus, int r, int nvec_max, int mode_fixed, int variant) {
     if (n > 1023) throw RlsError{-1, "heev: n > 1023 unsupported"};
-    if (n <= 128) {
+    if (n <= 128 && variant == 0) {
         // wave-per-matrix LDS kernel; its workspace fits inside the general one
         int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
         size_t lds = (size_t)n * (n + 1) / 2 * sizeof(zd);               // packed matrix

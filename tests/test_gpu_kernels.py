@@ -15,6 +15,8 @@ def _lib():
     lib = _capi.load()
     vp = C.c_void_p
     lib.rls_test_heev.argtypes = [C.c_int, vp, C.c_double, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    lib.rls_test_heev_variant.argtypes = [C.c_int, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          vp, vp, vp]
     lib.rls_test_trsm.argtypes = [C.c_int64, vp, C.c_double, C.c_double, C.c_int, C.c_int, vp]
     lib.rls_test_tall_qr.argtypes = [C.c_int64, C.c_int, vp, vp]
     return lib
@@ -47,6 +49,46 @@ def test_heev_matches_lapack(n, nvec):
     # eigenvectors: orthonormal and A G = G diag(lam)
     assert np.linalg.norm(G.conj().T @ G - np.eye(nvec)) < 1e-9
     assert np.linalg.norm(A @ G - G * w[:nvec]) < 1e-9 * np.abs(w).max() * np.sqrt(n)
+
+
+@pytest.mark.parametrize("n,kind", [(64, "rand"), (128, "rand"), (129, "rand"), (193, "rand"),
+                                    (360, "rand"), (512, "rand"), (700, "rand"), (300, "diag"),
+                                    (260, "block"), (100, "block")])
+def test_heev_workgroup_kernel(n, kind):
+    """The workgroup eigensolver (taken for n > 128, variant 1 forces it for any n) against
+    LAPACK: eigenvalues, eigenvectors and the Ritz dimension, over trailing sizes that use
+    every lanes-per-row split of its pass (2 to 16).  'diag' has no off-diagonal mass (every
+    reflector is the identity, tau = 0); 'block' has decoupled 2 x 2 blocks (half the columns
+    already reduced).  For n <= 128 the wave kernel (variant 0) must agree with it."""
+    _say(f"heev workgroup kernel n={n} kind={kind}")
+    rng = np.random.default_rng(1000 + n)
+    if kind == "rand":
+        A, _ = _herm(n, 1000 + n, spread=3.0)
+    elif kind == "diag":
+        A = np.diag(rng.standard_normal(n)).astype(complex)
+    else:
+        A = np.zeros((n, n), dtype=complex)
+        for i in range(0, n - 1, 2):
+            a, b, c = rng.standard_normal(3)
+            z = rng.standard_normal() + 1j * rng.standard_normal()
+            A[i, i], A[i + 1, i + 1], A[i + 1, i], A[i, i + 1] = a, b, z, np.conj(z)
+    A = np.asfortranarray(0.5 * (A + A.conj().T))
+    w, _ = np.linalg.eigh(A)
+    scale = np.abs(w).max()
+    nvec = min(24, n)
+    out = {}
+    for variant in (0, 1):
+        lam = np.zeros(n)
+        G = np.zeros((n, nvec), dtype=complex, order="F")
+        rr = np.zeros(1, dtype=np.int32)
+        assert _lib().rls_test_heev_variant(n, A.ctypes.data, 1e-10, 6, nvec, nvec, variant, lam.ctypes.data,
+                                            G.ctypes.data, rr.ctypes.data) == 0
+        assert rr[0] == nvec
+        np.testing.assert_allclose(lam[:nvec], w[:nvec], rtol=0, atol=1e-12 * scale)
+        assert np.linalg.norm(G.conj().T @ G - np.eye(nvec)) < 1e-9
+        assert np.linalg.norm(A @ G - G * w[:nvec]) < 1e-9 * scale * np.sqrt(n)
+        out[variant] = lam[:nvec]
+    np.testing.assert_allclose(out[0], out[1], rtol=0, atol=1e-13 * scale)
 
 
 def test_heev_gap_rule_and_clusters():
