@@ -580,6 +580,14 @@ def bench_prec(args):
             "unit": "GB/s", "frac": None if ach is None else ach / 8000.0, "traffic": None,
             "launches": d["launches"], "avg_launch_us": 1e6 * avg_s,
             "share_of_step": d["ms"] / 1e3 / total}
+    # HBM bytes per IC solve from the FETCH_SIZE / WRITE_SIZE counter runs committed for this
+    # workload, when available
+    tpath = os.path.join(ROOT, "profiles", f"traffic_skewlap3d{n}.json")
+    if os.path.exists(tpath) and d["launches"]:
+        with open(tpath) as fh:
+            tr_ = json.load(fh)
+        roof["traffic"] = tr_["sptrsv_bytes_per_launch"]
+        roof["traffic_GBps"] = roof["traffic"] / avg_s / 1e9
     from oracle import truth
     idx = np.random.default_rng(1).choice(npts, size=min(args.truth_samples, npts), replace=False)
     tr = truth.sparse_sigma_min(A, rec.Z[idx])
