@@ -117,14 +117,21 @@ def main():
     if sp.issparse(C) and not sparse:
         C = C.toarray()
     t0 = time.perf_counter()
+    pre_parts = {}
     if sparse:
         T = Q = None
         solver = R.SparseTriSolver(C)
+    elif args.adjoint:
+        T, Q = R.schur_factor(C)     # host Schur with vectors (gradients in C's basis)
+        solver = R.TriSolver(T)
     else:
-        T, Q = R.schur_factor(C)
+        # GPU Hessenberg reduction + host QR iterations (no Schur vectors needed)
+        T = R.schur_preprocess(C, 0, timings=pre_parts)
+        Q = None
         solver = R.TriSolver(T)
     t_pre = time.perf_counter() - t0
-    log(f"{'csr setup' if sparse else 'schur'} n={n} {t_pre:.1f}s, normC={normC:.6g}")
+    log(f"{'csr setup' if sparse else 'schur'} n={n} {t_pre:.1f}s "
+        f"{' '.join(f'{k} {v:.2f}s' for k, v in pre_parts.items())}, normC={normC:.6g}")
     if args.adjoint and not sparse:
         solver.set_schur_vectors(Q)
         solver.keep_vectors(True)
@@ -321,7 +328,7 @@ def main():
         "window_shifts": int(st.n_window_shifts),
         "window_shifts_per_point": st.n_window_shifts / nZ,
         "host": host_info(),
-        "preprocess_s": t_pre,
+        "preprocess_s": t_pre, "preprocess_parts": pre_parts or None,
         "refine_factor_s": getattr(solver, "factor_time_s", None),
         "refine_factors": getattr(solver, "factor_count", None),
         "cpu_baseline": cpu,

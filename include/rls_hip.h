@@ -98,6 +98,17 @@ void rls_tri_control_default(rls_tri_control* c, double normC);
  * column-major) is copied to device memory. */
 int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld);
 
+/* Householder reduction of a dense complex matrix to upper Hessenberg form on the GPU: the
+ * O(n^3) stage of the dense Schur factorisation in pre_process (psTriRecy.jl:10-14,
+ * schur(C)); the QR iterations that finish the Schur form (LAPACK zhseqr) and any Schur-vector
+ * accumulation stay with the caller on the host.  A: n x n complex (column-major, lda), read
+ * only; H (ldh): Q^H A Q, upper Hessenberg, with the reflectors stored below the subdiagonal
+ * (the LAPACK zgehrd layout, beta real and v[0] = 1); tau: n complex, Q = H(0) ... H(n-3),
+ * H(k) = I - tau[k] v_k v_k^H, tau[n-2] = tau[n-1] = 0.  Works on a private device copy of A;
+ * the context's operator is not changed. */
+int rls_hessenberg(rls_ctx* ctx, const double* A, int64_t n, int64_t lda, double* H, int64_t ldh,
+                   double* tau);
+
 /* Sparse operator for the sparse psTriRecy overload (psTriRecy.jl:47-61, M = C, S = I here):
  * C in 0-based CSR (complex values).  Julia's 1-based SparseMatrixCSC of C is, read as CSR,
  * C^T; the shim transposes it once.  The refine factor of each refine shift z comes from the

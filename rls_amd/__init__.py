@@ -26,7 +26,7 @@ from . import _capi
 __all__ = [
     "Recursion", "recursion_grid", "recursion_grid_zigzag", "recursion_natural_permutation",
     "reorder_to_natural", "reorder_to_natural_grid", "PsTriRecyControl", "PsTriRecyStats",
-    "psTriRecy", "schur_preprocess", "schur_factor", "TriSolver", "SparseTriSolver",
+    "psTriRecy", "schur_preprocess", "schur_factor", "hessenberg", "TriSolver", "SparseTriSolver",
 ]
 
 EPS = float(np.finfo(np.float64).eps)
@@ -206,32 +206,103 @@ class PsTriRecyStats:
 # ----------------------------------------------------------------------------------------
 # Preprocessing and solver
 # ----------------------------------------------------------------------------------------
-def schur_preprocess(C) -> np.ndarray:
-    """Upper-triangular Schur factor T of C = Z T Z^H (pre_process, psTriRecy.jl:6-14).
-    Sparse input is densified: sigma_min(C - zI) is invariant under the unitary change of
-    basis, so the Schur path serves the sparse overload (psTriRecy.jl:47-61) too."""
-    import scipy.linalg as sla
+def _square_dense(C) -> np.ndarray:
     import scipy.sparse as sp
     if sp.issparse(C):
         C = C.toarray()
     C = np.asarray(C)
     if C.ndim != 2 or C.shape[0] != C.shape[1]:
         raise ValueError(f"pre_process only supports square matrices, got size {C.shape}")
-    T, _ = sla.schur(C.astype(np.complex128), output="complex")
-    return np.asfortranarray(T)
+    return np.asfortranarray(C.astype(np.complex128))
+
+
+_ZHSEQR = None
+
+
+def _zhseqr(H: np.ndarray, Z: Optional[np.ndarray] = None) -> np.ndarray:
+    """LAPACK zhseqr on the host (job 'S'): H (upper Hessenberg, Fortran order) is overwritten
+    by the triangular Schur factor; with Z given (compz 'V') the Schur vectors are accumulated
+    into it.  Called through the function table scipy.linalg.cython_lapack exports."""
+    global _ZHSEQR
+    if _ZHSEQR is None:
+        import scipy.linalg.cython_lapack as cl
+        cap = cl.__pyx_capi__["zhseqr"]
+        get_name = C.pythonapi.PyCapsule_GetName
+        get_name.restype = C.c_char_p
+        get_name.argtypes = [C.py_object]
+        get_ptr = C.pythonapi.PyCapsule_GetPointer
+        get_ptr.restype = C.c_void_p
+        get_ptr.argtypes = [C.py_object, C.c_char_p]
+        P = C.POINTER(C.c_int)
+        proto = C.CFUNCTYPE(None, C.c_char_p, C.c_char_p, P, P, P, C.c_void_p, P, C.c_void_p, C.c_void_p, P,
+                            C.c_void_p, P, P)
+        _ZHSEQR = proto(get_ptr(cap, get_name(cap)))
+    n = H.shape[0]
+    if not (H.flags.f_contiguous and H.dtype == np.complex128):
+        raise ValueError("H must be complex128 in Fortran order")
+    compz = b"N" if Z is None else b"V"
+    zbuf = np.zeros(1, dtype=np.complex128) if Z is None else Z
+    if Z is not None and not (Z.flags.f_contiguous and Z.dtype == np.complex128 and Z.shape == (n, n)):
+        raise ValueError("Z must be n x n complex128 in Fortran order")
+    nn, ilo, ihi, ldh = C.c_int(n), C.c_int(1), C.c_int(n), C.c_int(n)
+    ldz = C.c_int(1 if Z is None else n)
+    w = np.zeros(n, dtype=np.complex128)
+    info = C.c_int(0)
+    work = np.zeros(1, dtype=np.complex128)
+    lwork = C.c_int(-1)
+    _ZHSEQR(b"S", compz, C.byref(nn), C.byref(ilo), C.byref(ihi), H.ctypes.data, C.byref(ldh), w.ctypes.data,
+            zbuf.ctypes.data, C.byref(ldz), work.ctypes.data, C.byref(lwork), C.byref(info))
+    lwork = C.c_int(max(int(work[0].real), n))
+    work = np.zeros(lwork.value, dtype=np.complex128)
+    _ZHSEQR(b"S", compz, C.byref(nn), C.byref(ilo), C.byref(ihi), H.ctypes.data, C.byref(ldh), w.ctypes.data,
+            zbuf.ctypes.data, C.byref(ldz), work.ctypes.data, C.byref(lwork), C.byref(info))
+    if info.value != 0:
+        raise RuntimeError(f"zhseqr failed to converge (info = {info.value})")
+    return H
+
+
+def hessenberg(C, device: int = 0):
+    """(H, tau): Householder reduction of C to upper Hessenberg form on the GPU
+    (rls_hessenberg), H in the LAPACK zgehrd layout (reflectors below the subdiagonal)."""
+    A = _square_dense(C)
+    n = A.shape[0]
+    ctx = _capi.Context(device)
+    H = np.zeros((n, n), dtype=np.complex128, order="F")
+    tau = np.zeros(n, dtype=np.complex128)
+    ctx.check(ctx.lib.rls_hessenberg(ctx.h, A.ctypes.data, n, n, H.ctypes.data, n, tau.ctypes.data))
+    return H, tau
+
+
+def schur_preprocess(C, device: Optional[int] = None, timings: Optional[dict] = None) -> np.ndarray:
+    """Upper-triangular Schur factor T of C = Z T Z^H (pre_process, psTriRecy.jl:6-14).
+    Sparse input is densified: sigma_min(C - zI) is invariant under the unitary change of
+    basis, so the Schur path serves the sparse overload (psTriRecy.jl:47-61) too.
+
+    With a device, the Hessenberg reduction (the O(n^3) two-sided stage) runs on the GPU
+    (rls_hessenberg) and the QR iterations on the host (LAPACK zhseqr, no Schur vectors);
+    without one, scipy's schur does both on the host.  Any triangular T unitarily similar to
+    C serves: the solver only uses sigma(T - zI) = sigma(C - zI)."""
+    A = _square_dense(C)
+    if device is None:
+        import scipy.linalg as sla
+        T, _ = sla.schur(A, output="complex")
+        return np.asfortranarray(T)
+    t0 = time.perf_counter()
+    H, _ = hessenberg(A, device)
+    t1 = time.perf_counter()
+    H = np.asfortranarray(np.triu(H, -1))
+    T = np.asfortranarray(np.triu(_zhseqr(H)))
+    if timings is not None:
+        timings["hessenberg_s"] = t1 - t0
+        timings["qr_s"] = time.perf_counter() - t1
+    return T
 
 
 def schur_factor(C):
     """(T, Q) with C = Q T Q^H, T upper triangular: schur_preprocess plus the Schur vectors,
     which the adjoint needs to return d sigma / dC in C's basis."""
     import scipy.linalg as sla
-    import scipy.sparse as sp
-    if sp.issparse(C):
-        C = C.toarray()
-    C = np.asarray(C)
-    if C.ndim != 2 or C.shape[0] != C.shape[1]:
-        raise ValueError(f"pre_process only supports square matrices, got size {C.shape}")
-    T, Q = sla.schur(C.astype(np.complex128), output="complex")
+    T, Q = sla.schur(_square_dense(C), output="complex")
     return np.asfortranarray(T), np.asfortranarray(Q)
 
 
@@ -524,7 +595,7 @@ def psTriRecy(C, rec: Recursion, control: PsTriRecyControl = None, verbose: int 
         # sparse overload (psTriRecy.jl:47-61): C stays sparse, refine factors per shift
         solver = SparseTriSolver(C, device)
     else:
-        solver = TriSolver(schur_preprocess(C), device)
+        solver = TriSolver(schur_preprocess(C, device), device)
     t_pre = time.perf_counter_ns() - t0
     sigma, stats = solver.solve(rec.Z, control, rec)
     stats.preprocess_time_ns = int(t_pre)

@@ -21,6 +21,7 @@ EXPORTED = [
     "rls_last_error",
     "rls_tri_control_default",
     "rls_set_tri_dense",
+    "rls_hessenberg",
     "rls_tri_solve",
     "rls_tri_state_get",
     "rls_kernel_timing_enable",
@@ -195,6 +196,8 @@ def load() -> C.CDLL:
     lib.rls_tri_control_default.restype = None
     lib.rls_set_tri_dense.argtypes = [vp, vp, C.c_int64, C.c_int64]
     lib.rls_set_tri_dense.restype = C.c_int
+    lib.rls_hessenberg.argtypes = [vp, vp, C.c_int64, C.c_int64, vp, C.c_int64, vp]
+    lib.rls_hessenberg.restype = C.c_int
     lib.rls_tri_solve.argtypes = [vp, vp, C.c_int64, C.POINTER(TriControl), vp, C.POINTER(TriStats)]
     lib.rls_tri_solve.restype = C.c_int
     lib.rls_kernel_timing_enable.argtypes = [vp, C.c_int]
