@@ -60,6 +60,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="laser", choices=["laser", "toeplitz", "sprand", "cfd", "skewlap3d"])
     ap.add_argument("--window", type=int, default=0, help="max shifts per speculative window (0 = auto)")
+    ap.add_argument("--dense-path", default="hessenberg", choices=["hessenberg", "schur"],
+                    help="dense preprocessing: Hessenberg form on the device with a Hessenberg LU per "
+                         "refine shift, or the Schur form (device Hessenberg + host QR iterations)")
+    ap.add_argument("--sparse-path", default="auto", choices=["auto", "dense", "factor"],
+                    help="sparse C: densified (dense path) or kept sparse with a host LU per refine "
+                         "shift; auto = dense for n <= 40000")
     ap.add_argument("--max-points", type=int, default=0,
                     help="solve only the first N points of the traversal (0 = the whole grid)")
     ap.add_argument("--plain-points", type=int, default=4000,
@@ -111,27 +117,41 @@ def main():
     if args.max_points > 0:
         rec.Z = rec.Z[: args.max_points]
     import scipy.sparse as sp
-    # sparse configurations: the factor path for large n (the reference's structure), the
-    # Schur factor of the densified matrix for n <= 8000 (cheaper than per-shift sparse LUs)
-    sparse = sp.issparse(C) and n > 8000
-    if sp.issparse(C) and not sparse:
-        C = C.toarray()
+    # sparse configurations: kept sparse with a host LU per refine shift (the reference's
+    # structure) when the dense operator would not fit, else taken dense
+    sparse = sp.issparse(C) and (args.sparse_path == "factor" or (args.sparse_path == "auto" and n > 40000))
+    Cd = C.toarray() if sp.issparse(C) and not sparse else C
+    hess = not sparse and not args.adjoint and args.dense_path == "hessenberg"
     t0 = time.perf_counter()
     pre_parts = {}
+    T = H = Q = None
     if sparse:
-        T = Q = None
         solver = R.SparseTriSolver(C)
+        path = "sparse nnz=%d (host LU per refine)" % C.nnz
     elif args.adjoint:
-        T, Q = R.schur_factor(C)     # host Schur with vectors (gradients in C's basis)
+        T, Q = R.schur_factor(Cd)     # host Schur with vectors (gradients in C's basis)
         solver = R.TriSolver(T)
+        path = "dense (Schur with vectors)"
+    elif hess:
+        # pre_process on the device: Hessenberg form, neighbour-pivoted LU per refine shift
+        solver = R.TriSolver.from_dense(Cd, timings=pre_parts)
+        H = solver.operator()
+        path = "dense (device Hessenberg, Hessenberg LU per refine)"
     else:
         # GPU Hessenberg reduction + host QR iterations (no Schur vectors needed)
-        T = R.schur_preprocess(C, 0, timings=pre_parts)
-        Q = None
+        T = R.schur_preprocess(Cd, 0, timings=pre_parts)
         solver = R.TriSolver(T)
+        path = "dense (Schur: device Hessenberg + host QR)"
     t_pre = time.perf_counter() - t0
-    log(f"{'csr setup' if sparse else 'schur'} n={n} {t_pre:.1f}s "
+    log(f"{path} n={n} preprocess {t_pre:.1f}s "
         f"{' '.join(f'{k} {v:.2f}s' for k, v in pre_parts.items())}, normC={normC:.6g}")
+    from oracle import rls_oracle as O
+    if sparse:
+        oproblem = lambda: O.sparse_tri_problem(C)     # noqa: E731
+    elif hess:
+        oproblem = lambda: O.dense_hess_problem(H)     # noqa: E731
+    else:
+        oproblem = lambda: O.dense_tri_problem(T)      # noqa: E731
     if args.adjoint and not sparse:
         solver.set_schur_vectors(Q)
         solver.keep_vectors(True)
@@ -162,10 +182,10 @@ def main():
     dom = max(kt, key=lambda k: kt[k]["ms"])
     d = kt[dom]
     avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
-    if dom == "trsm":
+    if dom in ("trsm", "factor"):
         ach = d["bytes"] / max(d["launches"], 1) / avg_s / 1e9
-        roof = {"kernel": "k_trsm_persistent", "bound": "hbm", "achieved": ach, "peak": 8000.0,
-                "unit": "GB/s", "frac": ach / 8000.0, "traffic": None}
+        roof = {"kernel": "k_trsm_persistent" if dom == "trsm" else "k_hlu_factor", "bound": "hbm",
+                "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0, "traffic": None}
     else:
         ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
         roof = {"kernel": "k_zgemm_mfma" if dom == "gemm" else "k_heev_rr/k_heev_wave",
@@ -259,7 +279,7 @@ def main():
         if sparse:
             sig_i, st_i = IL.psInvLanc(C, _Points, ictl)
         else:
-            sig_i, st_i = IL.psInvLanc(None, _Points, ictl, T=T)
+            sig_i, st_i = IL.psInvLanc(None, _Points, ictl, T=T, H=H)
         t_i = time.perf_counter() - t0
         inv = {"points": int(ninv), "pts_per_s": ninv / t_i, "mv_mean": float(st_i.mv.mean()),
                "converged": bool(st_i.converged.all()),
@@ -277,27 +297,30 @@ def main():
     idx = truth_sample(sig, args.truth_samples)
     t0 = time.perf_counter()
     rel_any = None
-    if sparse:
+    if sp.issparse(C):
         tr = truth.sparse_sigma_min(C, rec.Z[idx])
     elif n <= 2000:
         import scipy.linalg as sla
-        Cd = np.asarray(C.toarray() if hasattr(C, "toarray") else C)
-        spectra = [sla.svdvals(Cd - z * np.eye(n)) for z in rec.Z[idx]]
+        Ca = np.asarray(C)
+        spectra = [sla.svdvals(Ca - z * np.eye(n)) for z in rec.Z[idx]]
         tr = np.array([sv[-1] for sv in spectra])
         rel_any = np.array([np.min(np.abs(sv - s_)) / s_ for sv, s_ in zip(spectra, sig[idx])])
     else:
-        tr = truth.tri_sigma_min(T, rec.Z[idx], block=16)
+        # inverse iteration on a Schur factor of C (the truth's own preprocessing when the
+        # solver ran on the Hessenberg form)
+        Tt = T if T is not None else R.schur_preprocess(Cd, 0)
+        tr = truth.tri_sigma_min(Tt, rec.Z[idx], block=16)
     rel = np.abs(sig[idx] - tr) / tr
     log(f"truth on {len(idx)} points {time.perf_counter() - t0:.1f}s: max rel {rel.max():.2e}"
         + (f", to nearest singular value {rel_any.max():.2e}" if rel_any is not None else ""))
 
     acc_checks = None
     if args.acceptance_checks > 0 and rel_any is not None:
-        acc_checks = acceptance_checks(args, solver, ctl, sig, rec, T, idx, rel, tr, (x_st, x_ed, y_st, y_ed, nx, ny, k),
-                                       normC)
+        acc_checks = acceptance_checks(args, solver, ctl, sig, rec, oproblem, n, idx, rel, tr,
+                                       (x_st, x_ed, y_st, y_ed, nx, ny, k), normC)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC,
+        cpu = cpu_baseline_segments(args, solver, ctl, sig, st, rec, oproblem, n, sparse, normC,
                                     (x_st, x_ed, y_st, y_ed, nx, ny, k))
 
     out = {
@@ -313,7 +336,7 @@ def main():
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} {'sparse nnz=%d (LU per refine)' % C.nnz if sparse else 'dense (Schur)'} n={n}, "
+        "config": {"workload": f"{args.workload} {path} n={n}, "
                                f"grid {nx}x{ny} (k={k}), psTriRecy tol=1e-4 p=60 r=6 recy_tau=1.6, "
                                f"normC={normC:.6g}"
                                + (f", first {nZ} traversal points" if args.max_points > 0 else ""),
@@ -343,7 +366,7 @@ def main():
     return 0
 
 
-def acceptance_checks(args, solver, ctl, sig, rec, T, idx, rel, tr, grid, normC, back=8):
+def acceptance_checks(args, solver, ctl, sig, rec, oproblem, n, idx, rel, tr, grid, normC, back=8):
     """For up to args.acceptance_checks sampled points i whose sigma is more than 1e-4 from
     sigma_min: the GPU state at i - back (the traversal resumed in pieces, one pass in total)
     continued in the oracle over points i - back .. i; reports the oracle's sigma at i beside
@@ -363,9 +386,9 @@ def acceptance_checks(args, solver, ctl, sig, rec, T, idx, rel, tr, grid, normC,
         st0 = solver.state() if pos > 0 else None
         orec = O.recursion_grid(*grid)
         orec.Z = orec.Z[P:i + 1]
-        osig, ost, _ = O.ps_tri_recy(O.dense_tri_problem(T), orec,
+        osig, ost, _ = O.ps_tri_recy(oproblem(), orec,
                                      O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6), state=st0,
-                                     X_init=None if st0 else R.initial_block(T.shape[0], ctl.r, ctl.seed))
+                                     X_init=None if st0 else R.initial_block(n, ctl.r, ctl.seed))
         j = list(idx).index(i)
         out.append({"point": i, "gpu_sigma": float(sig[i]), "oracle_sigma": float(osig[-1]),
                     "sigma_min": float(tr[j]), "rel_gpu_vs_oracle": float(abs(sig[i] - osig[-1]) / osig[-1]),
@@ -413,7 +436,7 @@ def host_info():
             "blas": blas}
 
 
-def cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC, grid):
+def cpu_baseline_segments(args, solver, ctl, sig, st, rec, oproblem, n, sparse, normC, grid):
     """The numpy/OpenBLAS FP64 restatement (the oracle) timed on segments spread across the
     traversal.  A segment starting at point P > 0 continues from the state the GPU solver
     reaches there (rls_tri_state_get: recycle space, block, acceptance state), so the CPU
@@ -427,7 +450,7 @@ def cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC, 
     starts = sorted(set(int(f * nZ) for f in np.arange(nseg) / nseg))
     per = args.cpu_seconds / len(starts)
     octl = O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6)
-    problem = O.sparse_tri_problem(C) if sparse else O.dense_tri_problem(T)
+    problem = oproblem()
 
     class Stop(Exception):
         pass
@@ -454,8 +477,7 @@ def cpu_baseline_segments(args, solver, ctl, sig, st, rec, T, C, sparse, normC, 
         def keep(i, z, what, *a):
             out[i] = (a[-1], what.startswith("refine"))
             cb(i)
-        x0 = None if state is not None else R.initial_block(T.shape[0] if T is not None else C.shape[0],
-                                                             ctl.r, ctl.seed)
+        x0 = None if state is not None else R.initial_block(n, ctl.r, ctl.seed)
         try:
             O.ps_tri_recy(problem, orec, octl, on_point=keep, state=state, X_init=x0)
         except Stop:

@@ -109,6 +109,26 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld);
 int rls_hessenberg(rls_ctx* ctx, const double* A, int64_t n, int64_t lda, double* H, int64_t ldh,
                    double* tau);
 
+/* Dense operator preprocessed on the device: replaces pre_process's schur(C) (psTriRecy.jl:
+ * 6-14, `M = UpperTriangular(F.T), S = I`) by the Householder reduction H = Q^H C Q to upper
+ * Hessenberg form, run in place on a device copy of C (n <= 65535; no host O(n^3) work, no
+ * QR iterations).  M = H, S = I: sigma(H - zI) = sigma(C - zI).  The refine factor of a shift
+ * z (psTriRecy.jl:87-99) becomes G (H - zI) = U, Gaussian elimination with partial pivoting
+ * between neighbouring rows, O(n^2) on the device; the LOBPCG solves apply U^{-1} G and
+ * G^H U^{-H}.  The same path serves the sparse overload on densified C (psTriRecy.jl:47-61:
+ * the reference refactors M - zS with sparse QR per refine; here one dense reduction serves
+ * every shift).  tau_out (optional, n complex): the reflector factors (rls_hessenberg's tau);
+ * the reflectors themselves are not kept. */
+int rls_set_dense(rls_ctx* ctx, const double* A, int64_t n, int64_t lda, double* tau_out);
+
+/* The same operator from an upper Hessenberg H computed elsewhere (entries below the
+ * subdiagonal are ignored). */
+int rls_set_hess_dense(rls_ctx* ctx, const double* H, int64_t n, int64_t ld);
+
+/* Copy of the context's dense operator (n x n, ld >= n): T (rls_set_tri_dense) or H with
+ * zeros below the subdiagonal (rls_set_dense / rls_set_hess_dense). */
+int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld);
+
 /* Sparse operator for the sparse psTriRecy overload (psTriRecy.jl:47-61, M = C, S = I here):
  * C in 0-based CSR (complex values).  Julia's 1-based SparseMatrixCSC of C is, read as CSR,
  * C^T; the shim transposes it once.  The refine factor of each refine shift z comes from the
@@ -164,7 +184,8 @@ int rls_tri_state_get(rls_ctx* ctx, rls_tri_state* info, double* V, double* J1, 
 #define RLS_KF_GEMM 0     /* batched recycled-Rayleigh-Ritz products (tiled complex GEMM) */
 #define RLS_KF_HEEV 1     /* per-shift Hermitian eigensolve of the projected matrix      */
 #define RLS_KF_TRSM 2     /* shifted triangular solves: Schur factor, sparse LU or IC factor */
-#define RLS_KF_COUNT 3
+#define RLS_KF_FACTOR 3   /* refine factors on the device: Hessenberg LU of H - zI per refine shift */
+#define RLS_KF_COUNT 4
 int rls_kernel_timing_enable(rls_ctx* ctx, int enable);
 int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,
                       double* bytes);

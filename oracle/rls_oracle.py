@@ -783,6 +783,92 @@ def dense_tri_problem(T: np.ndarray):
     return M, S, refine_op
 
 
+def dense_hess_problem(H: np.ndarray, factor: str = "lapack"):
+    """M = H (upper Hessenberg, unitarily similar to C), S = I: the dense pre_process of
+    src/core/psTriRecy.jl:6-14 with the QR iterations of schur(C) left out.  Not a reference
+    function: the library's device-preprocessed dense path runs on H, and sigma(H - zI) =
+    sigma(C - zI).  The refine operator (psTriRecy.jl:797, InvMat(adjoint(Rz))) is
+    (H - zI)^{-H}, from LAPACK's LU with partial pivoting (factor = "lapack", scipy
+    lu_factor: independent of the library's factor) or from hessenberg_lu below with
+    triangular solves (factor = "hessenberg": the library's factorisation algorithm, for
+    trajectory comparisons on operators whose acceptance decisions are sensitive to the
+    rounding of the solves)."""
+    H = np.triu(np.asarray(H, dtype=np.complex128), -1)
+    n = H.shape[0]
+    M = Operator(lambda X: H @ X, lambda X: H.conj().T @ X, n)
+    S = Operator(lambda X: X.copy(), lambda X: X.copy(), n)
+    if factor not in ("lapack", "hessenberg"):
+        raise ValueError("factor must be 'lapack' or 'hessenberg'")
+
+    def refine_op(z):
+        if factor == "hessenberg":
+            U, l, piv = hessenberg_lu(H, z)
+            return Operator(
+                lambda X: hessenberg_g_apply(l, piv, sla.solve_triangular(U, X, trans="C"), adjoint=True),
+                lambda Y: sla.solve_triangular(U, hessenberg_g_apply(l, piv, Y)),
+                n)
+        lu = sla.lu_factor(H - z * np.eye(n))
+        return Operator(lambda X: sla.lu_solve(lu, X, trans=2),   # (H - zI)^{-H} X
+                        lambda Y: sla.lu_solve(lu, Y),             # (H - zI)^{-1} Y
+                        n)
+    return M, S, refine_op
+
+
+def hessenberg_lu(H: np.ndarray, z: complex = 0.0):
+    """G (H - zI) = U for upper Hessenberg H: Gaussian elimination with partial pivoting
+    between neighbouring rows (the row-elimination order of LAPACK's getrf specialised to one
+    subdiagonal; pure loops, small n).  The active row c starts as row 0; step k compares
+    |c_k| with |a_{k+1,k}| of the incoming row r = row k + 1:
+        |r_k| >  |c_k| (swap):  U[k] = r, l_k = c_k / r_k, c <- c - l_k r
+        otherwise:             U[k] = c, l_k = r_k / c_k (0 if c_k = 0), c <- r - l_k c
+    and U[n-1] = c.  Returns (U, l, piv) with piv[k] = 1 for a swap.  Not a reference
+    function (the reference solves with a Schur factor): the library's refine factor for
+    its Hessenberg operator, pinned by G (H - zI) = U and by solves against LAPACK."""
+    H = np.triu(np.asarray(H, dtype=np.complex128), -1)
+    n = H.shape[0]
+    A = H - z * np.eye(n)
+    U = np.zeros((n, n), dtype=np.complex128)
+    l = np.zeros(max(n - 1, 0), dtype=np.complex128)
+    piv = np.zeros(max(n - 1, 0), dtype=np.int32)
+    c = A[0].copy()
+    for k in range(n - 1):
+        r = A[k + 1].copy()
+        if abs(r[k]) > abs(c[k]):
+            U[k] = r
+            l[k] = c[k] / r[k]
+            c = c - l[k] * r
+            piv[k] = 1
+        else:
+            U[k] = c
+            l[k] = r[k] / c[k] if c[k] != 0 else 0.0
+            c = r - l[k] * c
+        c[k] = 0.0
+    if n:
+        U[n - 1] = c
+    return U, l, piv
+
+
+def hessenberg_g_apply(l: np.ndarray, piv: np.ndarray, X: np.ndarray, adjoint: bool = False) -> np.ndarray:
+    """G X (adjoint = False) or G^H X for the elimination steps of hessenberg_lu: G_k mixes
+    rows k and k+1 by [[1, 0], [-l_k, 1]] (no swap) or [[0, 1], [1, -l_k]] (swap)."""
+    Y = np.array(X, dtype=np.complex128, copy=True)
+    steps = range(len(l)) if not adjoint else range(len(l) - 1, -1, -1)
+    for k in steps:
+        a, b = Y[k].copy(), Y[k + 1].copy()
+        if not adjoint:
+            if piv[k]:
+                Y[k], Y[k + 1] = b, a - l[k] * b
+            else:
+                Y[k], Y[k + 1] = a, b - l[k] * a
+        else:
+            lc = np.conj(l[k])
+            if piv[k]:
+                Y[k], Y[k + 1] = b, a - lc * b
+            else:
+                Y[k], Y[k + 1] = a - lc * b, b
+    return Y
+
+
 def sparse_tri_problem(C: sp.spmatrix, perm: Optional[np.ndarray] = None):
     """M = C[:,p], S = I[:,p] (src/core/psTriRecy.jl:47-61) with the SPQR refine replaced
     by SuperLU of M - zS (left-unitary equivalent, see module docstring)."""

@@ -130,9 +130,14 @@ class PsTriRecyControl:
     ``window`` (max shifts per window, 0 = auto), ``seed`` (initial block), ``maxit``,
     ``resume`` continues from the previous solve of the same solver (recycle space, start
     block and acceptance state), so a traversal split into several calls follows the same
-    trajectory as one call.  ``sparse_path`` for sparse C: "factor" keeps C sparse and factors C - zI at each refine
-    shift (the reference's structure), "schur" densifies and uses the Schur factor (cheaper
-    when the shifted LU fills in heavily and n is moderate), "auto" = "schur" for n <= 8000."""
+    trajectory as one call.  ``dense_path`` (dense C, or sparse C taken dense): "hessenberg"
+    reduces C to upper Hessenberg form on the device and factors H - zI at each refine shift
+    on the device (no host O(n^3) work); "schur" finishes the Schur form with host QR
+    iterations (the reference's pre_process).  ``sparse_path`` for sparse C: "factor" keeps C
+    sparse and factors C - zI on the host at each refine shift (the reference's structure),
+    "dense" (alias "schur") densifies and takes ``dense_path`` (cheaper when the shifted
+    sparse LU fills in heavily), "auto" = "dense" for n <= 40000 (a dense operator of that
+    order is 25.6 GB of HBM)."""
     tol: float = 1e-3
     normC: float = 1.0
     shift: Optional[float] = None
@@ -155,6 +160,7 @@ class PsTriRecyControl:
     maxit: int = 2000
     progress: int = 0
     sparse_path: str = "auto"
+    dense_path: str = "hessenberg"
     resume: bool = False
 
     def __post_init__(self):
@@ -164,8 +170,10 @@ class PsTriRecyControl:
             raise ValueError("lobpcg_side must be 'L' or 'S'")
         if self.mode not in ("plain", "batched"):
             raise ValueError("mode must be 'plain' or 'batched'")
-        if self.sparse_path not in ("auto", "schur", "factor"):
-            raise ValueError("sparse_path must be 'auto', 'schur' or 'factor'")
+        if self.sparse_path not in ("auto", "dense", "schur", "factor"):
+            raise ValueError("sparse_path must be 'auto', 'dense', 'schur' or 'factor'")
+        if self.dense_path not in ("hessenberg", "schur"):
+            raise ValueError("dense_path must be 'hessenberg' or 'schur'")
 
 
 @dataclass
@@ -341,11 +349,64 @@ class TriSolver:
 
     def __init__(self, T: np.ndarray, device: int = 0):
         self.ctx = _capi.Context(device)
+        self.device = device
         T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
         n = T.shape[0]
         self.n = n
         self._last_nZ = 0
         self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, n, n))
+
+    @classmethod
+    def from_dense(cls, Cmat, device: int = 0, timings: Optional[dict] = None) -> "TriSolver":
+        """Dense operator preprocessed on the device (rls_set_dense): C (dense, or sparse
+        densified) is reduced to upper Hessenberg form H = Q^H C Q in device memory and each
+        refine shift factors H - zI there (neighbour-pivoted Hessenberg LU); no Schur form and
+        no host O(n^3) work.  sigma(H - zI) = sigma(C - zI)."""
+        A = _square_dense(Cmat)
+        obj = cls.__new__(cls)
+        obj.ctx = _capi.Context(device)
+        obj.device = device
+        obj.n = A.shape[0]
+        obj._last_nZ = 0
+        t0 = time.perf_counter()
+        obj.ctx.check(obj.ctx.lib.rls_set_dense(obj.ctx.h, A.ctypes.data, obj.n, obj.n, None))
+        if timings is not None:
+            timings["hessenberg_s"] = time.perf_counter() - t0
+        return obj
+
+    @classmethod
+    def from_hessenberg(cls, H, device: int = 0) -> "TriSolver":
+        """The same operator from an upper Hessenberg H given on the host (rls_set_hess_dense;
+        entries below the subdiagonal are ignored)."""
+        H = np.asfortranarray(np.asarray(H, dtype=np.complex128))
+        obj = cls.__new__(cls)
+        obj.ctx = _capi.Context(device)
+        obj.device = device
+        obj.n = H.shape[0]
+        obj._last_nZ = 0
+        obj.ctx.check(obj.ctx.lib.rls_set_hess_dense(obj.ctx.h, H.ctypes.data, obj.n, obj.n))
+        return obj
+
+    def operator(self) -> np.ndarray:
+        """The device-resident dense operator: T (Schur) or H (Hessenberg, zero below the
+        subdiagonal) (rls_get_operator)."""
+        M = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        self.ctx.check(self.ctx.lib.rls_get_operator(self.ctx.h, M.ctypes.data, self.n))
+        return M
+
+    def apply_inverse(self, X, z: complex, adjoint: bool = False) -> np.ndarray:
+        """(M - zI)^{-1} X or (M - zI)^{-H} X with the refine factor of shift z on the device
+        (rls_dev_apply RLS_OP_TRI_INV / _H; torch device buffers)."""
+        import torch
+        X = np.asarray(X, dtype=np.complex128)
+        Xd = torch.from_numpy(np.ascontiguousarray(X.T)).to(f"cuda:{self.device}")
+        Yd = torch.empty_like(Xd)
+        op = _capi.RLS_OP_TRI_INV_H if adjoint else _capi.RLS_OP_TRI_INV
+        self.ctx.check(self.ctx.lib.rls_dev_apply(self.ctx.h, op, float(z.real), float(z.imag),
+                                                  C.c_void_p(Xd.data_ptr()), self.n, X.shape[1],
+                                                  C.c_void_p(Yd.data_ptr()), self.n))
+        self.ctx.check(self.ctx.lib.rls_dev_sync(self.ctx.h))
+        return Yd.cpu().numpy().T.copy()
 
     def state(self) -> dict:
         """The state a resumed solve would continue from (rls_tri_state_get): the recycle
@@ -431,7 +492,7 @@ class TriSolver:
     def kernel_timing(self) -> dict:
         out = {}
         for name, fam in (("gemm", _capi.RLS_KF_GEMM), ("heev", _capi.RLS_KF_HEEV),
-                          ("trsm", _capi.RLS_KF_TRSM)):
+                          ("trsm", _capi.RLS_KF_TRSM), ("factor", _capi.RLS_KF_FACTOR)):
             L = C.c_int64()
             ms, fl, by = C.c_double(), C.c_double(), C.c_double()
             self.ctx.check(self.ctx.lib.rls_kernel_timing(self.ctx.h, fam, C.byref(L), C.byref(ms),
@@ -590,10 +651,13 @@ def psTriRecy(C, rec: Recursion, control: PsTriRecyControl = None, verbose: int 
     import scipy.sparse as sp
     t0 = time.perf_counter_ns()
     use_factor = sp.issparse(C) and (control.sparse_path == "factor" or
-                                     (control.sparse_path == "auto" and C.shape[0] > 8000))
+                                     (control.sparse_path == "auto" and C.shape[0] > 40000))
     if use_factor:
         # sparse overload (psTriRecy.jl:47-61): C stays sparse, refine factors per shift
         solver = SparseTriSolver(C, device)
+    elif control.dense_path == "hessenberg":
+        # pre_process on the device: Hessenberg form, Hessenberg LU per refine shift
+        solver = TriSolver.from_dense(C, device)
     else:
         solver = TriSolver(schur_preprocess(C, device), device)
     t_pre = time.perf_counter_ns() - t0

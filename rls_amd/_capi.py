@@ -22,6 +22,9 @@ EXPORTED = [
     "rls_tri_control_default",
     "rls_set_tri_dense",
     "rls_hessenberg",
+    "rls_set_dense",
+    "rls_set_hess_dense",
+    "rls_get_operator",
     "rls_tri_solve",
     "rls_tri_state_get",
     "rls_kernel_timing_enable",
@@ -52,7 +55,7 @@ EXPORTED = [
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
-RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM = 0, 1, 2
+RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM, RLS_KF_FACTOR = 0, 1, 2, 3
 RLS_NOALLOC_GRAM_MAX_ELEMS = 1048576
 RLS_ORTH_LAPACK, RLS_ORTH_SHIFTCHOL = 0, 1
 
@@ -198,6 +201,12 @@ def load() -> C.CDLL:
     lib.rls_set_tri_dense.restype = C.c_int
     lib.rls_hessenberg.argtypes = [vp, vp, C.c_int64, C.c_int64, vp, C.c_int64, vp]
     lib.rls_hessenberg.restype = C.c_int
+    lib.rls_set_dense.argtypes = [vp, vp, C.c_int64, C.c_int64, vp]
+    lib.rls_set_dense.restype = C.c_int
+    lib.rls_set_hess_dense.argtypes = [vp, vp, C.c_int64, C.c_int64]
+    lib.rls_set_hess_dense.restype = C.c_int
+    lib.rls_get_operator.argtypes = [vp, vp, C.c_int64]
+    lib.rls_get_operator.restype = C.c_int
     lib.rls_tri_solve.argtypes = [vp, vp, C.c_int64, C.POINTER(TriControl), vp, C.POINTER(TriStats)]
     lib.rls_tri_solve.restype = C.c_int
     lib.rls_kernel_timing_enable.argtypes = [vp, C.c_int]

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["rls_engine.hip", "rls_prec.hip", "rls_tall.hip", "rls_small.hip", "rls_sparse.hip", "rls_hess.hip",
+SOURCES = ["rls_engine.hip", "rls_prec.hip", "rls_tall.hip", "rls_small.hip", "rls_sparse.hip", "rls_hess.hip", "rls_hlu.hip",
            "rls_ic.cpp"]
 LIB = os.path.join(HERE, "librls_hip.so")
 

@@ -77,6 +77,15 @@ struct rls_ctx {
     DArr<zd> T, TT;   // dense upper-triangular operator (column-major, ld n) and its upper
                       // triangle in 64 x 64 tiles for the shifted solves
     bool sparse = false;
+    // dense operator in upper Hessenberg form (T holds H, zero below the subdiagonal): the
+    // refine factor of a shift is G (H - zI) = U with U in TT and the elimination steps in
+    // hlu_l / hlu_piv (rls_hlu.hip)
+    bool hess = false;
+    DArr<zd> hlu_l;
+    DArr<int> hlu_piv;
+    DArr<unsigned int> hlu_ctr, hlu_flags;
+    DArr<int> hlu_err;
+    unsigned int hlu_epoch = 0;
     std::vector<int64_t> h_rp, h_ci;     // host CSR of C (the normal matrix of IC rebuilds)
     std::vector<cplx> h_v;
     DArr<int64_t> crp, cci, hrp, hci;
@@ -197,6 +206,7 @@ struct rls_ctx {
             f->rp.release(); f->ci.release(); f->v.release(); f->dep.release();
         }
         ic_p.release(); ic_invp.release(); ic_tmp.release();
+        hlu_l.release(); hlu_piv.release(); hlu_ctr.release(); hlu_flags.release(); hlu_err.release();
         tri_ctr.release();
         tri_flags.release();
         tri_err.release();
@@ -352,10 +362,14 @@ struct rls_ctx {
         }
         for (int c0 = 0; c0 < k; c0 += 8) {
             int kc = std::min(8, k - c0);
+            zd* Xc = X + (int64_t)n * c0;
             ++tri_epoch;
             kt_begin(RLS_KF_TRSM);
-            launch_trsm_persistent(st, n, TT.p, tri_F.p, X + (int64_t)n * c0, n, kc, upper, tri_ctr.p,
+            // Hessenberg operator: (H - zI)^{-1} = U^{-1} G, (H - zI)^{-H} = G^H U^{-H}
+            if (hess && upper) launch_hlu_apply_g(st, n, hlu_l.p, hlu_piv.p, Xc, n, kc, false);
+            launch_trsm_persistent(st, n, TT.p, tri_F.p, Xc, n, kc, upper, tri_ctr.p,
                                    tri_flags.p, tri_epoch, tri_err.p);
+            if (hess && !upper) launch_hlu_apply_g(st, n, hlu_l.p, hlu_piv.p, Xc, n, kc, true);
             // algorithmic: the triangle of T once + the RHS block read and written
             kt_end(RLS_KF_TRSM, 8.0 * 0.5 * (double)n * n * kc,
                    16.0 * (0.5 * (double)n * n + 2.0 * (double)n * kc));
@@ -372,6 +386,16 @@ struct rls_ctx {
             sync();
             throw RlsError{RLS_E_HIP, "triangular solve: dependency wait exceeded its bound"};
         }
+        if (hess && hlu_err.p) {
+            d2h(&e, hlu_err.p, 1);
+            if (e) {
+                HIP_TRY(hipMemsetAsync(hlu_err.p, 0, sizeof(int), st));
+                sync();
+                throw RlsError{e == 2 ? RLS_E_NUMERIC : RLS_E_HIP,
+                               e == 2 ? "Hessenberg factor: H - zI is exactly singular at a refine shift"
+                                      : "Hessenberg factor: dependency wait exceeded its bound"};
+            }
+        }
         if (sp_err.p) d2h(&e, sp_err.p, 1);
         if (e) {
             HIP_TRY(hipMemsetAsync(sp_err.p, 0, sizeof(int), st));
@@ -385,11 +409,15 @@ struct rls_ctx {
         else {
             trmm_part.ensure(trmm_part_elems(n));
             launch_trmm_upper(st, n, T.p, n, X, n, k, Y, n, trmm_part.p);
+            if (hess) launch_hess_subdiag(st, n, T.p, n, X, n, k, Y, n, false);
         }
     }
     void apply_MH(const zd* X, int k, zd* Y) {
         if (sparse) launch_csr_spmm(st, n, hrp.p, hci.p, hv.p, zzero(), X, n, k, Y, n);
-        else launch_trmmh_upper(st, n, T.p, n, X, n, k, Y, n);
+        else {
+            launch_trmmh_upper(st, n, T.p, n, X, n, k, Y, n);
+            if (hess) launch_hess_subdiag(st, n, T.p, n, X, n, k, Y, n, true);
+        }
     }
     // the refine operator's factor for shift z: dense -> diagonal-block inverses of T - zI;
     // sparse -> the host's factor callback, which uploads L, U (rls_set_sparse_factor)
@@ -398,6 +426,12 @@ struct rls_ctx {
     bool have_tri_fz = false;
     cplx tri_fz = cplx(0, 0);
     void refine_factor(cplx z) {
+        if (!sparse && hess) {
+            hess_factor(z);
+            have_tri_fz = true;
+            tri_fz = z;
+            return;
+        }
         if (!sparse) {
             launch_tri_factors(st, n, T.p, n, zmk(z.real(), z.imag()), tri_F.p);
             have_tri_fz = true;
@@ -412,6 +446,30 @@ struct rls_ctx {
             throw RlsError{RLS_E_NUMERIC, "factor callback failed for a refine shift"};
         have_tri_fz = true;
         tri_fz = z;
+    }
+    // G (H - zI) = U: U into the tiles TT, then U's block factors for the persistent solves;
+    // an exactly singular H - zI or an overrun wait is reported before the factor is used
+    void hess_factor(cplx z) {
+        const int nb = cdiv(n, 64);
+        if (!hlu_ctr.p || hlu_flags.n < (size_t)nb) {
+            hlu_flags.ensure((size_t)nb);
+            hlu_ctr.ensure(1);
+            hlu_err.ensure(1);
+            HIP_TRY(hipMemsetAsync(hlu_flags.p, 0, sizeof(unsigned int) * hlu_flags.n, st));
+            HIP_TRY(hipMemsetAsync(hlu_ctr.p, 0, sizeof(unsigned int), st));
+            HIP_TRY(hipMemsetAsync(hlu_err.p, 0, sizeof(int), st));
+            hlu_epoch = 0;
+        }
+        hlu_l.ensure((size_t)std::max<int64_t>(n, 1));
+        hlu_piv.ensure((size_t)std::max<int64_t>(n, 1));
+        tri_F.ensure(tri_factor_elems(n));
+        kt_begin(RLS_KF_FACTOR);
+        launch_hlu_factor(st, n, T.p, n, zmk(z.real(), z.imag()), TT.p, hlu_l.p, hlu_piv.p, hlu_ctr.p, hlu_flags.p,
+                          ++hlu_epoch, hlu_err.p);
+        launch_tri_factors_tiled(st, n, TT.p, tri_F.p);
+        // algorithmic: the upper Hessenberg part of H read once, U's triangle written once
+        kt_end(RLS_KF_FACTOR, 8.0 * 0.5 * (double)n * n, 16.0 * (double)n * n);
+        check_tri_err();
     }
     // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {

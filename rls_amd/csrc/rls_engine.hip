@@ -423,25 +423,95 @@ int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, d
     });
 }
 
+// A new dense operator of order n (hess: upper Hessenberg, else upper triangular): the old
+// operator's state (factors, Schur vectors, kept vectors, resumable trajectory) is dropped.
+static void dense_operator_reset(rls_ctx* ctx, int64_t n, bool hess) {
+    ctx->n = n;
+    ctx->sparse = false;
+    ctx->hess = hess;
+    ctx->resumable = false;
+    ctx->have_q = false;   // Schur vectors and kept singular vectors belong to the old operator
+    ctx->nkeep = 0;
+    ctx->tri_flags.release();   // block count may change: fresh ticket counters and flags
+    ctx->tri_ctr.release();
+    ctx->hlu_flags.release();
+    ctx->hlu_ctr.release();
+    ctx->have_tri_fz = false;
+    ctx->T.ensure((size_t)n * n);
+    ctx->TT.ensure(tri_tiles_elems(n));
+}
+
 int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
     return guarded(ctx, [&]() -> int {
         if (!T || n <= 0 || ld < n) throw RlsError{RLS_E_ARG, "rls_set_tri_dense: bad T / n / ld"};
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
-        ctx->n = n;
-        ctx->sparse = false;
-        ctx->resumable = false;
-        ctx->have_q = false;   // Schur vectors and kept singular vectors belong to the old T
-        ctx->nkeep = 0;
-        ctx->tri_flags.release();   // block count may change: fresh ticket counter and flags
-        ctx->tri_ctr.release();
-        ctx->have_tri_fz = false;
-        ctx->T.ensure((size_t)n * n);
+        dense_operator_reset(ctx, n, false);
         HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
                                  hipMemcpyHostToDevice, ctx->st));
         // the tiles of the shifted solves (contiguous 64 KB blocks for both directions)
-        ctx->TT.ensure(tri_tiles_elems(n));
         launch_tile_upper(ctx->st, n, ctx->T.p, n, ctx->TT.p);
+        ctx->sync();
+        return RLS_OK;
+    });
+}
+
+int rls_set_hess_dense(rls_ctx* ctx, const double* H, int64_t n, int64_t ld) {
+    return guarded(ctx, [&]() -> int {
+        if (!H || n <= 0 || ld < n) throw RlsError{RLS_E_ARG, "rls_set_hess_dense: bad H / n / ld"};
+        if (n > 65535) throw RlsError{RLS_E_ARG, "rls_set_hess_dense: n > 65535 unsupported"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
+        dense_operator_reset(ctx, n, true);
+        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, H, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice, ctx->st));
+        launch_hess_mask(ctx->st, n, ctx->T.p, n);
+        ctx->sync();
+        return RLS_OK;
+    });
+}
+
+int rls_set_dense(rls_ctx* ctx, const double* A, int64_t n, int64_t lda, double* tau_out) {
+    return guarded(ctx, [&]() -> int {
+        if (!A || n <= 0 || lda < n) throw RlsError{RLS_E_ARG, "rls_set_dense: bad A / n / lda"};
+        if (n > 65535) throw RlsError{RLS_E_ARG, "rls_set_dense: n > 65535 unsupported"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->sync();
+        dense_operator_reset(ctx, n, true);
+        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, A, sizeof(zd) * lda, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice, ctx->st));
+        // pre_process on the device: Householder reduction to Hessenberg form in place; the
+        // reflectors (below the subdiagonal) are dropped, only H = Q^H A Q is kept
+        if (n >= 3) {
+            DArr<zd> dtau, dwork;
+            try {
+                dtau.ensure((size_t)n);
+                dwork.ensure(hessenberg_work_elems(n));
+                hessenberg_device(ctx->st, n, ctx->T.p, dtau.p, dwork.p, dwork.n);
+                if (tau_out) ctx->d2h(reinterpret_cast<zd*>(tau_out), dtau.p, (size_t)n);
+                ctx->sync();
+            } catch (...) {
+                dtau.release(); dwork.release();
+                throw;
+            }
+            dtau.release(); dwork.release();
+        } else if (tau_out) {
+            for (int64_t i = 0; i < 2 * n; ++i) tau_out[i] = 0.0;
+        }
+        launch_hess_mask(ctx->st, n, ctx->T.p, n);
+        ctx->sync();
+        return RLS_OK;
+    });
+}
+
+int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld) {
+    return guarded(ctx, [&]() -> int {
+        if (ctx->sparse || !ctx->T.p || ctx->n <= 0) throw RlsError{RLS_E_STATE, "rls_get_operator: no dense operator set"};
+        if (!M || ld < ctx->n) throw RlsError{RLS_E_ARG, "rls_get_operator: bad M / ld"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = ctx->n;
+        HIP_TRY(hipMemcpy2DAsync(M, sizeof(zd) * ld, ctx->T.p, sizeof(zd) * n, sizeof(zd) * n, n,
+                                 hipMemcpyDeviceToHost, ctx->st));
         ctx->sync();
         return RLS_OK;
     });
@@ -481,6 +551,7 @@ int rls_set_csr(rls_ctx* ctx, const rls_csr* C) {
         ctx->T.release();
         ctx->TT.release();
         ctx->sparse = true;
+        ctx->hess = false;
         ctx->resumable = false;
         ctx->have_q = false;
         ctx->nkeep = 0;

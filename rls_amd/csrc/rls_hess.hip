@@ -18,6 +18,9 @@
 // The reflectors follow the LAPACK zlarfg convention (beta real, v[0] = 1), so H equals
 // zgehrd's to rounding; they are kept below the subdiagonal of the output (zgehrd layout).
 // Every reduction runs in a fixed order: the result does not depend on scheduling.
+#include <chrono>
+#include <cstdlib>
+
 #include "rls_common.h"
 #include "rls_kernels.h"
 #include "rls_ctx.h"
@@ -381,7 +384,20 @@ void hessenberg_device(hipStream_t st, int64_t n, zd* A, zd* tau, zd* work, size
     hipLaunchKernelGGL(k_hs_zero, dim3(cdiv(n, 256)), dim3(256), 0, st, tau, n);
     LAUNCH_CHECK();
     const int64_t last = n - 3;        // columns 0 .. n-3 carry two or more subdiagonal entries
+    // progress lines on stderr for long reductions when RLS_PROGRESS is set (every ~10 %)
+    const char* pe = std::getenv("RLS_PROGRESS");
+    const bool progress = pe && pe[0] && pe[0] != '0' && n >= 4096;
+    const auto t_start = std::chrono::steady_clock::now();
+    int64_t next_report = last / 10;
     for (int64_t j = 0; j <= last; j += HS_NB) {
+        if (progress && j >= next_report && j > 0) {
+            HIP_TRY(hipStreamSynchronize(st));
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+            fprintf(stderr, "[hessenberg] n=%lld: %lld of %lld columns reduced, %.1f s\n", (long long)n,
+                    (long long)j, (long long)(last + 1), s);
+            fflush(stderr);
+            next_report += last / 10;
+        }
         const int ib = (int)std::min<int64_t>(HS_NB, last - j + 1);
         hipLaunchKernelGGL(k_hs_zero, dim3(cdiv((int64_t)n * HS_NB, 256)), dim3(256), 0, st, V,
                            (int64_t)n * HS_NB);

@@ -48,6 +48,28 @@ void launch_trmmh_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, con
 constexpr int TRI_NB = 64;
 size_t tri_factor_elems(int64_t n);
 void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* F);
+// the same block factors of an upper-triangular matrix given as 64 x 64 tiles (no shift)
+void launch_tri_factors_tiled(hipStream_t st, int64_t n, const zd* TT, zd* F);
+
+// ---- dense operator in upper Hessenberg form (rls_hlu.hip) --------------------------------
+// G (H - zI) = U with adjacent-row partial pivoting: U into the tiles TT (tri_tiles_elems(n)),
+// the multipliers lmul[k] and swap bits piv[k] of steps k = 0 .. n-2; counter / flags (one
+// per 64-column tile) start at zero, epoch grows by one per call; *err = 1 if a dependency
+// wait overran its bound, 2 if H - zI is exactly singular.
+void launch_hlu_factor(hipStream_t st, int64_t n, const zd* H, int64_t ldh, zd z, zd* TT, zd* lmul, int* piv,
+                       unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err);
+// X <- G X (adjoint = false) or X <- G^H X (adjoint = true), X n x k (ld ldx)
+void launch_hlu_apply_g(hipStream_t st, int64_t n, const zd* lmul, const int* piv, zd* X, int64_t ldx, int k,
+                        bool adjoint);
+// Y += sub(H) X or Y += sub(H)^H X, sub(H) = the subdiagonal of H
+void launch_hess_subdiag(hipStream_t st, int64_t n, const zd* H, int64_t ldh, const zd* X, int64_t ldx, int k,
+                         zd* Y, int64_t ldy, bool adjoint);
+// zero the entries of H below its subdiagonal
+void launch_hess_mask(hipStream_t st, int64_t n, zd* H, int64_t ldh);
+// Householder reduction of the device matrix A (n x n, ld n) to upper Hessenberg form in
+// place (rls_hess.hip); tau: n device complex values; work: hessenberg_work_elems(n)
+size_t hessenberg_work_elems(int64_t n);
+void hessenberg_device(hipStream_t st, int64_t n, zd* A, zd* tau, zd* work, size_t work_elems);
 
 
 // Block rotations of the recycle space: for i = 0..p-2 (block index i, newest block pb),

@@ -268,7 +268,8 @@ k_trmmh_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __rest
 // is padded with the identity).  Thread c computes column c by back substitution
 // (x_a = (delta_ac - sum_{q=a+1..c} D[a][q] x_q) / D[a][a]) with the column in LDS.
 __global__ void __launch_bounds__(64)
-k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __restrict__ Dinv) {
+k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ TT, zd z,
+               zd* __restrict__ Dinv) {
     __shared__ zd D[64][65];
     __shared__ zd Xs[64][65];
     const int jb = blockIdx.x;
@@ -278,7 +279,8 @@ k_tri_diag_inv(int64_t n, const zd* __restrict__ T, int64_t ldt, zd z, zd* __res
     for (int a = 0; a < 64; ++a) {
         zd v;
         if (a < m && c < m) {
-            v = (a <= c) ? T[(o + a) + ldt * (o + c)] : zzero();
+            v = (a > c) ? zzero()
+                : TT ? TT[((int64_t)jb * (jb + 1) / 2 + jb) * 4096 + 64 * c + a] : T[(o + a) + ldt * (o + c)];
             if (a == c) v = zsub(v, z);
         } else {
             v = (a == c) ? zmk(1.0, 0.0) : zzero();
@@ -757,8 +759,8 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ D
 //   variant 1: EL_j = (T_{j-1,j} D_j^{-1})^H   (j >= 1; used by the lower solve)
 // D_j^{-1} from k_tri_diag_inv; T blocks above the diagonal, columns past n read as zero.
 __global__ void __launch_bounds__(PT_THREADS)
-k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ Dinv,
-                   zd* __restrict__ EU, zd* __restrict__ EL) {
+k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restrict__ TT,
+                   const zd* __restrict__ Dinv, zd* __restrict__ EU, zd* __restrict__ EL) {
     __shared__ zd As[64][65];
     __shared__ zd Bs[64][PT_KC];
     const int j = blockIdx.x, var = blockIdx.y;
@@ -771,6 +773,7 @@ k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* _
         const int r = idx & 63, c = idx >> 6;
         zd v;
         if (var == 0) v = Di[idx];
+        else if (TT) v = TT[tri_tile_index(j - 1, j) * 4096 + 64 * c + r];   // zero-padded past n
         else v = (o + c < n) ? T[(o - 64 + r) + ldt * (o + c)] : zzero();
         As[r][c] = v;
     }
@@ -781,7 +784,8 @@ k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* _
         for (int idx = tid; idx < 64 * PT_KC; idx += PT_THREADS) {
             const int r = idx & 63, c = c0 + (idx >> 6);
             zd v;
-            if (var == 0) v = (o + 64 + c < n) ? T[(o + r) + ldt * (o + 64 + c)] : zzero();
+            if (var == 0 && TT) v = TT[tri_tile_index(j, j + 1) * 4096 + 64 * c + r];
+            else if (var == 0) v = (o + 64 + c < n) ? T[(o + r) + ldt * (o + 64 + c)] : zzero();
             else v = Di[r + 64 * c];
             Bs[r][idx >> 6] = v;
         }
@@ -801,16 +805,24 @@ k_tri_chain_blocks(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* _
 
 size_t tri_factor_elems(int64_t n) { return (size_t)3 * cdiv(n, 64) * 4096; }
 
-void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* F) {
+static void tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, const zd* TT, zd z, zd* F) {
     const int nblk = cdiv(n, 64);
     zd* EU = F + (size_t)nblk * 4096;
     zd* EL = EU + (size_t)nblk * 4096;
-    hipLaunchKernelGGL(k_tri_diag_inv, dim3(nblk), dim3(64), 0, st, n, T, ldt, z, F);
+    hipLaunchKernelGGL(k_tri_diag_inv, dim3(nblk), dim3(64), 0, st, n, T, ldt, TT, z, F);
     LAUNCH_CHECK();
     if (nblk > 1) {
-        hipLaunchKernelGGL(k_tri_chain_blocks, dim3(nblk, 2), dim3(PT_THREADS), 0, st, n, T, ldt, F, EU, EL);
+        hipLaunchKernelGGL(k_tri_chain_blocks, dim3(nblk, 2), dim3(PT_THREADS), 0, st, n, T, ldt, TT, F, EU, EL);
         LAUNCH_CHECK();
     }
+}
+
+void launch_tri_factors(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd z, zd* F) {
+    tri_factors(st, n, T, ldt, nullptr, z, F);
+}
+
+void launch_tri_factors_tiled(hipStream_t st, int64_t n, const zd* TT, zd* F) {
+    tri_factors(st, n, nullptr, 0, TT, zzero(), F);
 }
 
 namespace {

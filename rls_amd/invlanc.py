@@ -28,7 +28,7 @@ from typing import Optional
 import numpy as np
 
 from . import _capi
-from . import Recursion, schur_preprocess
+from . import Recursion
 
 
 @dataclass
@@ -66,7 +66,8 @@ class _DeviceBackend:
     callback (SparseTriSolver): solves through rls_dev_apply, Grams and combinations through
     rls_dev_gram / rls_dev_gemm, on one stream."""
 
-    def __init__(self, T: Optional[np.ndarray] = None, Csp=None, device: int = 0):
+    def __init__(self, T: Optional[np.ndarray] = None, Csp=None, device: int = 0, H: Optional[np.ndarray] = None,
+                 A: Optional[np.ndarray] = None):
         import torch
         from .prec import _Blas
         from . import SparseTriSolver
@@ -77,10 +78,20 @@ class _DeviceBackend:
             self.ctx = self.sparse.ctx
             self.n = self.sparse.n
         else:
+            # dense: a Schur factor T (rls_set_tri_dense), an upper Hessenberg H
+            # (rls_set_hess_dense), or C itself reduced to Hessenberg form on the device
+            # (rls_set_dense); the solves of the last two use the Hessenberg LU per shift
             self.ctx = _capi.Context(device)
-            T = np.asfortranarray(np.asarray(T, dtype=np.complex128))
-            self.n = T.shape[0]
-            self.ctx.check(self.ctx.lib.rls_set_tri_dense(self.ctx.h, T.ctypes.data, self.n, self.n))
+            M = T if T is not None else (H if H is not None else A)
+            M = np.asfortranarray(np.asarray(M, dtype=np.complex128))
+            self.n = M.shape[0]
+            if T is not None:
+                rc = self.ctx.lib.rls_set_tri_dense(self.ctx.h, M.ctypes.data, self.n, self.n)
+            elif H is not None:
+                rc = self.ctx.lib.rls_set_hess_dense(self.ctx.h, M.ctypes.data, self.n, self.n)
+            else:
+                rc = self.ctx.lib.rls_set_dense(self.ctx.h, M.ctypes.data, self.n, self.n, None)
+            self.ctx.check(rc)
         self.stream = torch.cuda.Stream(self.dev)
         self.ctx.check(self.ctx.lib.rls_set_stream(self.ctx.h, C.c_void_p(self.stream.cuda_stream)))
         self.la = _Blas(self)
@@ -228,22 +239,28 @@ def _rand_vec(n: int, seed: int, i: int) -> np.ndarray:
 
 
 def psInvLanc(C, rec: Recursion, control: Optional[PsInvLancControl] = None, *, T: Optional[np.ndarray] = None,
-              device: int = 0):
+              H: Optional[np.ndarray] = None, device: int = 0):
     """sigma_min(C - zI) over rec.Z by inverse Lanczos on the device (psInvLanc.jl:87-261).
-    Returns (sigma_min, stats).  Dense C goes through its Schur factor (T when already known
-    skips pre_process); sparse C is factored per shift (stats.qr_time_ns holds the host
-    factorization time)."""
+    Returns (sigma_min, stats).  Dense C is reduced to upper Hessenberg form on the device
+    and each shift factors H - zI there (a known Schur factor T or Hessenberg form H skips
+    the reduction); sparse C is factored per shift on the host (stats.qr_time_ns holds the
+    host factorization time)."""
     import scipy.sparse as sp
     ctl = control or PsInvLancControl()
     t0 = time.perf_counter_ns()
-    sparse_in = T is None and sp.issparse(C)
+    sparse_in = T is None and H is None and sp.issparse(C)
     if sparse_in:
         if C.shape[0] != C.shape[1]:
             raise ValueError(f"psInvLanc only supports square sparse matrices, got size {C.shape}")
-    elif T is None:
-        T = schur_preprocess(C)
+        be = _DeviceBackend(Csp=C, device=device)
+    elif T is not None or H is not None:
+        be = _DeviceBackend(T=T, H=H, device=device)
+    else:
+        A = C.toarray() if sp.issparse(C) else np.asarray(C)
+        if A.ndim != 2 or A.shape[0] != A.shape[1]:
+            raise ValueError(f"pre_process only supports square matrices, got size {A.shape}")
+        be = _DeviceBackend(A=A, device=device)
     pre_ns = time.perf_counter_ns() - t0
-    be = _DeviceBackend(Csp=C, device=device) if sparse_in else _DeviceBackend(T=T, device=device)
     n = be.n
     Z = np.asarray(rec.Z, dtype=np.complex128)
     nZ = len(Z)

@@ -77,3 +77,143 @@ def test_gpu_schur_preprocess_preserves_sigma_min(n):
         s_c = sla.svdvals(C - z * np.eye(n))[-1]
         s_t = sla.svdvals(T - z * np.eye(n))[-1]
         assert abs(s_t - s_c) <= 1e-10 * max(s_c, 1e-3)
+
+
+# ---- dense operator in Hessenberg form: device preprocessing, refine factor per shift ----
+
+def _hess_of(n, seed):
+    A = np.asfortranarray(_rand(n, seed))
+    if n < 3:
+        return A
+    H, _ = R.hessenberg(A)
+    return np.asfortranarray(np.triu(H, -1))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 65, 130, 517])
+def test_hessenberg_refine_factor_solves(n):
+    """(H - zI)^{-1} X and (H - zI)^{-H} X through the C ABI (rls_dev_apply with the
+    neighbour-pivoted Hessenberg LU of the shift, the persistent triangular solves on U's
+    tiles and the elimination scans): small residuals, and equal to the oracle's restated
+    factor (hessenberg_lu) solved with LAPACK's triangular solve; k = 3 and k = 11 (two
+    column chunks)."""
+    from oracle import rls_oracle as O
+    _say(f"hessenberg refine factor n={n}")
+    H = _hess_of(n, 100 + n)
+    s = R.TriSolver.from_hessenberg(H)
+    np.testing.assert_array_equal(s.operator(), np.triu(H, -1))
+    rng = np.random.default_rng(n)
+    for z in (0.3 + 0.2j, -0.7 + 0.05j):
+        Az = np.triu(H, -1) - z * np.eye(n)
+        U, l, piv = O.hessenberg_lu(H, z)
+        for k in (3, 11):
+            X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+            for adj in (False, True):
+                Y = s.apply_inverse(X, z, adjoint=adj)
+                B = Az.conj().T if adj else Az
+                res = np.linalg.norm(B @ Y - X) / (np.linalg.norm(B) * np.linalg.norm(Y) + np.linalg.norm(X))
+                assert res <= 1e-13 * max(1, np.sqrt(n)), f"z={z} k={k} adj={adj}: residual {res:.2e}"
+                if adj:
+                    Yo = O.hessenberg_g_apply(l, piv, sla.solve_triangular(U, X, trans="C"), adjoint=True)
+                else:
+                    Yo = sla.solve_triangular(U, O.hessenberg_g_apply(l, piv, X))
+                err = np.abs(Y - Yo).max() / np.abs(Yo).max()
+                assert err <= 1e-10, f"z={z} k={k} adj={adj}: differs from the oracle factor by {err:.2e}"
+
+
+def test_device_preprocessing_is_the_hessenberg_reduction():
+    """rls_set_dense leaves exactly rls_hessenberg's H (zero below the subdiagonal) as the
+    operator, and psTriRecy on it gives sigma_min(C - zI)."""
+    _say("device preprocessing n=180")
+    n = 180
+    C = WL.laser(n)
+    s = R.TriSolver.from_dense(C)
+    H, _ = R.hessenberg(C)
+    np.testing.assert_array_equal(s.operator(), np.triu(H, -1))
+    rec = R.recursion_grid(-0.5, 0.5, -0.5, 0.5, 5, 8, 4)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=1.0, p=12, recy_tau=1.6)
+    sig, st = s.solve(rec.Z, ctl, rec)
+    from oracle import truth
+    tr = truth.svd_sigma_min(C, rec.Z)
+    rel = np.abs(sig - tr) / tr
+    _say(f"max rel err vs SVD {rel.max():.2e}, refined {st.n_pts}")
+    assert rel.max() <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["laser", "toeplitz"])
+def test_hessenberg_path_p60_matches_oracle_and_svd(name):
+    """psTriRecy on the device-preprocessed dense operator with the dense workloads'
+    controls (p = 60, r = 6, tol 1e-4): sigma within 1e-4 of FP64 SVD at every point (at the
+    refined points on the Toeplitz operator, whose recycled points may certify a non-minimal
+    singular value), and the oracle on the same H from the same start block gives the same
+    trajectory (laser: the same accept / refine decisions).  The oracle factors with the
+    library's algorithm (neighbour-pivoted Hessenberg LU).  On the Toeplitz window the
+    decisions themselves are chaotic at rounding level: the oracle alone, run on H and on H
+    perturbed by 1e-15 relative, differs at 23 of these 260 points (and with LAPACK's LU in
+    the place of this one at 31), sigma within ~1e-7; there the comparison is of sigma.
+    The laser run must reach projected matrices above 128 (the large-matrix eigensolver with
+    the gap rule)."""
+    from oracle import rls_oracle as O
+    from oracle import truth
+    n = 300
+    if name == "laser":
+        C, win, normC = WL.laser(n), (-1.1, 1.2, -1.1, 1.1, 40, 40, 20), 1.0
+    else:
+        C = WL.toeplitz_real(n).astype(np.complex128)
+        win, normC = (-4.0, 6.5, -5.0, 2.0, 40, 40, 20), float(sla.svdvals(C)[0])
+    npts = 260
+    _say(f"hessenberg path {name} n={n} p=60, first {npts} points of a 40x40 grid")
+    s = R.TriSolver.from_dense(C)
+    H = s.operator()
+    rec = R.recursion_grid(*win)
+    rec.Z = rec.Z[:npts]
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6)
+    sig, st = s.solve(rec.Z, ctl, rec)
+    orec = O.recursion_grid(*win)
+    orec.Z = orec.Z[:npts]
+    osig, ost, _ = O.ps_tri_recy(O.dense_hess_problem(H, factor="hessenberg"), orec,
+                                 O.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6),
+                                 X_init=R.initial_block(n, 6, 0))
+    tr = truth.svd_sigma_min(C, rec.Z)
+    rel, orel = np.abs(sig - tr) / tr, np.abs(sig - osig) / osig
+    mism = int(((st.lobpcg_it > 0) != ost.refined).sum())
+    _say(f"max rel err vs SVD {rel.max():.2e}, vs oracle {orel.max():.2e}, refined {st.n_pts} "
+         f"(oracle {int(ost.refined.sum())}), differing decisions {mism}, max p {int(st.p_count.max())}")
+    assert orel.max() <= 1e-4
+    if name == "laser":
+        assert int(st.p_count.max()) * ctl.r > 128
+        assert mism <= max(2, npts // 50)
+        assert rel.max() <= 1e-4
+    else:
+        # the reference's acceptance test may certify a non-minimal singular value on this
+        # operator's clusters; every GPU value is a singular value of C - zI, and the refined
+        # points give sigma_min
+        ref = st.lobpcg_it > 0
+        assert (rel[ref] <= 1e-4).all()
+
+
+def test_banded_cfd_shaped_dense_path():
+    """The CFD-Jacobian-shaped sparse workload at n = 3000 taken dense (sparse_path = "dense":
+    device Hessenberg reduction, Hessenberg LU per refine shift) on a window of its grid,
+    against sparse FP64 ground truth and against the oracle on the same H."""
+    from oracle import rls_oracle as O
+    from oracle import truth
+    n = 3000
+    C = WL.banded_cfd(n=n, half_band=60, norm2=645.74)
+    g = WL.GRIDS["cfd"]
+    dx, dy = (g[1] - g[0]) / 199, (g[3] - g[2]) / 199
+    win = (-0.4, -0.4 + 20 * dx, -0.2, -0.2 + 28 * dy, 6, 8, 4)
+    _say(f"banded cfd-shaped n={n} dense Hessenberg path, grid 6x8")
+    rec = R.recursion_grid(*win)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=645.74, p=20, recy_tau=1.6, sparse_path="dense")
+    s = R.TriSolver.from_dense(C)
+    sig, st = s.solve(rec.Z, ctl, rec)
+    tr = truth.sparse_sigma_min(C, rec.Z)
+    rel = np.abs(sig - tr) / tr
+    osig, ost, _ = O.ps_tri_recy(O.dense_hess_problem(s.operator()), O.recursion_grid(*win),
+                                 O.PsTriRecyControl(tol=1e-4, normC=645.74, p=20, recy_tau=1.6),
+                                 X_init=R.initial_block(n, 6, 0))
+    orel = np.abs(sig - osig) / osig
+    _say(f"max rel err vs truth {rel.max():.2e}, vs oracle {orel.max():.2e}, refined {st.n_pts}")
+    assert rel.max() <= 1e-4
+    assert orel.max() <= 1e-6
+    assert int(((st.lobpcg_it > 0) != ost.refined).sum()) <= 1

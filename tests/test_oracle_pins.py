@@ -12,6 +12,7 @@ the wrong field or direction) in the brute force and checks that the case tells 
 """
 import numpy as np
 import pytest
+import scipy.linalg as sla
 import scipy.sparse.linalg as spla
 
 from oracle import rls_ic as I
@@ -251,3 +252,68 @@ def test_rebuild_rule_never_and_always():
     for v in ("fields_swapped", "directions_flipped"):
         assert _rule(False, 0.5, 10.0, 0.0, np.inf, v)
         assert not _rule(False, 0.5, 10.0, 0.0, np.inf)
+
+
+# ---- Hessenberg LU of the device-preprocessed dense operator (not a reference function) ----
+
+def test_hessenberg_lu_two_by_two_closed_form():
+    """Both branches of the neighbour pivot on a 2 x 2 matrix in closed form."""
+    z = 0.5 - 0.25j
+    for a, c in ((3.0 + 1j, 0.5j), (0.1, 2.0 - 1j)):
+        b, d = 1.5 - 0.5j, -0.75 + 2j
+        H = np.array([[a, b], [c, d]], dtype=np.complex128)
+        U, l, piv = O.hessenberg_lu(H, z)
+        ap, dp = a - z, d - z
+        if abs(c) > abs(ap):
+            assert piv[0] == 1
+            np.testing.assert_allclose(U, [[c, dp], [0, b - ap / c * dp]], rtol=1e-15, atol=1e-15)
+            np.testing.assert_allclose(l[0], ap / c, rtol=1e-15)
+        else:
+            assert piv[0] == 0
+            np.testing.assert_allclose(U, [[ap, b], [0, dp - c / ap * b]], rtol=1e-15, atol=1e-15)
+            np.testing.assert_allclose(l[0], c / ap, rtol=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 90])
+def test_hessenberg_lu_reconstructs_and_solves(n):
+    """G (H - zI) = U with U upper triangular and every multiplier of modulus <= 1 (partial
+    pivoting; a rule that never swaps breaks this bound), G^H from the adjoint apply is the
+    conjugate transpose of G, and U^{-1} G b solves (H - zI) x = b like LAPACK's LU."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = sla.hessenberg(A) if n >= 3 else np.triu(A, -1)
+    z = 0.2 + 0.3j
+    U, l, piv = O.hessenberg_lu(H, z)
+    Az = np.triu(H, -1) - z * np.eye(n)
+    assert np.all(np.tril(U, -1) == 0)
+    assert np.all(np.abs(l) <= 1.0 + 1e-15)
+    G = O.hessenberg_g_apply(l, piv, np.eye(n))
+    np.testing.assert_allclose(G @ Az, U, rtol=0, atol=1e-13 * np.abs(Az).max())
+    np.testing.assert_allclose(O.hessenberg_g_apply(l, piv, np.eye(n), adjoint=True), G.conj().T,
+                               rtol=0, atol=1e-15)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x = sla.solve_triangular(U, O.hessenberg_g_apply(l, piv, b[:, None]))[:, 0]
+    xl = sla.lu_solve(sla.lu_factor(Az), b)
+    assert np.linalg.norm(Az @ x - b) <= 1e-13 * np.linalg.norm(Az) * np.linalg.norm(x) * max(1, n)
+    np.testing.assert_allclose(x, xl, rtol=0, atol=1e-9 * np.abs(xl).max())
+    if n > 10:
+        assert piv.any() and not piv.all(), "the case must exercise both pivot branches"
+
+
+def test_dense_hess_problem_solves_and_products():
+    """The oracle's Hessenberg problem: M X = H X, M^H X, and the refine operator's two
+    solves against dense solves of H - zI."""
+    rng = np.random.default_rng(5)
+    n = 40
+    H = sla.hessenberg(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    H[np.tril_indices(n, -2)] = 7.0   # entries below the subdiagonal are ignored
+    M, S, refine_op = O.dense_hess_problem(H)
+    Hh = np.triu(H, -1)
+    X = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    np.testing.assert_allclose(M.mul(X), Hh @ X, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(M.rmul(X), Hh.conj().T @ X, rtol=1e-14, atol=1e-14)
+    z = 0.1 - 0.4j
+    op = refine_op(z)
+    Az = Hh - z * np.eye(n)
+    np.testing.assert_allclose(Az.conj().T @ op.mul(X), X, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(Az @ op.rmul(X), X, rtol=0, atol=1e-10)
