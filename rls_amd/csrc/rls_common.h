@@ -45,6 +45,19 @@ __host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
 
 #define RLS_WAVE 64
 
+// Stores other workgroups read after a flag hand-off: relaxed atomic stores at agent scope
+// are written through this XCD's L2 (sc1), so the release that publishes them has no dirty
+// lines to write back (a plain store + agent-scope fence writes back the whole L2).
+__device__ __forceinline__ void st_wt(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_wt(zd* p, zd v) {
+    double* q = reinterpret_cast<double*>(p);
+    st_wt(q, v.x);
+    st_wt(q + 1, v.y);
+}
+
 // Block-wide sum of one double per thread (blockDim.x multiple of 64, <= 1024).
 __device__ __forceinline__ double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

@@ -62,17 +62,6 @@ __device__ __forceinline__ zd hl_readlane(zd v, int src) {
     return zmk(a.d, b.d);
 }
 
-// Stores whose lines must not stay dirty in this XCD's L2: relaxed atomic stores at agent
-// scope are written through (sc1), so the release that publishes a block's decisions has no
-// dirty U-tile lines to write back (a plain store + agent-scope fence writes back the whole
-// L2, megabytes of tiles on every hand-off).
-__device__ __forceinline__ void hl_store_wt(zd* p, zd v) {
-    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
 constexpr unsigned int HL_SPIN_MAX = 1u << 24;
 constexpr int HL_T = 256;                        // 4 waves: wave 0 steps, waves 1-3 move panels
 constexpr int HL_PRE = (4096 + 191) / 192;       // panel elements per lane of waves 1-3
@@ -113,7 +102,7 @@ k_hlu_factor(int64_t n, const zd* __restrict__ H, int64_t ldh, zd z, zd* __restr
         const int64_t r0 = (int64_t)j * 64;
         for (int e = tid; e < 4096; e += HL_T) {
             const int r = e & 63, c = e >> 6;
-            hl_store_wt(tile + e, (r0 + r < n && c0 + c < n) ? S[r][c] : zzero());
+            st_wt(tile + e, (r0 + r < n && c0 + c < n) ? S[r][c] : zzero());
         }
     };
     for (int j = 0; j < t; ++j) {                  // far blocks: replay block j's decisions
@@ -204,7 +193,7 @@ k_hlu_factor(int64_t n, const zd* __restrict__ H, int64_t ldh, zd z, zd* __restr
         }
         if (msteps < 64) D[msteps][lane] = carry;   // last block: U row n - 1
         if (lane < msteps) {
-            hl_store_wt(lmul + r0 + lane, lv);
+            st_wt(lmul + r0 + lane, lv);
             __hip_atomic_store(piv + r0 + lane, pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (sing && lane == 0) atomicMax(err, 2);    // exactly singular H - zI

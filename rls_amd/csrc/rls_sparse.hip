@@ -192,7 +192,7 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
                 if (__any(!ok)) break;
             }
             if (!ok) s_ok = 0;
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the blocks' x after their flags
         }
         __syncthreads();
     };
@@ -287,14 +287,13 @@ k_sptrsv(int64_t n, const int64_t* __restrict__ rp, const int64_t* __restrict__ 
         __syncthreads();
         for (int idx = tid; idx < rows * 8; idx += SP_THREADS) {
             const int lr = idx % rows, c = idx / rows;
-            if (c < kc) Xc[(r0 + lr) * rs + cs * c] = xs[lr][c];
+            if (c < kc) st_wt(Xc + (r0 + lr) * rs + cs * c, xs[lr][c]);
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        __threadfence();
+    // the release orders the workgroup's x stores (before the barrier) before the flag
+    if (tid == 0)
         __hip_atomic_store(&flags[(int64_t)b * nch + ch], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 }  // namespace

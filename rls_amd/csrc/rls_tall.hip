@@ -682,8 +682,8 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ D
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > (1u << 24)) { s_bail = 1; atomicExch(err, 1); break; }
             }
-            __threadfence();
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // block kb's x after its flag
         __syncthreads();
         if (s_bail) return;
         zd xv[PT_KC * 64 / PT_THREADS];
@@ -744,14 +744,12 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ D
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = 16 * w + (lane >> 4) + 4 * v;
-            if (i < m && c < k) xd[2 * ((o + i) + ldx * c) + (jc >= 8 ? 1 : 0)] = xo[v];
+            if (i < m && c < k) st_wt(xd + 2 * ((o + i) + ldx * c) + (jc >= 8 ? 1 : 0), xo[v]);
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        __hip_atomic_store(&flags[j], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    // the release orders the workgroup's x_j stores (before the barrier) before the flag
+    if (tid == 0) __hip_atomic_store(&flags[j], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Chain blocks of the shifted solve, 64 x 64 column-major each (blockIdx.y = variant):
