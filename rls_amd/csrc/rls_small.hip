@@ -1052,6 +1052,24 @@ k_svdrr_small(const zd* __restrict__ Gram, int64_t sGram, int nS, int r, int sid
         }
         __syncthreads();
         __shared__ double red[SMALL_THREADS / 64];
+        if (m <= 64) {
+            // wave 0 alone (lane a holds row a; every LDS entry is read and written by its own
+            // lane, so no barriers): the same sums as the block-wide reduction below
+            if (tid < 64) {
+                for (int c = 0; c < r; ++c) {
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int q = 0; q < c; ++q) {
+                            const zd t = tid < m ? zcmul(P[tid + m * q], P[tid + m * c]) : zzero();
+                            const double dr = wave_sum(t.x), di = wave_sum(t.y);
+                            if (tid < m) P[tid + m * c] = zsub(P[tid + m * c], zmul(P[tid + m * q], zmk(dr, di)));
+                        }
+                    const double nr2 = wave_sum(tid < m ? zabs2(P[tid + m * c]) : 0.0);
+                    const double nr = nr2 > 0 ? 1.0 / sqrt(nr2) : 0.0;
+                    if (tid < m) P[tid + m * c] = zscale(P[tid + m * c], nr);
+                }
+            }
+            __syncthreads();
+        } else
         for (int c = 0; c < r; ++c) {
             for (int pass = 0; pass < 2; ++pass) {
                 for (int q = 0; q < c; ++q) {
