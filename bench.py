@@ -60,9 +60,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="laser", choices=["laser", "toeplitz", "sprand", "cfd", "skewlap3d"])
     ap.add_argument("--window", type=int, default=0, help="max shifts per speculative window (0 = auto)")
-    ap.add_argument("--dense-path", default="hessenberg", choices=["hessenberg", "schur"],
+    ap.add_argument("--dense-path", default="auto", choices=["auto", "hessenberg", "schur"],
                     help="dense preprocessing: Hessenberg form on the device with a Hessenberg LU per "
-                         "refine shift, or the Schur form (device Hessenberg + host QR iterations)")
+                         "refine shift, or the Schur form (device Hessenberg + host QR iterations); "
+                         "auto = schur for n <= 8000 (fastest grid; its host QR is timed as "
+                         "preprocessing), hessenberg above")
+    ap.add_argument("--no-compare-paths", action="store_true",
+                    help="skip the end-to-end (preprocessing + grid) run of the other dense path")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the measured choice of the window cap (psTriRecy) or of the sparse "
+                         "solve's chunk widths (psPrecRecy); both leave the results unchanged")
     ap.add_argument("--sparse-path", default="auto", choices=["auto", "dense", "factor"],
                     help="sparse C: densified (dense path) or kept sparse with a host LU per refine "
                          "shift; auto = dense for n <= 40000")
@@ -121,7 +128,8 @@ def main():
     # structure) when the dense operator would not fit, else taken dense
     sparse = sp.issparse(C) and (args.sparse_path == "factor" or (args.sparse_path == "auto" and n > 40000))
     Cd = C.toarray() if sp.issparse(C) and not sparse else C
-    hess = not sparse and not args.adjoint and args.dense_path == "hessenberg"
+    dense_path = args.dense_path if args.dense_path != "auto" else ("schur" if n <= 8000 else "hessenberg")
+    hess = not sparse and not args.adjoint and dense_path == "hessenberg"
     t0 = time.perf_counter()
     pre_parts = {}
     T = H = Q = None
@@ -158,6 +166,30 @@ def main():
     ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched", progress=1,
                              window=args.window)
     nZ = len(rec.Z)
+    tuned = None
+    if not args.no_autotune and args.window == 0:
+        # the window cap by measurement on this traversal (every cap gives the same
+        # trajectory); the candidates' solves also warm the device up
+        from rls_amd import autotune as AT
+        ctl_q = dataclasses.replace(ctl, progress=0)
+        tuned = AT.tune_window(solver, rec.Z, ctl_q, rec, use_cache=False)
+        ctl = dataclasses.replace(ctl, window=tuned["window"])
+        log(f"autotune window: {tuned['window']} ({', '.join(f'{k}: {v:.2f}s' for k, v in tuned['seconds'].items())}; "
+            f"max rel diff {tuned['max_rel_diff']:.1e} at point {tuned['max_rel_diff_point']}, "
+            f"decisions differ {tuned['decisions_differ']})")
+        if tuned["max_rel_diff"] > 1e-8:
+            # the candidates' sigma at the point where they differ most, against ground truth
+            from oracle import truth
+            i = tuned["max_rel_diff_point"]
+            if sp.issparse(C):
+                t_i = float(truth.sparse_sigma_min(C, rec.Z[i:i + 1])[0])
+            else:
+                import scipy.linalg as sla
+                t_i = float(sla.svdvals(np.asarray(Cd) - rec.Z[i] * np.eye(n))[-1]) if n <= 6000 else None
+            tuned["sigma_min_at_point"] = t_i
+            if t_i:
+                log("  at that point: " + ", ".join(f"cap {k}: rel err {abs(v - t_i) / t_i:.1e}"
+                                                   for k, v in tuned["sigmas_at_point"].items()))
 
     for w in range(args.warmup):
         t0 = time.perf_counter()
@@ -176,6 +208,30 @@ def main():
     value = nZ * len(times) / total
     kt = solver.kernel_timing()
     solver.enable_kernel_timing(False)
+    # end to end (preprocessing + one grid) for both dense paths: the Schur route pays host QR
+    # iterations once, the Hessenberg route a device factor per refine shift
+    e2e = None
+    if not sparse and not args.adjoint and not args.no_compare_paths and n <= 8000:
+        e2e = {("hessenberg" if hess else "schur"): {"preprocess_s": t_pre, "grid_s": total / len(times),
+                                                      "pts_per_s": nZ / (t_pre + total / len(times))}}
+        t0 = time.perf_counter()
+        other_parts = {}
+        if hess:
+            T2 = R.schur_preprocess(Cd, 0, timings=other_parts)
+            s2 = R.TriSolver(T2)
+        else:
+            s2 = R.TriSolver.from_dense(Cd, timings=other_parts)
+        t_pre2 = time.perf_counter() - t0
+        s2.solve(rec.Z, dataclasses.replace(ctl, progress=0), rec)    # warm-up of the second context
+        t0 = time.perf_counter()
+        _, st2 = s2.solve(rec.Z, dataclasses.replace(ctl, progress=0), rec)
+        g2 = time.perf_counter() - t0
+        e2e["schur" if hess else "hessenberg"] = {"preprocess_s": t_pre2, "preprocess_parts": other_parts,
+                                                   "grid_s": g2, "pts_per_s": nZ / (t_pre2 + g2),
+                                                   "refined_points": int((st2.lobpcg_it > 0).sum())}
+        del s2
+        log("end to end: " + ", ".join(f"{k} {v['preprocess_s']:.2f}s + {v['grid_s']:.2f}s = "
+                                       f"{v['pts_per_s']:.0f} pts/s" for k, v in e2e.items()))
     log("kernel families: " + ", ".join(f"{k} {v['ms']:.0f} ms / {v['launches']} launches"
                                          for k, v in kt.items()))
     # roofline of the dominant kernel family (largest event-timed share of the step)
@@ -311,6 +367,8 @@ def main():
         Tt = T if T is not None else R.schur_preprocess(Cd, 0)
         tr = truth.tri_sigma_min(Tt, rec.Z[idx], block=16)
     rel = np.abs(sig[idx] - tr) / tr
+    worst = [{"point": int(idx[j]), "sigma": float(sig[idx[j]]), "sigma_min": float(tr[j]), "rel_err": float(rel[j]),
+              "refined": bool(st.lobpcg_it[idx[j]] > 0)} for j in np.argsort(-rel)[:5]]
     log(f"truth on {len(idx)} points {time.perf_counter() - t0:.1f}s: max rel {rel.max():.2e}"
         + (f", to nearest singular value {rel_any.max():.2e}" if rel_any is not None else ""))
 
@@ -343,7 +401,8 @@ def main():
                    "grid_points": nZ},
         "max_rel_err": float(rel.max()),
         "max_rel_err_nearest_singular_value": None if rel_any is None else float(rel_any.max()),
-        "rel_err_samples": int(len(idx)),
+        "rel_err_samples": int(len(idx)), "rel_err_worst": worst,
+        "rel_err_above_1e-4": int((rel > 1e-4).sum()),
         "plain_fp64": None if plain is None else {**plain, "mid_grid": mid},
         "psInvLanc": inv,
         "refined_points": int((st.lobpcg_it > 0).sum()),
@@ -351,7 +410,7 @@ def main():
         "window_shifts": int(st.n_window_shifts),
         "window_shifts_per_point": st.n_window_shifts / nZ,
         "host": host_info(),
-        "preprocess_s": t_pre, "preprocess_parts": pre_parts or None,
+        "preprocess_s": t_pre, "preprocess_parts": pre_parts or None, "autotune": tuned, "end_to_end": e2e,
         "refine_factor_s": getattr(solver, "factor_time_s", None),
         "refine_factors": getattr(solver, "factor_count", None),
         "cpu_baseline": cpu,
@@ -398,17 +457,20 @@ def acceptance_checks(args, solver, ctl, sig, rec, oproblem, n, idx, rel, tr, gr
 
 
 def truth_sample(sig, count, seed=1):
-    """Grid points checked against ground truth: half uniformly at random, half among the
-    smallest computed sigma (where a non-minimal acceptance or an FP error would show)."""
+    """Grid points checked against ground truth: the 16 smallest computed sigma (points next
+    to an eigenvalue, where the recycled sigma's relative accuracy is lowest), then half of
+    the rest among the smallest 5 % and half uniformly at random."""
     nZ = len(sig)
     count = min(count, nZ)
     rng = np.random.default_rng(seed)
-    n_small = count // 2
     order = np.argsort(sig)
-    small = rng.choice(order[: max(n_small, nZ // 20)], size=n_small, replace=False)
-    rest = np.setdiff1d(np.arange(nZ), small)
-    uni = rng.choice(rest, size=count - n_small, replace=False)
-    return np.sort(np.concatenate([small, uni]))
+    lowest = order[: min(16, count)]
+    n_small = max(0, count // 2 - len(lowest))
+    pool = np.setdiff1d(order[: max(n_small + len(lowest), nZ // 20)], lowest)
+    small = rng.choice(pool, size=min(n_small, len(pool)), replace=False)
+    rest = np.setdiff1d(np.arange(nZ), np.concatenate([lowest, small]))
+    uni = rng.choice(rest, size=max(0, count - len(lowest) - len(small)), replace=False)
+    return np.sort(np.concatenate([lowest, small, uni]))
 
 
 def host_info():
@@ -589,6 +651,14 @@ def bench_prec(args):
     log(f"skewlap3d n={n} nnz={A.nnz} normC={normC:.6g}; ordering + device setup {t_pre:.1f}s")
     for w in range(args.warmup):
         RP.psPrecRecy(A, rec, ctl, perm=perm, solver=solver)
+    tuned = None
+    if not args.no_autotune and args.warmup > 0:
+        # chunk widths of the IC solves by measurement on the factor the warm-up left
+        from rls_amd import autotune as AT
+        from rls_amd import _capi
+        tuned = AT.tune_sparse_chunks(solver.ctx, n, _capi.RLS_OP_KINV, use_cache=False)
+        log(f"autotune IC-solve chunk widths: k<=8 -> {tuned['small']}, k>8 -> {tuned['large']} "
+            f"(k=6 ms {tuned['ms_k6']}, k=40 ms {tuned['ms_k40']}, identical outputs {tuned['identical_outputs']})")
     solver.enable_kernel_timing(True)
     times = []
     sig = st = None
@@ -635,7 +705,7 @@ def bench_prec(args):
                                f"rsize=500, first {npts} of the 200x200 traversal",
                    "grid_points": npts},
         "max_rel_err": float(rel.max()), "rel_err_samples": int(len(idx)),
-        "main_refines": int(st.n_pts), "pred_refines": int(st.n_pred_pts),
+        "main_refines": int(st.n_pts), "pred_refines": int(st.n_pred_pts), "autotune": tuned,
         "preconditioner_builds": int(st.n_precond_rebuild_pts),
         "stage_seconds": {"precond": st.precond_time_ns / 1e9, "main_recycle": st.main_recycle_time_ns / 1e9,
                           "main_refine": st.main_refine_time_ns / 1e9, "main_update": st.main_update_time_ns / 1e9,

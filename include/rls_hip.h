@@ -129,6 +129,18 @@ int rls_set_hess_dense(rls_ctx* ctx, const double* H, int64_t n, int64_t ld);
  * zeros below the subdiagonal (rls_set_dense / rls_set_hess_dense). */
 int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld);
 
+/* Launch-shape parameters of a context that leave every result bitwise unchanged, chosen by
+ * measurement (the Python tuner rls_amd.autotune) or by hand.  No counterpart in the
+ * reference (its operators are host library calls).  Values outside the allowed set are
+ * rejected with RLS_E_ARG. */
+#define RLS_TUNE_SPTRSV_CW_SMALL 0   /* right-hand-side columns per workgroup of a sparse
+                                        triangular solve with k <= 8: 1, 2, 4 or 8 (default 2) */
+#define RLS_TUNE_SPTRSV_CW_LARGE 1   /* the same for k > 8: 1, 2, 4 or 8 (default 8)          */
+#define RLS_TUNE_PREC_WINDOW 2       /* max shifts per speculative window of rls_prec_solve's
+                                        main stage: 16 .. 1024 (default 256)                 */
+int rls_set_tuning(rls_ctx* ctx, int key, int value);
+int rls_get_tuning(rls_ctx* ctx, int key, int* value);
+
 /* Sparse operator for the sparse psTriRecy overload (psTriRecy.jl:47-61, M = C, S = I here):
  * C in 0-based CSR (complex values).  Julia's 1-based SparseMatrixCSC of C is, read as CSR,
  * C^T; the shim transposes it once.  The refine factor of each refine shift z comes from the

@@ -25,6 +25,8 @@ EXPORTED = [
     "rls_set_dense",
     "rls_set_hess_dense",
     "rls_get_operator",
+    "rls_set_tuning",
+    "rls_get_tuning",
     "rls_tri_solve",
     "rls_tri_state_get",
     "rls_kernel_timing_enable",
@@ -56,6 +58,7 @@ EXPORTED = [
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
 RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM, RLS_KF_FACTOR = 0, 1, 2, 3
+RLS_TUNE_SPTRSV_CW_SMALL, RLS_TUNE_SPTRSV_CW_LARGE, RLS_TUNE_PREC_WINDOW = 0, 1, 2
 RLS_NOALLOC_GRAM_MAX_ELEMS = 1048576
 RLS_ORTH_LAPACK, RLS_ORTH_SHIFTCHOL = 0, 1
 
@@ -207,6 +210,10 @@ def load() -> C.CDLL:
     lib.rls_set_hess_dense.restype = C.c_int
     lib.rls_get_operator.argtypes = [vp, vp, C.c_int64]
     lib.rls_get_operator.restype = C.c_int
+    lib.rls_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.rls_set_tuning.restype = C.c_int
+    lib.rls_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    lib.rls_get_tuning.restype = C.c_int
     lib.rls_tri_solve.argtypes = [vp, vp, C.c_int64, C.POINTER(TriControl), vp, C.POINTER(TriStats)]
     lib.rls_tri_solve.restype = C.c_int
     lib.rls_kernel_timing_enable.argtypes = [vp, C.c_int]
@@ -273,6 +280,15 @@ class Context:
     def check(self, rc: int):
         if rc != RLS_OK:
             raise RlsError(rc, self.lib.rls_last_error(self.h).decode())
+
+    def set_tuning(self, key: int, value: int):
+        """rls_set_tuning: a launch-shape parameter that leaves results unchanged."""
+        self.check(self.lib.rls_set_tuning(self.h, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        v = C.c_int()
+        self.check(self.lib.rls_get_tuning(self.h, int(key), C.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -77,6 +77,9 @@ struct rls_ctx {
     DArr<zd> T, TT;   // dense upper-triangular operator (column-major, ld n) and its upper
                       // triangle in 64 x 64 tiles for the shifted solves
     bool sparse = false;
+    // launch shapes that leave results unchanged (rls_set_tuning)
+    int tune_cw_small = 2, tune_cw_large = 8, tune_prec_window = 256;
+    int sp_cw(int k) const { return k <= 8 ? tune_cw_small : tune_cw_large; }
     // dense operator in upper Hessenberg form (T holds H, zero below the subdiagonal): the
     // refine factor of a shift is G (H - zI) = U with U in TT and the elimination steps in
     // hlu_l / hlu_piv (rls_hlu.hip)
@@ -474,7 +477,7 @@ struct rls_ctx {
     // X <- F^{-1} X for a row-major block X (n x k, element (j, c) at j k + c)
     void sp_solve(const SpTri& F, bool lower, bool unit, zd* X, int k) {
         const int nb = (int)cdiv(n, sptrsv_block_rows());
-        const int cw = sptrsv_chunk_width(k);
+        const int cw = sp_cw(k);
         const size_t nflags = (size_t)nb * (size_t)((k + cw - 1) / cw);   // one per (row block, chunk)
         if (!sp_cnt.p || sp_cnt.n < nflags + 1) {
             check_tri_err();   // a pending wait error is reported, not cleared by the re-allocation
@@ -486,7 +489,7 @@ struct rls_ctx {
             sp_tbase = 0;
         }
         kt_begin(RLS_KF_TRSM);
-        sp_tbase += launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, sp_cnt.p, sp_tbase,
+        sp_tbase += launch_sptrsv(st, n, F.rp.p, F.ci.p, F.v.p, lower, unit, F.dep.p, X, k, 1, k, cw, sp_cnt.p, sp_tbase,
                                   sp_cnt.p + 1, ++sp_epoch, sp_err.p);
         // algorithmic: the factor once per chunk of columns (values, column indices, row
         // pointers) + the block read and written

@@ -517,6 +517,42 @@ int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld) {
     });
 }
 
+int rls_set_tuning(rls_ctx* ctx, int key, int value) {
+    return guarded(ctx, [&]() -> int {
+        const bool pow2_le8 = value == 1 || value == 2 || value == 4 || value == 8;
+        switch (key) {
+        case RLS_TUNE_SPTRSV_CW_SMALL:
+        case RLS_TUNE_SPTRSV_CW_LARGE:
+            if (!pow2_le8) throw RlsError{RLS_E_ARG, "rls_set_tuning: chunk width must be 1, 2, 4 or 8"};
+            ctx->sync();   // solves in flight keep the width they were launched with
+            (key == RLS_TUNE_SPTRSV_CW_SMALL ? ctx->tune_cw_small : ctx->tune_cw_large) = value;
+            // the flags array is sized by the chunk count: a fresh ticket counter and flags
+            ctx->check_tri_err();
+            ctx->sp_cnt.release();
+            ctx->sp_err.release();
+            return RLS_OK;
+        case RLS_TUNE_PREC_WINDOW:
+            if (value < 16 || value > 1024) throw RlsError{RLS_E_ARG, "rls_set_tuning: window must be in 16 .. 1024"};
+            ctx->tune_prec_window = value;
+            return RLS_OK;
+        default:
+            throw RlsError{RLS_E_ARG, "rls_set_tuning: unknown key"};
+        }
+    });
+}
+
+int rls_get_tuning(rls_ctx* ctx, int key, int* value) {
+    return guarded(ctx, [&]() -> int {
+        if (!value) throw RlsError{RLS_E_ARG, "rls_get_tuning: null value"};
+        switch (key) {
+        case RLS_TUNE_SPTRSV_CW_SMALL: *value = ctx->tune_cw_small; return RLS_OK;
+        case RLS_TUNE_SPTRSV_CW_LARGE: *value = ctx->tune_cw_large; return RLS_OK;
+        case RLS_TUNE_PREC_WINDOW: *value = ctx->tune_prec_window; return RLS_OK;
+        default: throw RlsError{RLS_E_ARG, "rls_get_tuning: unknown key"};
+        }
+    });
+}
+
 int rls_tri_solve(rls_ctx* ctx, const double* Z, int64_t nZ, const rls_tri_control* control,
                   double* sigma_out, rls_tri_stats* stats) {
     return guarded(ctx, [&]() -> int {

@@ -160,8 +160,8 @@ int sptrsv_block_rows();
 // One workgroup per (row block, chunk of w = sptrsv_chunk_width(k) columns); flags holds
 // nblk * ceil(k / w) words (block b, chunk c at b * ceil(k / w) + c), the launch's tickets are
 // tbase .. tbase + nblk * ceil(k / w) - 1; returns that ticket count.
-int sptrsv_chunk_width(int k);
+int sptrsv_chunk_width(int k);   // the default chunk width
 unsigned int launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v,
-                           bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k,
+                           bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k, int cw,
                            unsigned int* counter, unsigned int tbase, unsigned int* flags, unsigned int epoch,
                            int* err);

@@ -1016,7 +1016,7 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
         c->pmax_min = ctl.p_min_main;
     }
     auto main_p = [&]() { return engine_main ? c->p : mainS.p; };
-    const int window_max = 256;
+    const int window_max = c->tune_prec_window;
     double avg_run = 32.0;
     int run_hint = 32;
     int64_t i = 0;

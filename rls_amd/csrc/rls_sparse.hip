@@ -332,12 +332,12 @@ int sptrsv_block_rows() { return SP_RB; }
 int sptrsv_chunk_width(int k) { return k <= 8 ? 2 : 8; }
 
 unsigned int launch_sptrsv(hipStream_t st, int64_t n, const int64_t* rp, const int64_t* ci, const zd* v,
-                           bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k,
+                           bool lower, bool unit_diag, const int* dep, zd* X, int64_t rs, int64_t cs, int k, int cw,
                            unsigned int* counter, unsigned int tbase, unsigned int* flags, unsigned int epoch,
                            int* err) {
     if (k <= 0) return 0;
     const int nblk = (int)cdiv(n, SP_RB);
-    const int cw = sptrsv_chunk_width(k);
+    if (cw < 1 || cw > 8) cw = sptrsv_chunk_width(k);
     const int nch = (k + cw - 1) / cw;
     const unsigned int nt = (unsigned int)nblk * (unsigned int)nch;
     if (lower)

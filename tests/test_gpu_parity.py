@@ -271,3 +271,19 @@ def test_state_export_continues_in_the_oracle():
     assert np.max(np.abs(s2 - osig) / osig) <= 1e-4
     mism = int(((st2.lobpcg_it > 0) != ost.refined).sum())
     assert mism <= max(2, len(s2) // 50)
+
+
+def test_window_tuner_keeps_the_trajectory():
+    """The window-cap tuner (rls_amd.autotune.tune_window): every candidate cap gives the
+    same refine decisions and sigma (to 1e-10), and the chosen cap is one of them."""
+    from rls_amd import autotune as AT
+    n = 300
+    C = WL.laser(n)
+    print(f"[gpu-case] window tuner laser n={n}", flush=True)
+    s = R.TriSolver.from_dense(C)
+    rec = R.recursion_grid(-1.1, 1.2, -1.1, 1.1, 40, 40, 20)
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=1.0, p=30, recy_tau=1.6)
+    res = AT.tune_window(s, rec.Z, ctl, rec, candidates=(16, 64, 256), use_cache=False)
+    assert res["window"] in (16, 64, 256)
+    assert res["decisions_differ"] == 0
+    assert res["max_rel_diff"] <= 1e-10

@@ -213,3 +213,41 @@ def test_prec_solve_errors_through_the_c_abi():
     Ad = A.toarray()
     tr = np.array([np.linalg.svd(Ad - z * np.eye(n), compute_uv=False)[-1] for z in Z])
     assert np.max(np.abs(sig - tr) / tr) < 1e-3
+
+
+def test_sparse_solve_chunk_widths_are_bitwise_identical_and_tunable():
+    """The IC solves through the C ABI give bitwise the same result for every chunk width
+    (rls_set_tuning), the tuner picks one of the candidates and applies it, and bad tuning
+    values are rejected."""
+    from rls_amd import autotune as AT
+    A = WL.skewlap3d(10)
+    n = A.shape[0]
+    p = RP.column_ordering(A)
+    print(f"[prec tuning] n={n}", flush=True)
+    f = RP.ic_build(RP.normal_lower(A, -40.0 + 6.0j), p, 1e-3, 1e-6, 1e-4, 20, 40)
+    ps = RP.PrecSolver(A)
+    ps.set_ic(f)
+    ctx = ps.ctx
+    import torch
+    rng = np.random.default_rng(3)
+    outs = {}
+    with torch.cuda.stream(ps.stream):
+        for k in (3, 6, 11):
+            Xd = ps.to_dev(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))
+            for cw in (1, 2, 4, 8):
+                ctx.set_tuning(_capi.RLS_TUNE_SPTRSV_CW_SMALL, cw)
+                ctx.set_tuning(_capi.RLS_TUNE_SPTRSV_CW_LARGE, cw)
+                assert ctx.get_tuning(_capi.RLS_TUNE_SPTRSV_CW_SMALL) == cw
+                outs[(k, cw)] = ps.apply(_capi.RLS_OP_KINV, Xd).cpu().numpy()
+                ps.sync()
+            for cw in (2, 4, 8):
+                np.testing.assert_array_equal(outs[(k, cw)], outs[(k, 1)])
+    res = AT.tune_sparse_chunks(ctx, n, _capi.RLS_OP_KINV, ks=(6, 12), reps=2, use_cache=False)
+    assert res["identical_outputs"]
+    assert res["small"] in AT.CHUNK_CANDIDATES and res["large"] in AT.CHUNK_CANDIDATES
+    assert ctx.get_tuning(_capi.RLS_TUNE_SPTRSV_CW_SMALL) == res["small"]
+    for key, bad in ((_capi.RLS_TUNE_SPTRSV_CW_SMALL, 3), (_capi.RLS_TUNE_PREC_WINDOW, 8), (99, 1)):
+        with pytest.raises(_capi.RlsError):
+            ctx.set_tuning(key, bad)
+    ctx.set_tuning(_capi.RLS_TUNE_PREC_WINDOW, 64)
+    assert ctx.get_tuning(_capi.RLS_TUNE_PREC_WINDOW) == 64
