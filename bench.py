@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--acceptance-checks", type=int, default=0, metavar="K",
                     help="for up to K sampled points whose sigma differs from sigma_min by more than "
                          "1e-4, continue the oracle from the GPU state a few points before and report "
-                         "its sigma there (dense n <= 2000)")
+                         "its sigma there")
     ap.add_argument("--adjoint", action="store_true",
                     help="keep the certified singular vectors and time d sigma/dz (and the "
                          "Toeplitz-diagonal gradient for --workload toeplitz) after the steps")
@@ -373,7 +373,7 @@ def main():
         + (f", to nearest singular value {rel_any.max():.2e}" if rel_any is not None else ""))
 
     acc_checks = None
-    if args.acceptance_checks > 0 and rel_any is not None:
+    if args.acceptance_checks > 0:
         acc_checks = acceptance_checks(args, solver, ctl, sig, rec, oproblem, n, idx, rel, tr,
                                        (x_st, x_ed, y_st, y_ed, nx, ny, k), normC)
     cpu = None

@@ -129,7 +129,9 @@ struct rls_ctx {
     DArr<zd> Hz, heevW, Gev, C1, CzX, Gc, V2, CW, CG, Wr, Xr, zdev;
     DArr<double> lamev, sigr, tausd, lamev2;
     DArr<int> rr, coff, rr2;
-    DArr<zd> heevW2;
+    DArr<zd> heevW2, Yk, Bk, Gk, V3;
+    DArr<double> sig3;
+    DArr<int> coffk_d, rrk_d;
     // last X (Ritz vectors) of the final state, for callers
     std::vector<cplx> lastX;
     // state carried from one solve to the next when the control asks to resume
@@ -217,6 +219,10 @@ struct rls_ctx {
             for (auto e : k.ev) (void)hipEventDestroy(e);
         heevW2.release();
         rr2.release();
+        for (auto* a : {&Yk, &Bk, &Gk, &V3}) a->release();
+        sig3.release();
+        coffk_d.release();
+        rrk_d.release();
         for (auto* a : ds) a->release();
         info.release();
         rr.release();
@@ -756,8 +762,9 @@ struct rls_ctx {
                         Gc.p + (int64_t)b * rmax * rmax, rmax);
         }
         // SVD of the r_ x r_ factor: the smallest r singular triplets of CzX_ (rls_svd!,
-        // ascending).  Small r_: Cholesky + one-sided Jacobi; large r_: Hermitian eigensolve of
-        // the padded Gram.
+        // ascending).  Small r_: Cholesky + one-sided Jacobi; large r_: the eigenvectors of the
+        // kk smallest eigenvalues of the padded Gram, then Cholesky + Jacobi on CzX_ times them
+        // (k_recy_refine_block).
         V2.ensure((size_t)B * rmax * rmax);
         sigr.ensure((size_t)B * rmax);
         int64_t ldv2 = rmax, sv2 = (int64_t)rmax * rmax, ssig = rmax;
@@ -766,13 +773,41 @@ struct rls_ctx {
             launch_gram_svd_asc(st, Gc.p, (int64_t)rmax * rmax, rr.p, rmax, shift_scale, V2.p,
                                 (int64_t)rmax * rmax, sigr.p, rmax, B);
         } else {
+            const int kk = std::min(rmax, std::max(r, std::min(2 * r, 32)));
             launch_pad_gram(st, Gc.p, rmax, rr.p, B);
-            heevW2.ensure((size_t)B * heev_workspace_elems(rmax, r));
+            heevW2.ensure((size_t)B * heev_workspace_elems(rmax, kk));
             lamev2.ensure((size_t)B * rmax);
             rr2.ensure(B);
-            launch_heev_rr(st, Gc.p, rmax, (int64_t)rmax * rmax, rmax, B, heevW2.p, lamev2.p, rmax, V2.p,
-                           rmax, (int64_t)rmax * r, rr2.p, tausd.p, r, r, r);
-            launch_lam_to_sig(st, lamev2.p, rmax, sigr.p, rmax, r, B);
+            Yk.ensure((size_t)B * rmax * kk);
+            launch_heev_rr(st, Gc.p, rmax, (int64_t)rmax * rmax, rmax, B, heevW2.p, lamev2.p, rmax, Yk.p,
+                           rmax, (int64_t)rmax * kk, rr2.p, tausd.p, kk, kk, kk);
+            std::vector<int> coffk(B), rrk(B);
+            for (int b = 0; b < B; ++b) {
+                coffk[b] = b * kk;
+                rrk[b] = std::min(rrh[b], kk);
+            }
+            coffk_d.ensure(B);
+            rrk_d.ensure(B);
+            h2d(coffk_d.p, coffk.data(), B);
+            h2d(rrk_d.p, rrk.data(), B);
+            Bk.ensure((size_t)n * B * kk);
+            launch_recy_refine_block(st, n, CzX.p, n, coff.p, rr.p, Yk.p, rmax, (int64_t)rmax * kk, kk, Bk.p, n, B);
+            Gk.ensure((size_t)B * kk * kk);
+            if (kk <= 32) {
+                part.ensure(gram_self_batched_elems(n, B));
+                launch_gram_self_batched(st, n, Bk.p, n, coffk_d.p, rrk_d.p, B, Gk.p, kk, (int64_t)kk * kk, part.p);
+            } else {
+                for (int b = 0; b < B; ++b)
+                    gram_op(Bk.p + (int64_t)n * coffk[b], n, rrk[b], Bk.p + (int64_t)n * coffk[b], n, rrk[b],
+                            Gk.p + (int64_t)b * kk * kk, kk);
+            }
+            V3.ensure((size_t)B * kk * kk);
+            sig3.ensure((size_t)B * kk);
+            const double shift_k = 11.0 * ((double)n * kk + (double)kk * (kk + 1));
+            launch_gram_svd_asc(st, Gk.p, (int64_t)kk * kk, rrk_d.p, kk, shift_k, V3.p, (int64_t)kk * kk, sig3.p,
+                                kk, B);
+            launch_recy_refine_back(st, Yk.p, rmax, (int64_t)rmax * kk, rr.p, kk, V3.p, kk, (int64_t)kk * kk,
+                                    sig3.p, kk, r, rmax, V2.p, rmax, (int64_t)rmax * r, sigr.p, rmax, B);
             sv2 = (int64_t)rmax * r;
         }
         CW.ensure((size_t)4 * pr * B * r);

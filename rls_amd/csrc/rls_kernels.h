@@ -112,8 +112,12 @@ size_t gram_self_batched_elems(int64_t n, int B);
 void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t lda, const int* coff,
                               const int* kk, int B, zd* C, int64_t ldc, int64_t sC, zd* part);
 void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B);
-void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
-                       int B);
+void launch_recy_refine_block(hipStream_t st, int64_t n, const zd* CzX, int64_t ldc, const int* coff,
+                              const int* rr, const zd* Y, int64_t ldy, int64_t sY, int kk, zd* Bk, int64_t ldb,
+                              int B);
+void launch_recy_refine_back(hipStream_t st, const zd* Y, int64_t ldy, int64_t sY, const int* rr, int kk,
+                             const zd* V3, int64_t ldv3, int64_t sV3, const double* sig3, int64_t sS3, int r,
+                             int rows, zd* V2, int64_t ldv2, int64_t sV2, double* sig, int64_t sS, int B);
 // Upper triangle of T (n x n, ldt) in 64 x 64 tiles: tri_tiles_elems(n) complex values.
 size_t tri_tiles_elems(int64_t n);
 void launch_tile_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd* TT);

@@ -326,7 +326,10 @@ struct Driver {
     // Ascending SVD of a tall block A (n x k): the smallest `r` singular values into sig (device,
     // k entries) and the right singular vectors into Gs (k x k, ld k; first r columns valid).
     // k <= SMALL_SVD_MAX: Cholesky-QR passes (the shift-retry QR of RLSLinearAlgebra.jl:725-796)
-    // and a Jacobi SVD of R (rls_svd!, 334-418); larger k: the Hermitian eigensolver on A^H A.
+    // and a Jacobi SVD of R (rls_svd!, 334-418).  Larger k: the eigenvectors Y of the kk
+    // smallest eigenvalues of A^H A, then the same Cholesky-QR + Jacobi route on A Y (the
+    // Gram's eigenvalues themselves are only accurate to eps ||A||^2, far above sigma^2 next
+    // to an eigenvalue of the operator).
     void svd_asc(const zd* A, int k, int r) {
         sigd.ensure(std::max(k, 1));
         Gs.ensure((size_t)k * k);
@@ -338,16 +341,25 @@ struct Driver {
             infod.ensure(1);
             launch_svdrr_from_r(st, c->Racc.p, k, k, false, Gs.p, sigd.p, lamd.p, infod.p);
         } else {
+            const int kk = std::min(k, std::max(r, std::min(2 * r, SMALL_SVD_MAX)));
+            if (kk > SMALL_SVD_MAX) throw RlsError{RLS_E_ARG, "svd_asc: block size r above 36 is not supported"};
             sqA.ensure((size_t)k * k);
             gram(A, k, A, k, sqA.p, k);
-            heevW.ensure(heev_workspace_elems(k, r));
+            heevW.ensure(heev_workspace_elems(k, kk));
             lamd.ensure(k);
             rrd.ensure(1);
             taud.ensure(1);
             const double tau = 1.0;
             c->h2d(taud.p, &tau, 1);
-            launch_heev_rr(st, sqA.p, k, 0, k, 1, heevW.p, lamd.p, k, Gs.p, k, 0, rrd.p, taud.p, r, r, r);
-            hipLaunchKernelGGL(k_sqrt_clamp, dim3(1), dim3(256), 0, st, lamd.p, sigd.p, r);
+            small2.ensure((size_t)k * kk);
+            launch_heev_rr(st, sqA.p, k, 0, k, 1, heevW.p, lamd.p, k, small2.p, k, 0, rrd.p, taud.p, kk, kk, kk);
+            Tq.ensure((size_t)n * kk);
+            mm({{A, k}}, small2.p, k, kk, Tq.p, n);
+            c->tall_qr(Tq.p, n, kk, true);
+            Gs2.ensure((size_t)kk * kk);
+            infod.ensure(1);
+            launch_svdrr_from_r(st, c->Racc.p, kk, kk, false, Gs2.p, sigd.p, lamd.p, infod.p);
+            launch_small_gemm(st, k, r, kk, small2.p, k, Gs2.p, kk, Gs.p, k, 0.0);
         }
     }
 

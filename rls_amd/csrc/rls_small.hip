@@ -1597,9 +1597,11 @@ __global__ void k_pad_gram(zd* __restrict__ Gc, int kmax, const int* __restrict_
     zd* G = Gc + (int64_t)b * kmax * kmax;
     __shared__ double big;
     if (threadIdx.x == 0) {
+        // above every eigenvalue of the leading k x k block (<= its trace) but on its scale:
+        // the eigensolver's absolute accuracy follows the padded matrix's norm
         double tr = 0.0;
         for (int j = 0; j < k; ++j) tr += fabs(G[j + kmax * j].x);
-        big = 1e3 * (tr + 1.0);
+        big = tr > 0.0 ? 2.0 * tr : 1.0;
     }
     __syncthreads();
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < kmax * kmax; idx += gridDim.x * blockDim.x) {
@@ -1614,16 +1616,77 @@ void launch_pad_gram(hipStream_t st, zd* Gc, int kmax, const int* kk, int B) {
     LAUNCH_CHECK();
 }
 
-// sig = sqrt(max(lam, 0)) for the first r entries of each problem
-__global__ void k_lam_to_sig(const double* __restrict__ lam, int64_t sL, double* __restrict__ sig,
-                             int64_t sS, int r) {
-    const int b = blockIdx.x, j = threadIdx.x;
-    if (j < r) sig[(int64_t)b * sS + j] = sqrt(fmax(lam[(int64_t)b * sL + j], 0.0));
+// Refined Ritz block of the recycled Rayleigh-Ritz for Ritz dimensions above the LDS SVD.
+// The square roots of the eigenvalues of Gram_b = CzX_b^H CzX_b carry an absolute error of
+// about eps ||CzX_b||^2, which swamps sigma^2 next to an eigenvalue of C; the Gram's
+// eigenvectors of its kk smallest eigenvalues (Y_b) still span the wanted singular directions
+// to that accuracy.  Bk_b = CzX_b Y_b (n x kb, kb = min(rr_b, kk)) has nearly orthogonal
+// columns, so the Cholesky-QR + Jacobi SVD of Bk_b (k_gram_svd_asc on its Gram) gives the
+// smallest singular values to the accuracy of a QR of CzX_b (rls_svd! on the R of rls_qr!,
+// fastRaylRitz.jl:190-193).  One thread per row, eight output columns per pass.
+__global__ void __launch_bounds__(256)
+k_recy_refine_block(int64_t n, const zd* __restrict__ CzX, int64_t ldc, const int* __restrict__ coff,
+                    const int* __restrict__ rr, const zd* __restrict__ Y, int64_t ldy, int64_t sY, int kk,
+                    zd* __restrict__ Bk, int64_t ldb) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = rr[b];
+    const int kb = k < kk ? k : kk;
+    const zd* A = CzX + ldc * coff[b];
+    const zd* Yb = Y + sY * b;
+    zd* Ob = Bk + ldb * ((int64_t)b * kk);
+    for (int c0 = 0; c0 < kb; c0 += 8) {
+        zd acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = zzero();
+        for (int j = 0; j < k; ++j) {
+            const zd a = A[i + ldc * j];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c0 + c < kb) acc[c] = zfma(a, Yb[j + ldy * (c0 + c)], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c0 + c < kb) Ob[i + ldb * (c0 + c)] = acc[c];
+    }
 }
 
-void launch_lam_to_sig(hipStream_t st, const double* lam, int64_t sL, double* sig, int64_t sS, int r,
-                       int B) {
-    hipLaunchKernelGGL(k_lam_to_sig, dim3(B), dim3(64), 0, st, lam, sL, sig, sS, r);
+// Back to CzX_b's coordinates: V2_b[:, c] = Y_b V3_b[:, c] for rows < rr_b (zero for the
+// padding rows up to `rows`) and sig_b[c] = sig3_b[c], c < r.
+__global__ void k_recy_refine_back(const zd* __restrict__ Y, int64_t ldy, int64_t sY, const int* __restrict__ rr,
+                                   int kk, const zd* __restrict__ V3, int64_t ldv3, int64_t sV3,
+                                   const double* __restrict__ sig3, int64_t sS3, int r, int rows,
+                                   zd* __restrict__ V2, int64_t ldv2, int64_t sV2, double* __restrict__ sig,
+                                   int64_t sS) {
+    const int b = blockIdx.x;
+    const int k = rr[b];
+    const int kb = k < kk ? k : kk;
+    const zd* Yb = Y + sY * b;
+    const zd* Vb = V3 + sV3 * b;
+    for (int idx = threadIdx.x; idx < rows * r; idx += blockDim.x) {
+        const int j = idx % rows, c = idx / rows;
+        zd g = zzero();
+        if (j < k)
+            for (int q = 0; q < kb; ++q) g = zfma(Yb[j + ldy * q], Vb[q + ldv3 * c], g);
+        V2[sV2 * b + j + ldv2 * c] = g;
+    }
+    if ((int)threadIdx.x < r) sig[sS * b + threadIdx.x] = sig3[sS3 * b + threadIdx.x];
+}
+
+void launch_recy_refine_block(hipStream_t st, int64_t n, const zd* CzX, int64_t ldc, const int* coff,
+                              const int* rr, const zd* Y, int64_t ldy, int64_t sY, int kk, zd* Bk, int64_t ldb,
+                              int B) {
+    hipLaunchKernelGGL(k_recy_refine_block, dim3(cdiv(n, 256), B), dim3(256), 0, st, n, CzX, ldc, coff, rr, Y,
+                       ldy, sY, kk, Bk, ldb);
+    LAUNCH_CHECK();
+}
+
+void launch_recy_refine_back(hipStream_t st, const zd* Y, int64_t ldy, int64_t sY, const int* rr, int kk,
+                             const zd* V3, int64_t ldv3, int64_t sV3, const double* sig3, int64_t sS3, int r,
+                             int rows, zd* V2, int64_t ldv2, int64_t sV2, double* sig, int64_t sS, int B) {
+    hipLaunchKernelGGL(k_recy_refine_back, dim3(B), dim3(256), 0, st, Y, ldy, sY, rr, kk, V3, ldv3, sV3, sig3,
+                       sS3, r, rows, V2, ldv2, sV2, sig, sS);
     LAUNCH_CHECK();
 }
 

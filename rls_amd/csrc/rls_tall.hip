@@ -552,8 +552,9 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 // solve off the dependency chain.  The off-diagonal blocks come from TT, the upper triangle
 // of T in 64 x 64 tiles (tile (bi, bj), bi <= bj, at tri_tile_index(bi, bj) * 4096, each
 // tile column-major and zero-padded past n): every block load is one contiguous 64 KB
-// tile, and both directions read the same tiles.  Block j publishes its solution by storing `epoch` into flag[j] after a device
-// fence; readers spin (bounded) on an agent-scope atomic load, then fence.  The counter is
+// tile, and both directions read the same tiles.  Block j writes its solution through the L2
+// (agent-scope atomic stores) and publishes it by a release store of `epoch` into flag[j];
+// readers spin (bounded) on an agent-scope atomic load, then take an acquire fence.  The counter is
 // never reset: tickets of solve e are [e*nblk, (e+1)*nblk) because launches on one stream
 // run in order.  A spin that exceeds its bound sets *err and the kernel still terminates.
 // ---------------------------------------------------------------------------------------

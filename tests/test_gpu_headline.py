@@ -214,3 +214,39 @@ def test_banded_cfd_shaped_sparse_path():
     assert orel.max() <= 1e-6
     assert int(((st.lobpcg_it > 0) != ost.refined).sum()) <= 1
     assert 0 < st.n_pts < len(sig)
+
+
+def test_laser_near_eigenvalue_sigma_accuracy():
+    """Laser n = 5000 on the benchmark's 200 x 200 grid, first 22100 points: next to an
+    eigenvalue of C, sigma_min falls to ~1e-10 while the gap rule's Ritz dimension passes 64
+    (the refined-Ritz-block route of the batched Rayleigh-Ritz).  The accepted sigma at the
+    points with the smallest sigma_min matches it to 1e-6 for two window caps; the square
+    roots of the Gram's eigenvalues were off by up to 20 % there (1e-5 at refined points,
+    where LOBPCG stops at its own residual tolerance)."""
+    from oracle import truth
+    C = WL.laser(5000)
+    _say("laser n=5000 near-eigenvalue points of the benchmark grid (first 22100 points)")
+    T = R.schur_preprocess(C, 0)
+    rec = R.recursion_grid(*WL.GRIDS["laser"])
+    rec.Z = rec.Z[:22100]
+    solver = R.TriSolver(T)
+    out = {}
+    for w in (64, 512):
+        ctl = R.PsTriRecyControl(tol=1e-4, normC=1.0, p=60, recy_tau=1.6, mode="batched", window=w)
+        sig, st = solver.solve(rec.Z, ctl, rec)
+        out[w] = (sig, st)
+        _say(f"window {w}: points with r_ > 64: {int((st.r_recy > 64).sum())}, max r_ {int(st.r_recy.max())}")
+    sig, st = out[64]
+    assert int((st.r_recy > 64).sum()) >= 1, "the slice must exercise Ritz dimensions above 64"
+    idx = np.unique(np.concatenate([np.argsort(sig)[:12], np.argsort(out[512][0])[:12], [22080, 22081]]))
+    tr = truth.tri_sigma_min(T, rec.Z[idx], block=16)
+    for w, (s, stw) in out.items():
+        rel = np.abs(s[idx] - tr) / tr
+        j = int(np.argmax(rel))
+        _say(f"window {w}: max rel err {rel.max():.2e} at point {int(idx[j])} (sigma_min {tr[j]:.3e}, "
+             f"r_ {int(stw.r_recy[idx[j]])}, refined {bool(stw.lobpcg_it[idx[j]] > 0)})")
+        # recycled points: the Rayleigh-Ritz sigma of a space that holds the singular vector to
+        # working accuracy; refined points: LOBPCG stops at a residual of 1e-5 sigma
+        refined = stw.lobpcg_it[idx] > 0
+        assert rel[~refined].max() <= 1e-6
+        assert rel.max() <= 1e-5
