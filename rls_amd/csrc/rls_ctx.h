@@ -231,6 +231,13 @@ struct rls_ctx {
     }
 
     void sync() { HIP_TRY(hipStreamSynchronize(st)); }
+    // 2-D copy between pageable host memory and the device: ordered after the stream's work
+    // and complete on return (a stream synchronisation, then the blocking hipMemcpy2D).
+    void copy2d_host(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t height,
+                     hipMemcpyKind kind) {
+        sync();
+        HIP_TRY(hipMemcpy2D(dst, dpitch, src, spitch, width, height, kind));
+    }
     template <typename T_>
     void d2h(T_* h, const T_* d, size_t cnt) {
         HIP_TRY(hipMemcpyAsync(h, d, cnt * sizeof(T_), hipMemcpyDeviceToHost, st));

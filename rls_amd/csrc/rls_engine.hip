@@ -447,8 +447,8 @@ int rls_set_tri_dense(rls_ctx* ctx, const double* T, int64_t n, int64_t ld) {
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
         dense_operator_reset(ctx, n, false);
-        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
-                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->copy2d_host(ctx->T.p, sizeof(zd) * n, T, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice);
         // the tiles of the shifted solves (contiguous 64 KB blocks for both directions)
         launch_tile_upper(ctx->st, n, ctx->T.p, n, ctx->TT.p);
         ctx->sync();
@@ -463,8 +463,8 @@ int rls_set_hess_dense(rls_ctx* ctx, const double* H, int64_t n, int64_t ld) {
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
         dense_operator_reset(ctx, n, true);
-        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, H, sizeof(zd) * ld, sizeof(zd) * n, n,
-                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->copy2d_host(ctx->T.p, sizeof(zd) * n, H, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice);
         launch_hess_mask(ctx->st, n, ctx->T.p, n);
         ctx->sync();
         return RLS_OK;
@@ -478,8 +478,8 @@ int rls_set_dense(rls_ctx* ctx, const double* A, int64_t n, int64_t lda, double*
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->sync();
         dense_operator_reset(ctx, n, true);
-        HIP_TRY(hipMemcpy2DAsync(ctx->T.p, sizeof(zd) * n, A, sizeof(zd) * lda, sizeof(zd) * n, n,
-                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->copy2d_host(ctx->T.p, sizeof(zd) * n, A, sizeof(zd) * lda, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice);
         // pre_process on the device: Householder reduction to Hessenberg form in place; the
         // reflectors (below the subdiagonal) are dropped, only H = Q^H A Q is kept
         if (n >= 3) {
@@ -510,8 +510,8 @@ int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld) {
         if (!M || ld < ctx->n) throw RlsError{RLS_E_ARG, "rls_get_operator: bad M / ld"};
         HIP_TRY(hipSetDevice(ctx->device));
         const int64_t n = ctx->n;
-        HIP_TRY(hipMemcpy2DAsync(M, sizeof(zd) * ld, ctx->T.p, sizeof(zd) * n, sizeof(zd) * n, n,
-                                 hipMemcpyDeviceToHost, ctx->st));
+        ctx->copy2d_host(M, sizeof(zd) * ld, ctx->T.p, sizeof(zd) * n, sizeof(zd) * n, n,
+                                 hipMemcpyDeviceToHost);
         ctx->sync();
         return RLS_OK;
     });
@@ -795,8 +795,8 @@ int rls_tri_state_get(rls_ctx* ctx, rls_tri_state* info, double* V, double* J1, 
         double* sq_out[3] = {L, H1, H2};
         for (int a = 0; a < 3; ++a) {
             if (!sq_out[a] || pr == 0) continue;
-            HIP_TRY(hipMemcpy2DAsync(sq_out[a], sizeof(zd) * pr, sq[a], sizeof(zd) * ctx->prm, sizeof(zd) * pr, pr,
-                                     hipMemcpyDeviceToHost, ctx->st));
+            ctx->copy2d_host(sq_out[a], sizeof(zd) * pr, sq[a], sizeof(zd) * ctx->prm, sizeof(zd) * pr, pr,
+                                     hipMemcpyDeviceToHost);
             ctx->sync();
         }
         if (pts)
@@ -827,8 +827,8 @@ int rls_set_schur_vectors(rls_ctx* ctx, const double* Q, int64_t ld) {
         if (!Q || ld < n) throw RlsError{RLS_E_ARG, "rls_set_schur_vectors: bad Q / ld"};
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->Qs.ensure((size_t)n * n);
-        HIP_TRY(hipMemcpy2DAsync(ctx->Qs.p, sizeof(zd) * n, Q, sizeof(zd) * ld, sizeof(zd) * n, n,
-                                 hipMemcpyHostToDevice, ctx->st));
+        ctx->copy2d_host(ctx->Qs.p, sizeof(zd) * n, Q, sizeof(zd) * ld, sizeof(zd) * n, n,
+                                 hipMemcpyHostToDevice);
         ctx->sync();
         ctx->have_q = true;
         return RLS_OK;

@@ -457,12 +457,12 @@ extern "C" int rls_hessenberg(rls_ctx* ctx, const double* A, int64_t n, int64_t 
             dA.ensure((size_t)n * n);
             dtau.ensure((size_t)n);
             dwork.ensure(hessenberg_work_elems(n));
-            HIP_TRY(hipMemcpy2DAsync(dA.p, sizeof(zd) * n, A, sizeof(zd) * lda, sizeof(zd) * n, n,
-                                     hipMemcpyHostToDevice, ctx->st));
+            ctx->copy2d_host(dA.p, sizeof(zd) * n, A, sizeof(zd) * lda, sizeof(zd) * n, n,
+                                     hipMemcpyHostToDevice);
             if (n >= 3) hessenberg_device(ctx->st, n, dA.p, dtau.p, dwork.p, dwork.n);
             else HIP_TRY(hipMemsetAsync(dtau.p, 0, sizeof(zd) * n, ctx->st));
-            HIP_TRY(hipMemcpy2DAsync(H, sizeof(zd) * ldh, dA.p, sizeof(zd) * n, sizeof(zd) * n, n,
-                                     hipMemcpyDeviceToHost, ctx->st));
+            ctx->copy2d_host(H, sizeof(zd) * ldh, dA.p, sizeof(zd) * n, sizeof(zd) * n, n,
+                                     hipMemcpyDeviceToHost);
             ctx->d2h(reinterpret_cast<zd*>(tau), dtau.p, (size_t)n);
             ctx->sync();
         } catch (...) {
