@@ -53,6 +53,9 @@ def build_workload(name: str):
     return C, normC, g, n
 
 
+TRUTH_SAMPLES_DEFAULT = 200
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,9 +84,10 @@ def main():
                     help="points of a mid-grid segment timed with both solvers (0 = skip)")
     ap.add_argument("--invlanc-points", type=int, default=40,
                     help="leading traversal points also solved by psInvLanc on the device (0 = skip)")
-    ap.add_argument("--truth-samples", type=int, default=200,
-                    help="grid points checked against FP64 ground truth: half uniformly at random, "
-                         "half among the smallest computed sigma")
+    ap.add_argument("--truth-samples", type=int, default=TRUTH_SAMPLES_DEFAULT,
+                    help="grid points checked against FP64 ground truth: the 16 smallest computed sigma, "
+                         "then half of the rest among the smallest 5 %% and half uniformly at random "
+                         "(skewlap3d: 24 random points unless given)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0,
                     help="CPU-restatement time, split over --cpu-segments segments of the traversal")
     ap.add_argument("--cpu-segments", type=int, default=4)
@@ -354,7 +358,11 @@ def main():
     t0 = time.perf_counter()
     rel_any = None
     if sp.issparse(C):
-        tr = truth.sparse_sigma_min(C, rec.Z[idx])
+        tr = np.empty(len(idx))
+        for j, i in enumerate(idx):   # a sparse factorization per point: one progress line each
+            t1 = time.perf_counter()
+            tr[j] = truth.sparse_sigma_min(C, rec.Z[i:i + 1])[0]
+            log(f"truth point {int(i)}: sigma_min {tr[j]:.6e} ({time.perf_counter() - t1:.1f}s)")
     elif n <= 2000:
         import scipy.linalg as sla
         Ca = np.asarray(C)
@@ -603,6 +611,7 @@ def cpu_baseline_prec(args, A, rec, ctl, perm, sig, st):
 
     def cb(i, z, what, s_):
         out[i] = (s_, "main(" in what)
+        log(f"cpu baseline point {i}: {what} ({time.perf_counter() - t_start:.1f}s)")
         if time.perf_counter() - t_start > args.cpu_seconds:
             raise Stop()
     try:
@@ -688,8 +697,14 @@ def bench_prec(args):
         roof["traffic"] = tr_["sptrsv_bytes_per_launch"]
         roof["traffic_GBps"] = roof["traffic"] / avg_s / 1e9
     from oracle import truth
-    idx = np.random.default_rng(1).choice(npts, size=min(args.truth_samples, npts), replace=False)
-    tr = truth.sparse_sigma_min(A, rec.Z[idx])
+    # a sparse factorization per sample (~1.6 s each at this size): 24 samples unless asked
+    n_truth = args.truth_samples if args.truth_samples != TRUTH_SAMPLES_DEFAULT else 24
+    idx = np.random.default_rng(1).choice(npts, size=min(n_truth, npts), replace=False)
+    tr = np.empty(len(idx))
+    for j, i in enumerate(idx):       # a sparse factorization per point: one progress line each
+        t0 = time.perf_counter()
+        tr[j] = truth.sparse_sigma_min(A, rec.Z[i:i + 1])[0]
+        log(f"truth point {int(i)}: sigma_min {tr[j]:.6e} ({time.perf_counter() - t0:.1f}s)")
     rel = np.abs(sig[idx] - tr) / tr
     log(f"truth on {len(idx)} points: max rel {rel.max():.2e}; trsm {d['ms']:.0f} ms / {d['launches']} launches")
     cpu = None
