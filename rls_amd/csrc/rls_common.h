@@ -90,6 +90,34 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
     b = sb;
 }
 
+// block_sum / block_sum2 without the barriers that protect `red` against an earlier or a
+// later use: the caller gives each reduction its own buffer and separates the uses of one
+// buffer by other barriers.  Same sums in the same order.
+__device__ __forceinline__ double block_sum_1b(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    return s;
+}
+__device__ __forceinline__ void block_sum2_1b(double& a, double& b, double* red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < nw; ++i) { sa += red[2 * i]; sb += red[2 * i + 1]; }
+    a = sa;
+    b = sb;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;

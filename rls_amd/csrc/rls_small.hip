@@ -751,7 +751,7 @@ void launch_heev_rr(hipStream_t st, const zd* A, int64_t lda, int64_t sA, int n,
         // wave-per-matrix LDS kernel; its workspace fits inside the general one
         int64_t sW = (int64_t)heev_workspace_elems(n, nvec_max);
         size_t lds = (size_t)n * (n + 1) / 2 * sizeof(zd);               // packed matrix
-        lds = std::max(lds, (size_t)2 * n * 64 * sizeof(double));       // Y and D chunks
+        lds = std::max(lds, (size_t)2 * (n | 1) * 64 * sizeof(double)); // Y and D chunks
         lds = std::max(lds, (size_t)(n + 1) * 64 * sizeof(zd));          // G chunk
         static bool attr_set = false;
         if (!attr_set) {
@@ -1776,6 +1776,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     const int wv = tid >> 6;
     __shared__ zd s_acc4[4][HW_NMAX];
     __shared__ double s_red[4];
+    __shared__ double s_red_n[4], s_red_w[8];        // the norm and w^H v sums of a step
     bool pending = false;
     for (int k = 0; k < n - 1; ++k) {
         const int m = n - k - 1;
@@ -1815,7 +1816,8 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
         __syncthreads();
         double part = 0.0;
         for (int i = 1 + tid; i < m; i += 256) part += zabs2(Lp[ok + 1 + i]);
-        const double xn2 = block_sum(part, s_red);
+        // s_red_n is next written in the next step, after this step's later barriers
+        const double xn2 = block_sum_1b(part, s_red_n);
         const zd alpha = Lp[ok + 1];
         zd t = zzero();
         double beta;
@@ -1869,8 +1871,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             const zd c = zcmul(wloc, sv[tid]);
             pr = c.x; pi = c.y;
         }
-        pr = block_sum(pr, s_red);
-        pi = block_sum(pi, s_red);
+        block_sum2_1b(pr, pi, s_red_w);   // next written in the next step, after its barriers
         const zd a2 = zscale(zmul(t, zmk(pr, pi)), -0.5);
         if (tid < m) { pw[tid] = zfma(a2, sv[tid], wloc); pv[tid] = sv[tid]; }
         pending = true;
@@ -1968,14 +1969,18 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
     const double ortol = 1e-9 * tnorm;
     const double floorv = eps * tnorm + pivmin;
     zd* Gb = G + (int64_t)b * sG;
-    double* Ys = reinterpret_cast<double*>(Lp);       // n x 64 (ld n)
-    double* Ds = Ys + (int64_t)n * 64;                // n x 64
+    // lane j's vector at Ys + j ldy: an odd leading dimension puts the 32 lanes of each half
+    // wave on distinct LDS banks in the sequential solves (ld = n, a multiple of 32 for most
+    // n, serialised them); 2 ldy 64 doubles fit in the (n + 1) 64 complex of the G chunk
+    const int ldy = n | 1;
+    double* Ys = reinterpret_cast<double*>(Lp);       // n x 64 (ld ldy)
+    double* Ds = Ys + (int64_t)ldy * 64;              // n x 64 (ld ldy)
     for (int c0 = 0; c0 < nv; c0 += 64) {
         const int nc = min(64, nv - c0);
         __syncthreads();
         const int j = lane;
-        double* y = Ys + (int64_t)j * n;
-        double* D = Ds + (int64_t)j * n;
+        double* y = Ys + (int64_t)j * ldy;
+        double* D = Ds + (int64_t)j * ldy;
         if (wave0 && j < nc) {
             const double lj = lam[c0 + j];
             // D_i of L D L^T for T - lj I
@@ -2009,9 +2014,9 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
                 int j1 = j0 + 1;
                 while (j1 < nc && lam[c0 + j1] - lam[c0 + j1 - 1] <= ortol) ++j1;
                 for (int jj = j0 + 1; jj < j1; ++jj) {
-                    double* yj = Ys + (int64_t)jj * n;
+                    double* yj = Ys + (int64_t)jj * ldy;
                     for (int qq = j0; qq < jj; ++qq) {
-                        const double* yq = Ys + (int64_t)qq * n;
+                        const double* yq = Ys + (int64_t)qq * ldy;
                         double sacc = 0.0;
                         for (int i = lane; i < n; i += 64) sacc += yj[i] * yq[i];
                         sacc = wave_sum(sacc);
@@ -2027,7 +2032,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
             __syncthreads();
         }
         // Y chunk -> global (coalesced), then G chunk in LDS (ld n+1)
-        for (int idx = tid0; idx < n * nc; idx += 256) Y[idx] = Ys[idx];
+        for (int idx = tid0; idx < n * nc; idx += 256) Y[idx] = Ys[(idx % n) + ldy * (idx / n)];
         __syncthreads();
         const int ldl = n + 1;
         zd* Gs = Lp;
@@ -2039,21 +2044,33 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
         {
             // lane = vector j; the four waves split the rows by absolute index (wave w owns
             // rows R = w mod 4, so no wave reads a row another wave updates); the wave
-            // partials of v^H g meet in LDS and are summed in a fixed order
+            // partials of v^H g meet in LDS and are summed in a fixed order.  Reflector k's
+            // vector is staged in LDS (one coalesced load by the workgroup) while reflector
+            // k + 1 is applied: three staging buffers, so a buffer is rewritten only after
+            // the barriers that follow its last reads.
             const int w = tid0 >> 6;
             zd* g = Gs + ldl * j;
+            zd* s_hv = &s_acc4[0][0];                    // 3 x HW_NMAX, idle after the reduction
+            auto stage = [&](int kk) {
+                if (kk < 0) return;
+                const zd* hv = Hh + poff(n, kk) + 1;     // v[0] = 1, v[1:] at hv[1..m)
+                zd* dst = s_hv + (kk % 3) * HW_NMAX;
+                for (int i = tid0; i < n - kk - 1; i += 256) dst[i] = (i == 0) ? zmk(1.0, 0.0) : hv[i];
+            };
+            stage(n - 2);
+            __syncthreads();
             for (int k = n - 2; k >= 0; --k) {
                 const zd tk = tau[k];
-                if (tk.x == 0.0 && tk.y == 0.0) continue;   // uniform over the workgroup
-                const zd* hv = Hh + poff(n, k) + 1;      // v[0] = 1, v[1:] at hv[1..m)
+                const zd* vk = s_hv + (k % 3) * HW_NMAX;
+                stage(k - 1);
+                if (tk.x == 0.0 && tk.y == 0.0) {        // uniform over the workgroup
+                    __syncthreads();
+                    continue;
+                }
                 const int R0 = k + 1 + ((w - (k + 1)) & 3);
                 zd part = zzero();
                 if (j < nc) {
-                    for (int R = R0; R < n; R += 4) {
-                        const int i = R - (k + 1);
-                        const zd vi = (i == 0) ? zmk(1.0, 0.0) : hv[i];
-                        part = zcfma(vi, g[R], part);
-                    }
+                    for (int R = R0; R < n; R += 4) part = zcfma(vk[R - (k + 1)], g[R], part);
                 }
                 s_bdot[w][lane] = part;
                 __syncthreads();
@@ -2061,11 +2078,7 @@ k_heev_wave(const zd* __restrict__ A, int64_t lda, int64_t sA, int n, zd* __rest
                 __syncthreads();                             // partials read before the next write
                 const zd f = zmul(tk, dot);
                 if (j < nc) {
-                    for (int R = R0; R < n; R += 4) {
-                        const int i = R - (k + 1);
-                        const zd vi = (i == 0) ? zmk(1.0, 0.0) : hv[i];
-                        g[R] = zsub(g[R], zmul(vi, f));
-                    }
+                    for (int R = R0; R < n; R += 4) g[R] = zsub(g[R], zmul(vk[R - (k + 1)], f));
                 }
             }
         }
