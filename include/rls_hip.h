@@ -138,6 +138,9 @@ int rls_get_operator(rls_ctx* ctx, double* M, int64_t ld);
 #define RLS_TUNE_SPTRSV_CW_LARGE 1   /* the same for k > 8: 1, 2, 4 or 8 (default 8)          */
 #define RLS_TUNE_PREC_WINDOW 2       /* max shifts per speculative window of rls_prec_solve's
                                         main stage: 16 .. 1024 (default 256)                 */
+#define RLS_TUNE_LOBPCG_GRAPH 3      /* 1: psTriRecy's LOBPCG iterations (dense operators)
+                                        replay a captured HIP graph per iteration; 0: eager
+                                        launches (default 1; results are bitwise the same)   */
 int rls_set_tuning(rls_ctx* ctx, int key, int value);
 int rls_get_tuning(rls_ctx* ctx, int key, int* value);
 
@@ -197,7 +200,10 @@ int rls_tri_state_get(rls_ctx* ctx, rls_tri_state* info, double* V, double* J1, 
 #define RLS_KF_HEEV 1     /* per-shift Hermitian eigensolve of the projected matrix      */
 #define RLS_KF_TRSM 2     /* shifted triangular solves: Schur factor, sparse LU or IC factor */
 #define RLS_KF_FACTOR 3   /* refine factors on the device: Hessenberg LU of H - zI per refine shift */
-#define RLS_KF_COUNT 4
+#define RLS_KF_ITER 4     /* LOBPCG iterations replayed from a captured graph (their two shifted
+                             solves, Cholesky-QR passes, small SVD and block products; the
+                             algorithmic figures are those of the two solves)                */
+#define RLS_KF_COUNT 5
 int rls_kernel_timing_enable(rls_ctx* ctx, int enable);
 int rls_kernel_timing(rls_ctx* ctx, int family, int64_t* launches, double* ms, double* flops,
                       double* bytes);

@@ -492,7 +492,8 @@ class TriSolver:
     def kernel_timing(self) -> dict:
         out = {}
         for name, fam in (("gemm", _capi.RLS_KF_GEMM), ("heev", _capi.RLS_KF_HEEV),
-                          ("trsm", _capi.RLS_KF_TRSM), ("factor", _capi.RLS_KF_FACTOR)):
+                          ("trsm", _capi.RLS_KF_TRSM), ("factor", _capi.RLS_KF_FACTOR),
+                          ("iter", _capi.RLS_KF_ITER)):
             L = C.c_int64()
             ms, fl, by = C.c_double(), C.c_double(), C.c_double()
             self.ctx.check(self.ctx.lib.rls_kernel_timing(self.ctx.h, fam, C.byref(L), C.byref(ms),

@@ -57,8 +57,8 @@ EXPORTED = [
 ]
 RLS_OP_SHIFTED, RLS_OP_SHIFTED_H, RLS_OP_INVF, RLS_OP_INVF_H, RLS_OP_KINV, RLS_OP_LINV, RLS_OP_LINV_H = range(7)
 RLS_OP_TRI_INV, RLS_OP_TRI_INV_H = 7, 8
-RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM, RLS_KF_FACTOR = 0, 1, 2, 3
-RLS_TUNE_SPTRSV_CW_SMALL, RLS_TUNE_SPTRSV_CW_LARGE, RLS_TUNE_PREC_WINDOW = 0, 1, 2
+RLS_KF_GEMM, RLS_KF_HEEV, RLS_KF_TRSM, RLS_KF_FACTOR, RLS_KF_ITER = 0, 1, 2, 3, 4
+RLS_TUNE_SPTRSV_CW_SMALL, RLS_TUNE_SPTRSV_CW_LARGE, RLS_TUNE_PREC_WINDOW, RLS_TUNE_LOBPCG_GRAPH = 0, 1, 2, 3
 RLS_NOALLOC_GRAM_MAX_ELEMS = 1048576
 RLS_ORTH_LAPACK, RLS_ORTH_SHIFTCHOL = 0, 1
 

@@ -78,7 +78,7 @@ struct rls_ctx {
                       // triangle in 64 x 64 tiles for the shifted solves
     bool sparse = false;
     // launch shapes that leave results unchanged (rls_set_tuning)
-    int tune_cw_small = 2, tune_cw_large = 8, tune_prec_window = 256;
+    int tune_cw_small = 2, tune_cw_large = 8, tune_prec_window = 256, tune_lobpcg_graph = 1;
     int sp_cw(int k) const { return k <= 8 ? tune_cw_small : tune_cw_large; }
     // dense operator in upper Hessenberg form (T holds H, zero below the subdiagonal): the
     // refine factor of a shift is G (H - zI) = U with U in TT and the elimination steps in
@@ -157,10 +157,11 @@ struct rls_ctx {
         double flops = 0.0, bytes = 0.0, ms = 0.0;
     };
     bool timing_on = false;
+    bool capturing = false;       // launches are being recorded into a graph (no events)
     KTimer ktm[RLS_KF_COUNT];
     int ktm_open = -1;
     void kt_begin(int fam) {
-        if (!timing_on) return;
+        if (!timing_on || capturing) return;
         KTimer& k = ktm[fam];
         if (k.used + 2 > k.ev.size()) {
             for (int i = 0; i < 64; ++i) {
@@ -173,7 +174,7 @@ struct rls_ctx {
         ktm_open = fam;
     }
     void kt_end(int fam, double flops, double bytes) {
-        if (!timing_on) return;
+        if (!timing_on || capturing) return;
         KTimer& k = ktm[fam];
         HIP_TRY(hipEventRecord(k.ev[k.used + 1], st));
         k.used += 2;
@@ -196,6 +197,7 @@ struct rls_ctx {
     }
 
     ~rls_ctx() {
+        release_graphs();
         DArr<zd>* zs[] = {&T, &TT, &V, &J1, &J2, &J5, &L, &H1, &H2, &S, &CS, &S2, &CS2, &Wt, &tri_F, &trsm_tmp, &trmm_part,
                           &part, &gram, &Gs, &Rm, &Rinv, &Racc, &Rtmp, &Wtot, &Wk, &Wneg, &U, &Hz,
                           &heevW, &Gev, &C1, &CzX, &Gc, &V2, &CW, &CG, &Wr, &Xr, &zdev,
@@ -362,20 +364,29 @@ struct rls_ctx {
     // operator applications (S = I, Schur factor)
     DArr<unsigned int> tri_ctr, tri_flags;
     DArr<int> tri_err;
-    unsigned int tri_epoch = 0;
-    void tri_solve(zd* X, int k, bool upper) {
+    unsigned int tri_epoch = 0;   // solves launched since the counter and flags were cleared
+    // The persistent solve derives its block and epoch from a 32-bit ticket counter; before
+    // `need` more solves could wrap it, counter and flags are cleared (on the stream, outside
+    // any captured graph).
+    void tri_solve_reserve(unsigned int need) {
         const int nblk = cdiv(n, 64);
-        if (tri_flags.n < (size_t)nblk + 1 || !tri_ctr.p) {
+        const uint64_t max_solves = 0xFFFFFFFFull / (uint64_t)nblk - 1;
+        const bool fresh = tri_flags.n < (size_t)nblk + 1 || !tri_ctr.p;
+        if (fresh) {
             check_tri_err();   // a pending wait error is reported, not cleared by the re-allocation
             tri_flags.ensure((size_t)nblk + 1);
             tri_ctr.ensure(1);
             tri_err.ensure(1);
+            HIP_TRY(hipMemsetAsync(tri_err.p, 0, sizeof(int), st));
+        }
+        if (fresh || (uint64_t)tri_epoch + need > max_solves) {
             HIP_TRY(hipMemsetAsync(tri_flags.p, 0, sizeof(unsigned int) * tri_flags.n, st));
             HIP_TRY(hipMemsetAsync(tri_ctr.p, 0, sizeof(unsigned int), st));
-            HIP_TRY(hipMemsetAsync(tri_err.p, 0, sizeof(int), st));
             tri_epoch = 0;
-            tri_ticket_base_ok = true;
         }
+    }
+    void tri_solve(zd* X, int k, bool upper) {
+        if (!capturing) tri_solve_reserve((unsigned)((k + 7) / 8));
         for (int c0 = 0; c0 < k; c0 += 8) {
             int kc = std::min(8, k - c0);
             zd* Xc = X + (int64_t)n * c0;
@@ -384,14 +395,13 @@ struct rls_ctx {
             // Hessenberg operator: (H - zI)^{-1} = U^{-1} G, (H - zI)^{-H} = G^H U^{-H}
             if (hess && upper) launch_hlu_apply_g(st, n, hlu_l.p, hlu_piv.p, Xc, n, kc, false);
             launch_trsm_persistent(st, n, TT.p, tri_F.p, Xc, n, kc, upper, tri_ctr.p,
-                                   tri_flags.p, tri_epoch, tri_err.p);
+                                   tri_flags.p, tri_err.p);
             if (hess && !upper) launch_hlu_apply_g(st, n, hlu_l.p, hlu_piv.p, Xc, n, kc, true);
             // algorithmic: the triangle of T once + the RHS block read and written
             kt_end(RLS_KF_TRSM, 8.0 * 0.5 * (double)n * n * kc,
                    16.0 * (0.5 * (double)n * n + 2.0 * (double)n * kc));
         }
     }
-    bool tri_ticket_base_ok = false;
     // Reports (once) a bounded dependency wait that gave up: the flag is cleared after it has
     // been read, so a later solve on the context starts clean.
     void check_tri_err() {
@@ -591,19 +601,125 @@ struct rls_ctx {
         apply_C(CS.p + (int64_t)n * r, r);
         int it = c.maxit;
         for (int i = 1; i <= c.maxit; ++i) {
-            if (i == 1) svd_rr_update(2 * r, side_L, 2 * r);
-            else svd_rr_update(3 * r, side_L, 2 * r);
-            residual_into(Wb(), side_L);
-            double res = colnorm(Wb(), n);
-            if (res < tol) { it = i; break; }
-            orth(S.p, n, 2 * r, Wb(), n, r);
-            launch_copy2d(st, n, r, Wb(), n, CS.p + (int64_t)n * 2 * r, n);
-            apply_C(CS.p + (int64_t)n * 2 * r, r);
+            if (i == 1) {
+                svd_rr_update(2 * r, side_L, 2 * r);
+                residual_into(Wb(), side_L);
+                nrm2.ensure(1);
+                part.ensure(gram_partial_elems(n, 1, 1) + 8);
+                launch_colnorm2(st, n, Wb(), n, 1, nrm2.p, part.p);
+            } else {
+                lobpcg_segment_run(side_L);
+            }
+            double res2 = 0;
+            d2h(&res2, nrm2.p, 1);
+            if (std::sqrt(res2) < tol) { it = i; break; }
         }
         double s0 = 0;
         d2h(&s0, sig.p, 1);
         check_tri_err();
         return {s0, it};
+    }
+
+    // The launches between two convergence tests of LOBPCG (iteration i >= 2): the end of
+    // iteration i - 1 (W orthogonalised against [X P], CW = C W) and iteration i up to its
+    // residual norm (svdRaylRitz! on [X P W], residual, ||W[:, 0]||^2 into nrm2).  The
+    // sequence has no host decision, so after two eager runs it is captured into a graph and
+    // replayed: one launch instead of ~45 small ones.  svd_rr_update swaps the S / CS double
+    // buffers, so two graphs alternate; each is keyed by every device pointer and size it
+    // captured and recaptured when one changes.  Bitwise the same as the eager sequence.
+    void lobpcg_segment(bool side_L) {
+        zd* W = S.p + (int64_t)n * 2 * r;
+        orth(S.p, n, 2 * r, W, n, r);
+        launch_copy2d(st, n, r, W, n, CS.p + (int64_t)n * 2 * r, n);
+        apply_C(CS.p + (int64_t)n * 2 * r, r);
+        svd_rr_update(3 * r, side_L, 2 * r);
+        W = S.p + (int64_t)n * 2 * r;
+        residual_into(W, side_L);
+        launch_colnorm2(st, n, W, n, 1, nrm2.p, part.p);
+    }
+    struct IterGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<uintptr_t> key;
+        uint64_t last_use = 0;
+    };
+    IterGraph igraph[2];
+    int64_t lobpcg_eager_segments = 0;
+    uint64_t graph_clock = 0;
+    std::vector<uintptr_t> lobpcg_graph_key(bool side_L) const {
+        auto u = [](const void* q) { return reinterpret_cast<uintptr_t>(q); };
+        return {u(S.p), u(S2.p), u(CS.p), u(CS2.p), u(gram.p), u(Gs.p), u(sig.p), u(lam.p), u(info.p),
+                u(Wk.p), u(Wneg.p), u(Rm.p), u(Rinv.p), u(Racc.p), u(Rtmp.p), u(offI.p), u(part.p), u(nrm2.p),
+                u(TT.p), u(tri_F.p), u(tri_ctr.p), u(tri_flags.p), u(tri_err.p), u(hlu_l.p), u(hlu_piv.p),
+                u(st), (uintptr_t)n, (uintptr_t)r, (uintptr_t)hess, (uintptr_t)side_L};
+    }
+    void release_graphs() {
+        for (auto& g : igraph) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            g.key.clear();
+        }
+    }
+    void lobpcg_segment_run(bool side_L) {
+        const unsigned solves = 2u * (unsigned)((r + 7) / 8);   // apply_C and apply_CH
+        if (sparse || !tune_lobpcg_graph || lobpcg_eager_segments < 2) {
+            lobpcg_segment(side_L);
+            ++lobpcg_eager_segments;
+            return;
+        }
+        tri_solve_reserve(solves);
+        const std::vector<uintptr_t> key = lobpcg_graph_key(side_L);
+        IterGraph* g = nullptr;
+        for (auto& e : igraph)
+            if (e.exec && e.key == key) g = &e;
+        if (!g) {
+            g = igraph[0].last_use <= igraph[1].last_use ? &igraph[0] : &igraph[1];
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            g->exec = nullptr;
+            g->key.clear();
+            // record the segment; nothing runs while capturing, and the host-side state it
+            // changes (the swapped buffers, the solve count) is what the replay leaves
+            zd* sv_ptr[4] = {S.p, S2.p, CS.p, CS2.p};
+            const unsigned ep0 = tri_epoch;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            capturing = true;
+            hipGraph_t gr = nullptr;
+            bool ok = true;
+            try {
+                lobpcg_segment(side_L);
+            } catch (...) {
+                ok = false;
+            }
+            capturing = false;
+            const hipError_t ee = hipStreamEndCapture(st, &gr);
+            hipGraphExec_t ex = nullptr;
+            if (ok && ee == hipSuccess && gr) ok = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess;
+            else ok = false;
+            if (gr) (void)hipGraphDestroy(gr);
+            if (!ok) {
+                // a capture that failed leaves no trace: restore the host state, run eagerly and
+                // stop capturing on this context
+                (void)hipGetLastError();
+                S.p = sv_ptr[0]; S2.p = sv_ptr[1]; CS.p = sv_ptr[2]; CS2.p = sv_ptr[3];
+                tri_epoch = ep0;
+                tune_lobpcg_graph = 0;
+                lobpcg_segment(side_L);
+                return;
+            }
+            // the swaps were done during capture: undo them so the key matches the replays
+            S.p = sv_ptr[0]; S2.p = sv_ptr[1]; CS.p = sv_ptr[2]; CS2.p = sv_ptr[3];
+            tri_epoch = ep0;
+            g->exec = ex;
+            g->key = key;
+        }
+        g->last_use = ++graph_clock;
+        kt_begin(RLS_KF_ITER);
+        HIP_TRY(hipGraphLaunch(g->exec, st));
+        // the algorithmic figures of the two shifted solves (the segment's dominant traffic)
+        kt_end(RLS_KF_ITER, solves * 8.0 * 0.5 * (double)n * n * std::min(r, 8),
+               solves * 16.0 * (0.5 * (double)n * n + 2.0 * (double)n * std::min(r, 8)));
+        tri_epoch += solves;
+        std::swap(S.p, S2.p);
+        std::swap(CS.p, CS2.p);
     }
 
     // ---- recycle space maintenance ----

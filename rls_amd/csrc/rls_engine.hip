@@ -535,6 +535,12 @@ int rls_set_tuning(rls_ctx* ctx, int key, int value) {
             if (value < 16 || value > 1024) throw RlsError{RLS_E_ARG, "rls_set_tuning: window must be in 16 .. 1024"};
             ctx->tune_prec_window = value;
             return RLS_OK;
+        case RLS_TUNE_LOBPCG_GRAPH:
+            if (value != 0 && value != 1) throw RlsError{RLS_E_ARG, "rls_set_tuning: LOBPCG graph must be 0 or 1"};
+            ctx->sync();
+            ctx->tune_lobpcg_graph = value;
+            if (!value) ctx->release_graphs();
+            return RLS_OK;
         default:
             throw RlsError{RLS_E_ARG, "rls_set_tuning: unknown key"};
         }
@@ -548,6 +554,7 @@ int rls_get_tuning(rls_ctx* ctx, int key, int* value) {
         case RLS_TUNE_SPTRSV_CW_SMALL: *value = ctx->tune_cw_small; return RLS_OK;
         case RLS_TUNE_SPTRSV_CW_LARGE: *value = ctx->tune_cw_large; return RLS_OK;
         case RLS_TUNE_PREC_WINDOW: *value = ctx->tune_prec_window; return RLS_OK;
+        case RLS_TUNE_LOBPCG_GRAPH: *value = ctx->tune_lobpcg_graph; return RLS_OK;
         default: throw RlsError{RLS_E_ARG, "rls_get_tuning: unknown key"};
         }
     });

@@ -125,7 +125,7 @@ void launch_tile_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd* 
 // launch_tile_upper and the block factors F of launch_tri_factors; counter/flags are device
 // words owned by the caller (flags: one per 64-row block), epoch increases per call.
 void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* TT, const zd* F, zd* X, int64_t ldx, int k,
-                            bool upper, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err);
+                            bool upper, unsigned int* counter, unsigned int* flags, int* err);
 // Tiled FP64 complex GEMM Y = [A_q] * B for wide B (batched recycle products).
 void launch_zgemm_tiled(hipStream_t st, int64_t n, const TallOperands& ops, const zd* B, int64_t ldb,
                         int N, zd* Y, int64_t ldy);

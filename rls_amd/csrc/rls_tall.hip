@@ -553,10 +553,14 @@ void launch_gram_self_batched(hipStream_t st, int64_t n, const zd* A, int64_t ld
 // of T in 64 x 64 tiles (tile (bi, bj), bi <= bj, at tri_tile_index(bi, bj) * 4096, each
 // tile column-major and zero-padded past n): every block load is one contiguous 64 KB
 // tile, and both directions read the same tiles.  Block j writes its solution through the L2
-// (agent-scope atomic stores) and publishes it by a release store of `epoch` into flag[j];
-// readers spin (bounded) on an agent-scope atomic load, then take an acquire fence.  The counter is
-// never reset: tickets of solve e are [e*nblk, (e+1)*nblk) because launches on one stream
-// run in order.  A spin that exceeds its bound sets *err and the kernel still terminates.
+// (agent-scope atomic stores) and publishes it by a release store of its solve's epoch into
+// flag[j]; readers spin (bounded) on an agent-scope atomic load, then take an acquire fence.
+// Launches on one stream run in order and every launch takes exactly nblk tickets, so the
+// tickets of solve e (counted from 0 since the counter and the flags were cleared) are
+// [e*nblk, (e+1)*nblk): the block is ticket mod nblk and the epoch ticket / nblk + 1, both
+// derived on the device, so a captured launch can be replayed.  The host clears counter and
+// flags before the ticket count could wrap.  A spin that exceeds its bound sets *err and the
+// kernel still terminates.
 // ---------------------------------------------------------------------------------------
 namespace {
 constexpr int PT_KC = 8;
@@ -592,7 +596,7 @@ __device__ __forceinline__ int64_t tri_tile_index(int bi, int bj) {   // bi <= b
 __global__ void __launch_bounds__(PT_THREADS)
 k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ Dinv,
                   const zd* __restrict__ Ech, zd* X, int64_t ldx, int k, int upper, unsigned int* counter,
-                  unsigned int* flags, unsigned int epoch, int* err) {
+                  unsigned int* flags, int* err) {
     constexpr int PER = 4096 / PT_THREADS;      // T / Dinv block elements per thread
     __shared__ zd Ts[64][65];
     __shared__ zd Ds[64][65];
@@ -609,6 +613,7 @@ k_trsm_persistent(int64_t n, const zd* __restrict__ TT, const zd* __restrict__ D
     }
     __syncthreads();
     const int t = (int)(s_ticket % (unsigned)nblk);
+    const unsigned int epoch = s_ticket / (unsigned)nblk + 1u;
     const int j = upper ? nblk - 1 - t : t;
     const int64_t o = (int64_t)j * 64;
     const int m = (int)min((int64_t)64, n - o);
@@ -853,12 +858,12 @@ void launch_tile_upper(hipStream_t st, int64_t n, const zd* T, int64_t ldt, zd* 
 }
 
 void launch_trsm_persistent(hipStream_t st, int64_t n, const zd* TT, const zd* F, zd* X, int64_t ldx, int k,
-                            bool upper, unsigned int* counter, unsigned int* flags, unsigned int epoch, int* err) {
+                            bool upper, unsigned int* counter, unsigned int* flags, int* err) {
     if (k > PT_KC) throw RlsError{-1, "trsm: k > 8 unsupported"};
     const int nblk = cdiv(n, 64);
     const zd* Ech = F + (size_t)(upper ? 1 : 2) * nblk * 4096;
     hipLaunchKernelGGL(k_trsm_persistent, dim3(nblk), dim3(PT_THREADS), 0, st, n, TT, F, Ech, X, ldx, k,
-                       upper ? 1 : 0, counter, flags, epoch, err);
+                       upper ? 1 : 0, counter, flags, err);
     LAUNCH_CHECK();
 }
 

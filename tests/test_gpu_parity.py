@@ -287,3 +287,35 @@ def test_window_tuner_keeps_the_trajectory():
     assert res["window"] in (16, 64, 256)
     assert res["decisions_differ"] == 0
     assert res["max_rel_diff"] <= 1e-10
+
+
+def test_lobpcg_graph_replays_match_eager():
+    """LOBPCG iterations replayed from captured HIP graphs give bitwise the eager launches'
+    results: Toeplitz n = 400 on the Toeplitz workload's window, first 120 points (refines of
+    ~100 iterations each, so every graph is replayed many times with eager work between the
+    replays).  Solved with graphs (captured, then again from the cached graphs) and without."""
+    from rls_amd import _capi
+    n = 400
+    C = WL.toeplitz_real(n).astype(np.complex128)
+    T = R.schur_preprocess(C, 0)
+    rec = R.recursion_grid(*WL.GRIDS["toeplitz"])
+    rec.Z = rec.Z[:120]
+    normC = float(sla.svdvals(C)[0])
+    ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched")
+    _say("toeplitz n=400, 120 points: LOBPCG graphs on (twice) and off")
+    solver = R.TriSolver(T)
+    assert solver.ctx.get_tuning(_capi.RLS_TUNE_LOBPCG_GRAPH) == 1
+    solver.enable_kernel_timing(True)
+    sig_g, st_g = solver.solve(rec.Z, ctl, rec)
+    kt = solver.kernel_timing()
+    solver.enable_kernel_timing(False)
+    sig_g2, st_g2 = solver.solve(rec.Z, ctl, rec)
+    solver.ctx.set_tuning(_capi.RLS_TUNE_LOBPCG_GRAPH, 0)
+    sig_e, st_e = solver.solve(rec.Z, ctl, rec)
+    _say(f"refined {int((st_e.lobpcg_it > 0).sum())}, LOBPCG iterations {int(st_e.lobpcg_it.sum())}, "
+         f"graph replays {kt['iter']['launches']}")
+    assert kt["iter"]["launches"] > 100
+    assert np.array_equal(st_g.lobpcg_it, st_e.lobpcg_it) and np.array_equal(st_g2.lobpcg_it, st_e.lobpcg_it)
+    assert np.array_equal(sig_g, sig_e) and np.array_equal(sig_g2, sig_e)
+    with pytest.raises(RuntimeError):
+        solver.ctx.set_tuning(_capi.RLS_TUNE_LOBPCG_GRAPH, 2)
