@@ -856,12 +856,17 @@ __device__ void lds_jacobi_svd(zd* A, zd* V, double* sig, int* perm, int k, int*
         // (wave-level fences order its LDS traffic); the other waves wait at the end.
         if (tid < 64) {
             const int lane = tid & 3, pb = tid >> 2;
+            const int km1 = kk - 1;
             for (int sweep = 0; sweep < 40; ++sweep) {
                 int rotated = 0;
-                for (int step = 0; step < kk - 1; ++step) {
-                    int a = (pb == 0) ? kk - 1 : (step + pb) % (kk - 1);
-                    int b = (step + kk - 1 - pb) % (kk - 1);
-                    if (pb == 0) b = step % (kk - 1);
+                // circle-method players of this lane's pair, advanced by one each step
+                // (step + pb and step - pb modulo kk - 1, without a division per step)
+                int ra = pb % km1, rb = (km1 - pb % km1) % km1;
+                for (int step = 0; step < km1; ++step) {
+                    const int a = (pb == 0) ? km1 : ra;
+                    const int b = (pb == 0) ? step : rb;
+                    if (++ra == km1) ra = 0;
+                    if (++rb == km1) rb = 0;
                     const int p = a < b ? a : b, q = a < b ? b : a;
                     const bool live = pb < npair && q < k;
                     double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
