@@ -313,13 +313,12 @@ __global__ void k_copy_cols(int64_t n, const zd* __restrict__ S, int64_t lds, zd
 
 // Column side of the block unitary rotations, one thread per row of one array.  The block
 // width R is a template parameter so the carried row blocks stay in registers; the next
-// block's loads are issued before the current rotation.  One wave per workgroup: n / 64
-// workgroups per array spread the sequential rotation chains over every CU (256-thread
-// workgroups left most CUs idle at n = 5000).
+// block's loads are issued before the current rotation.  (One-wave workgroups, n / 64 per
+// array, were slower at n = 5000: 158 against 138 us per launch.)
 constexpr int ROT_RMAX = 16;
 struct RotArrays { zd* a[8]; };
 template <int R>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int p, const zd* __restrict__ U) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
@@ -361,7 +360,7 @@ k_block_rot_cols(int64_t n, RotArrays arr, int64_t ld, int p, const zd* __restri
 
 template <int R>
 void launch_rot_cols_r(hipStream_t st, int64_t n, const RotArrays& ra, int narr, int64_t ld, int p, const zd* U) {
-    hipLaunchKernelGGL(k_block_rot_cols<R>, dim3(cdiv(n, 64), narr), dim3(64), 0, st, n, ra, ld, p, U);
+    hipLaunchKernelGGL(k_block_rot_cols<R>, dim3(cdiv(n, 256), narr), dim3(256), 0, st, n, ra, ld, p, U);
 }
 
 }  // namespace
