@@ -569,15 +569,6 @@ struct rls_ctx {
         launch_scale_sub(st, n, W, n, S.p, n, r, lam.p, nullptr, side_L ? 1 : 2);
     }
 
-    double colnorm(const zd* A, int64_t lda) {
-        nrm2.ensure(1);
-        part.ensure(gram_partial_elems(n, 1, 1) + 8);
-        launch_colnorm2(st, n, A, lda, 1, nrm2.p, part.p);
-        double h = 0;
-        d2h(&h, nrm2.p, 1);
-        return std::sqrt(h);
-    }
-
     // lobpcgSvd on C = (T - zI)^{-H}, side 'L', K = I, starting from S[:, 0:r].
     // Returns (sigma_min, iterations).
     std::pair<double, int> lobpcg(cplx z, const rls_tri_control& c) {
