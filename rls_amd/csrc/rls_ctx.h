@@ -31,6 +31,7 @@ struct DArr {
         if (cnt <= n) return;
         if (p) HIP_TRY(hipFree(p));
         p = nullptr;
+        n = 0;   // a failed allocation below leaves an empty array, not a stale size
         HIP_TRY(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(T)));
         n = cnt;
     }
