@@ -289,21 +289,23 @@ def test_window_tuner_keeps_the_trajectory():
     assert res["max_rel_diff"] <= 1e-10
 
 
-def test_lobpcg_graph_replays_match_eager():
+@pytest.mark.parametrize("operator", ["schur", "hessenberg"])
+def test_lobpcg_graph_replays_match_eager(operator):
     """LOBPCG iterations replayed from captured HIP graphs give bitwise the eager launches'
     results: Toeplitz n = 400 on the Toeplitz workload's window, first 120 points (refines of
     ~100 iterations each, so every graph is replayed many times with eager work between the
-    replays).  Solved with graphs (captured, then again from the cached graphs) and without."""
+    replays), on the Schur factor and on the Hessenberg operator (whose solves apply the
+    Hessenberg LU's row operations around the triangular solve).  Solved with graphs
+    (captured, then again from the cached graphs) and without."""
     from rls_amd import _capi
     n = 400
     C = WL.toeplitz_real(n).astype(np.complex128)
-    T = R.schur_preprocess(C, 0)
     rec = R.recursion_grid(*WL.GRIDS["toeplitz"])
     rec.Z = rec.Z[:120]
     normC = float(sla.svdvals(C)[0])
     ctl = R.PsTriRecyControl(tol=1e-4, normC=normC, p=60, recy_tau=1.6, mode="batched")
-    _say("toeplitz n=400, 120 points: LOBPCG graphs on (twice) and off")
-    solver = R.TriSolver(T)
+    _say(f"toeplitz n=400, 120 points, {operator} operator: LOBPCG graphs on (twice) and off")
+    solver = R.TriSolver(R.schur_preprocess(C, 0)) if operator == "schur" else R.TriSolver.from_dense(C)
     assert solver.ctx.get_tuning(_capi.RLS_TUNE_LOBPCG_GRAPH) == 1
     solver.enable_kernel_timing(True)
     sig_g, st_g = solver.solve(rec.Z, ctl, rec)
