@@ -657,6 +657,8 @@ def bench_prec(args):
         main_implict_update=False, main_implict_recycle=True, r_pred=40, p_pred=20, p_min_pred=1,
         pred_implict_update=True, pred_implict_recycle=True, hsl_tol=1e-3, hsl_cntl_1=1e-6,
         hsl_icntl_1=200, hsl_icntl_2=500, hsl_cntl_2=1e-4, gamma_tol=1e-3, lambda_n_tol=1e-3)
+    # the C++ driver prints a progress line every ~5 % of the traversal
+    os.environ.setdefault("RLS_PROGRESS", "1")
     t0 = time.perf_counter()
     perm = RP.column_ordering(A)
     solver = RP.PrecSolver(A)

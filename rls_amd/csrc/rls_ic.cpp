@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -403,6 +405,8 @@ int rls_ic_build(const rls_csc_lower* N, const int64_t* p, double tau1, double t
         }
         int64_t flag = 0;
         int nshift = 0;
+        const char* prog = std::getenv("RLS_PROGRESS");
+        const bool progress = prog && prog[0] && prog[0] != '0';
         Trial tr;
         std::vector<double> d(n + 1);
         while (true) {
@@ -411,6 +415,11 @@ int rls_ic_build(const rls_csc_lower* N, const int64_t* p, double tau1, double t
             flag = ictkl3(Ap, lsize, rsize, scale, tau1, tau2, d, tr);
             if (flag < 0) {
                 if (nshift > 200) fail("rls_ic_build: no stable shift found");
+                if (progress) {
+                    std::fprintf(stderr, "[rls_ic_build] breakdown at %lld, shift %.3g -> retry %d\n",
+                                 (long long)flag, a, nshift + 1);
+                    std::fflush(stderr);
+                }
                 a = std::max(kShiftFactor * a, lowalpha);
                 if (flag_previous != 0) {
                     const bool close = std::llabs(flag - flag_previous) <= n / 10;

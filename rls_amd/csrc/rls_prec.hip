@@ -16,6 +16,8 @@
 // control flow (singular values, residual norms, Ritz dimensions).
 #include "rls_ctx.h"
 
+#include <chrono>
+#include <cstdlib>
 #include <functional>
 
 namespace {
@@ -257,7 +259,23 @@ struct Driver {
     DBuf<double> lamd, sigd, nrmd, taud;
     DBuf<int> rrd, infod;
 
-    Driver(rls_ctx* c_, const rls_prec_control& k) : c(c_), ctl(k), n(c_->n), st(c_->st) {}
+    // progress lines on stderr (RLS_PROGRESS set): a long stage reports itself every ~10 s
+    bool progress = false;
+    std::chrono::steady_clock::time_point t_note = std::chrono::steady_clock::now();
+
+    Driver(rls_ctx* c_, const rls_prec_control& k) : c(c_), ctl(k), n(c_->n), st(c_->st) {
+        const char* e = std::getenv("RLS_PROGRESS");
+        progress = e && e[0] && e[0] != '0';
+    }
+
+    void note(const char* what, long long a, double v, bool force) {
+        if (!progress) return;
+        const auto now = std::chrono::steady_clock::now();
+        if (!force && std::chrono::duration<double>(now - t_note).count() < 10.0) return;
+        t_note = now;
+        std::fprintf(stderr, "[psPrecRecy]   %s %lld (%.6g)\n", what, a, v);
+        std::fflush(stderr);
+    }
 
     // ---- primitives ----
     void apply(int op, cplx z, const zd* X, int k, zd* Y) { c->dev_apply(op, z, X, n, k, Y, n); }
@@ -777,6 +795,7 @@ struct Driver {
             }
             c->orth(LS.p, n, 2 * r, W, n, r);
             A.mul(W, r, CW);
+            note("lobpcg iteration", i, sg[0], false);
         }
         LobpcgOut o;
         o.it = maxit;
@@ -958,6 +977,7 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
     // device upload, lambda_n = sigma_max((C - zI) invF)^2 and gamma = ||L^{-1}||
     auto build = [&](cplx z) {
         const uint64_t t0 = sync_now();
+        const uint64_t t_wall = now_ns();
         std::vector<int64_t> cp, ri;
         std::vector<cplx> vv;
         normal_lower(c, z, cp, ri, vv);
@@ -981,6 +1001,7 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
         lam_dag_n = sn.ret * sn.ret;
         gamma = g.ret;
         t_pc += sync_now() - t0;
+        d.note("preconditioner builds", n_builds, (double)(now_ns() - t_wall) * 1e-9, true);
     };
 
     Space mainS, predS;
@@ -1031,8 +1052,18 @@ int prec_solve(rls_ctx* c, const cplx* Z, int64_t nZ, const int64_t* perm_in, co
     const int window_max = c->tune_prec_window;
     double avg_run = 32.0;
     int run_hint = 32;
+    // progress lines on stderr every ~5 % of the traversal when RLS_PROGRESS is set
+    const bool progress = d.progress;
+    const auto t_start = std::chrono::steady_clock::now();
+    int64_t next_report = std::max<int64_t>(nZ / 20, 1);
     int64_t i = 0;
     while (i < nZ) {
+        if (progress && i >= next_report) {
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+            std::fprintf(stderr, "[psPrecRecy] %lld/%lld points, %.1f s\n", (long long)i, (long long)nZ, s);
+            std::fflush(stderr);
+            next_report += std::max<int64_t>(nZ / 20, 1);
+        }
         if (engine_main && c->p > 0) {
             const uint64_t t0 = sync_now();
             const int B = (int)std::min<int64_t>({(int64_t)window_max, (int64_t)std::max(run_hint, 1), nZ - i});
