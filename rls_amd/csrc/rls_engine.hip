@@ -932,8 +932,8 @@ int rls_sigma_grad_toeplitz(rls_ctx* ctx, int64_t first, int64_t count, double* 
 // --------------------------------------------------------------------------------------
 extern "C" {
 
-// variant: 0 = the size-based choice of the library, 1 = the full-square tridiagonalisation
-// pass, 2 = the lower-triangle tiled pass
+// variant: 0 = the size-based choice of the library (k_heev_wave for n <= 128, else
+// k_heev_rr), 1 = k_heev_rr for any n
 int rls_test_heev_variant(int n, const double* A, double tau, int r, int nvec, int mode_fixed,
                           int variant, double* lam, double* G, int* rr_out) {
     rls_ctx* ctx = nullptr;
