@@ -205,11 +205,33 @@ k_trmm_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __restr
 #pragma unroll
     for (int j = 0; j < KC; ++j) acc[j] = zzero();
     const bool valid = row < n;
-    for (int64_t col = cb + w; col < ce; col += 4) {
-        zd t = (valid && col >= row) ? T[row + ldt * col] : zzero();
+    // X columns past k read column k - 1 (in bounds, products unused), so that every load is
+    // issued without a branch; two columns of T per pass, all loads before the products
+    int xo[KC];
 #pragma unroll
-        for (int j = 0; j < KC; ++j)
-            if (j < k) acc[j] = zfma(t, X[col + ldx * j], acc[j]);
+    for (int j = 0; j < KC; ++j) xo[j] = (j < k ? j : k - 1);
+    int64_t col = cb + w;
+    for (; col + 4 < ce; col += 8) {
+        const zd t0 = (valid && col >= row) ? T[row + ldt * col] : zzero();
+        const zd t1 = (valid && col + 4 >= row) ? T[row + ldt * (col + 4)] : zzero();
+        zd x0[KC], x1[KC];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            x0[j] = X[col + ldx * xo[j]];
+            x1[j] = X[col + 4 + ldx * xo[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zfma(t0, x0[j], acc[j]);
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zfma(t1, x1[j], acc[j]);
+    }
+    for (; col < ce; col += 4) {
+        const zd t = (valid && col >= row) ? T[row + ldt * col] : zzero();
+        zd x0[KC];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) x0[j] = X[col + ldx * xo[j]];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zfma(t, x0[j], acc[j]);
     }
     __shared__ zd red[3][64][KC];
     if (w > 0) {
@@ -251,11 +273,32 @@ k_trmmh_upper(int64_t n, const zd* __restrict__ T, int64_t ldt, const zd* __rest
 #pragma unroll
     for (int j = 0; j < KC; ++j) acc[j] = zzero();
     const zd* Tc = T + ldt * i;
-    for (int64_t r = lane; r <= i; r += 64) {
-        zd t = Tc[r];
+    // X columns past kc read column c0 + kc - 1 (in bounds, products unused), so that every
+    // load is issued without a branch; two rows per lane and pass, all loads before the products
+    const zd* Xc[KC];
 #pragma unroll
-        for (int j = 0; j < KC; ++j)
-            if (j < kc) acc[j] = zcfma(t, X[r + ldx * (c0 + j)], acc[j]);
+    for (int j = 0; j < KC; ++j) Xc[j] = X + ldx * (c0 + (j < kc ? j : kc - 1));
+    int64_t r = lane;
+    for (; r + 64 <= i; r += 128) {
+        const zd t0 = Tc[r], t1 = Tc[r + 64];
+        zd x0[KC], x1[KC];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            x0[j] = Xc[j][r];
+            x1[j] = Xc[j][r + 64];
+        }
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zcfma(t0, x0[j], acc[j]);
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zcfma(t1, x1[j], acc[j]);
+    }
+    for (; r <= i; r += 64) {
+        const zd t = Tc[r];
+        zd x0[KC];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) x0[j] = Xc[j][r];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = zcfma(t, x0[j], acc[j]);
     }
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
