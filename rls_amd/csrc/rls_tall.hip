@@ -35,15 +35,20 @@ k_tall_gemm(int64_t n, TallOperands ops, const zd* __restrict__ B, int64_t ldb, 
         const int64_t lda = ops.lda[q];
         const int Kq = ops.K[q];
         const bool second = (q == q2) && (Y2 != nullptr);
+        // B columns past kc read column kc - 1 (in bounds, products unused): every load is
+        // issued without a branch, the A entry and the row of B before the products
         for (int kk = w; kk < Kq; kk += TG_WAVES) {
-            zd a = valid ? A[row + lda * kk] : zzero();
+            const zd a = valid ? A[row + lda * kk] : zzero();
+            zd bv[KC];
 #pragma unroll
-            for (int j = 0; j < KC; ++j)
-                if (j < kc) acc[j] = zfma(a, Bb[off + kk + ldb * j], acc[j]);
+            for (int j = 0; j < KC; ++j) bv[j] = Bb[off + kk + ldb * (j < kc ? j : kc - 1)];
+#pragma unroll
+            for (int j = 0; j < KC; ++j) acc[j] = zfma(a, bv[j], acc[j]);
             if (second) {
 #pragma unroll
-                for (int j = 0; j < KC; ++j)
-                    if (j < kc) acc2[j] = zfma(a, B2b[kk + ldb2 * j], acc2[j]);
+                for (int j = 0; j < KC; ++j) bv[j] = B2b[kk + ldb2 * (j < kc ? j : kc - 1)];
+#pragma unroll
+                for (int j = 0; j < KC; ++j) acc2[j] = zfma(a, bv[j], acc2[j]);
             }
         }
         off += Kq;
